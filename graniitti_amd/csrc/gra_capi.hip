@@ -1,0 +1,805 @@
+// C ABI of the integrand and VEGAS integrator (declarations in include/gra_hip.h).
+//
+// Host responsibilities: run constants (every transcendental constant is taken
+// from the host libm so it equals the reference's value), the eikonal-table
+// pipeline around the GPU Fourier-Bessel kernel, buffer management, the VEGAS
+// iteration schedule and grid update (MGraniitti::VEGAS, src/MGraniitti.cc:1146-1338).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+
+#include "gra_internal.h"
+#include "gra_kernels.h"
+
+namespace gra_hip {
+
+static thread_local std::string g_err;
+
+static std::string vformat(const char* fmt, va_list ap) {
+  char buf[1024];
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  return std::string(buf);
+}
+
+int set_global_error(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  g_err = vformat(fmt, ap);
+  va_end(ap);
+  return code;
+}
+const char* global_error() { return g_err.c_str(); }
+
+int fail(gra_handle* h, int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  const std::string m = vformat(fmt, ap);
+  va_end(ap);
+  g_err = m;
+  if (h) h->err = m;
+  return code;
+}
+
+int DevBuf::ensure(size_t need) {
+  if (need <= bytes && p) return 0;
+  release();
+  if (need == 0) need = 16;
+  if (hipMalloc(&p, need) != hipSuccess) {
+    p = nullptr;
+    bytes = 0;
+    return -1;
+  }
+  bytes = need;
+  return 0;
+}
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Host restatement of the per-run scalar functions (same operations as the
+// reference so the constants are bit-identical to its values)
+// ---------------------------------------------------------------------------
+namespace {
+
+inline double msqrt(double x) { return std::sqrt(std::max(0.0, x)); }
+
+struct hcx {
+  double re, im;
+};
+inline hcx hexp(hcx a) {
+  const double e = std::exp(a.re);
+  return {e * std::cos(a.im), e * std::sin(a.im)};
+}
+inline hcx hmul(hcx a, hcx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+
+// form::ReggeEta, src/MForm.cc:382-385
+hcx regge_eta(double alpha_t, double sigma) {
+  const double denom = std::sin(GRA_PI * alpha_t);
+  const hcx e = hexp({-0.0 * GRA_PI * alpha_t, -GRA_PI * alpha_t});
+  const hcx num = {1.0 + sigma * e.re, sigma * e.im};
+  return {-num.re / denom, -num.im / denom};
+}
+
+double s3f(const gra_cfg& c, double t) { return (1.0 / (1 - t / c.fc1)) * (1.0 / (1 - t / c.fc2)); }
+
+// form::S3HPL and form::S3PomAlpha, src/MForm.cc:402-426
+double s3hpl(const gra_cfg& c, double tau, double t) {
+  const double m = 1.0;
+  const double sqrtau = msqrt(1 + tau);
+  const double F_pi = 1.0 / (1.0 - t / c.fc3);
+  const double mpi = c.mpi_const;
+  return (4.0 / tau) * (F_pi * F_pi) *
+         (2.0 * tau - std::pow(1.0 + tau, 3.0 / 2.0) * std::log((sqrtau + 1.0) / (sqrtau - 1.0)) +
+          std::log((m * m) / (mpi * mpi)));
+}
+double pom_alpha(const gra_cfg& c, double t) {
+  const double mpi = c.mpi_const;
+  const double BETApi = (2.0 / 3.0) * c.gN_P;
+  const double PIC = ((BETApi * BETApi) * (mpi * mpi)) / (32.0 * (GRA_PI * (GRA_PI * GRA_PI)));
+  const double h = PIC * s3hpl(c, 4.0 * (mpi * mpi) / std::fabs(t), t);
+  return 1.0 + c.delta_P + c.alpha_P * t - h;
+}
+
+// MEikonal::SingleAmpElastic(s, t, +1), src/MEikonal.cc:139-154
+hcx single_amp_elastic(const gra_cfg& c, double s, double t) {
+  const double F = s3f(c, t);
+  const double aP = pom_alpha(c, t);
+  const hcx eta = regge_eta(aP, 1);
+  const double pre = ((c.gN_P * c.gN_P) * F) * F;
+  const double pw = std::pow(s / 1.0, aP);
+  return {(eta.re * pre) * pw, (eta.im * pre) * pw};
+}
+
+// IArray1D::Interpolate1D on rows (x, re, im), src/MEikonal.cc:489-512
+hcx interp_rows(const std::vector<double>& T, int N, double MIN, double STEP, double a) {
+  if (a < MIN) a = MIN;
+  int i = (int)std::floor((a - MIN) / STEP);
+  if (i < 0) i = 0;
+  if (i >= N) i = N - 1;
+  const double x0 = T[3 * i], x1 = T[3 * (i + 1)];
+  const hcx y0 = {T[3 * i + 1], T[3 * i + 2]}, y1 = {T[3 * (i + 1) + 1], T[3 * (i + 1) + 2]};
+  const double dx = x1 - x0;
+  const hcx q = {(y1.re - y0.re) / dx, (y1.im - y0.im) / dx};
+  return {y0.re + q.re * (a - x0), y0.im + q.im * (a - x0)};
+}
+
+// IArray1D::WriteArray / ReadArray round trip (std::setprecision(15), std::stod)
+double round15(double v) {
+  char buf[64];
+  snprintf(buf, sizeof(buf), "%.15g", v);
+  return std::strtod(buf, nullptr);
+}
+
+double gev2barn() {
+  const double c = 2.99792458E8, hbar = 6.582119514E-25;
+  const double GeV2m = hbar * c;
+  const double GeV2m2 = GeV2m * GeV2m;
+  return GeV2m2 / 1E-28;
+}
+double gev2fm() {
+  const double c = 2.99792458E8, hbar = 6.582119514E-25;
+  return hbar * c * 1E15;
+}
+
+double simpson38_w(int i, int N) {
+  if (i == 0 || i == N) return 1.0;
+  return (i % 3 == 0) ? 2.0 : 3.0;
+}
+
+int validate(const gra_cfg* c) {
+  if (!c) return set_global_error(GRA_E_ARG, "configuration is NULL");
+  if (c->abi_size != (int32_t)sizeof(gra_cfg))
+    return set_global_error(GRA_E_ARG, "gra_cfg.abi_size = %d, library expects %d", c->abi_size,
+                            (int)sizeof(gra_cfg));
+  if (c->process != GRA_PROC_REGGE_CON && c->process != GRA_PROC_TENSOR_CON)
+    return set_global_error(GRA_E_ARG, "unknown process %d", c->process);
+  if (c->process == GRA_PROC_TENSOR_CON)
+    return set_global_error(GRA_E_ARG, "process PP[CONTENSOR] is not available in this build");
+  const int dmin = (c->m_daughter > 0.0) ? 14 : (c->w_central > 0.0 ? 10 : 8);
+  if (c->dim < dmin || c->dim > GRA_MAX_DIM)
+    return set_global_error(GRA_E_ARG, "dim %d: this process needs %d..%d", c->dim, dmin,
+                            GRA_MAX_DIM);
+  if (c->amp_precision != GRA_AMP_FP64 && c->amp_precision != GRA_AMP_FP32)
+    return set_global_error(GRA_E_ARG, "amp_precision %d unknown", c->amp_precision);
+  if (std::fabs(c->beam_E1 - c->beam_E2) > 0.0)
+    return set_global_error(GRA_E_ARG, "asymmetric beams are not supported (E1 != E2)");
+  if (!(c->fwd_pt_max > c->fwd_pt_min) || !(c->kt_max > c->kt_min) || !(c->rap_max > c->rap_min))
+    return set_global_error(GRA_E_ARG, "empty generation-cut interval");
+  if (c->S_factor <= 0.0) return set_global_error(GRA_E_ARG, "S_factor must be > 0");
+  if (c->screening) {
+    if (c->n_loop_phi < 3 || c->n_loop_phi % 3 || c->n_loop_kt < 3 || c->n_loop_kt % 3)
+      return set_global_error(GRA_E_ARG,
+                              "loop intervals must be multiples of 3 (Simpson 3/8): phi %d, kt %d",
+                              c->n_loop_phi, c->n_loop_kt);
+    if ((c->n_loop_phi + 1) * (c->n_loop_kt + 1) > 1024)
+      return set_global_error(GRA_E_ARG, "too many loop nodes (max 1024, LDS node table)");
+    if (c->n_fb < 2 || c->n_fb % 2)
+      return set_global_error(GRA_E_ARG, "n_fb %d must be even (composite Simpson)", c->n_fb);
+    if (c->n_bt < 2 || c->n_kt2 < 2) return set_global_error(GRA_E_ARG, "tables need >= 2 intervals");
+    if (!(c->max_loop_kt > c->min_loop_kt))
+      return set_global_error(GRA_E_ARG, "MaxLoopKT must exceed MinLoopKT");
+    if (c->max_loop_kt * c->max_loop_kt > 25.0 * (1 + 1e-5))
+      return set_global_error(GRA_E_ARG, "Interpolate1D: loop kt^2 outside the table [1e-6, 25]");
+  }
+  int on = 0;
+  for (int k = 0; k < 3; ++k) on += c->regge_on[k] ? 1 : 0;
+  if (c->amp_precision == GRA_AMP_FP32 && on != 1)
+    return set_global_error(GRA_E_ARG, "FP32 amplitude needs exactly one active Regge trajectory");
+  return GRA_OK;
+}
+
+void fill_params(const gra_cfg& c, RunParams& P) {
+  std::memset(&P, 0, sizeof(P));
+  P.process = c.process;
+  P.amp_mode = c.amp_precision;
+  P.dim = c.dim;
+  P.screening = c.screening;
+  const double E1 = c.beam_E1, E2 = c.beam_E2, m = c.m_beam;
+  P.pbeam1 = {E1, 0.0, 0.0, msqrt(E1 * E1 - m * m)};
+  P.pbeam2 = {E2, 0.0, 0.0, -msqrt(E2 * E2 - m * m)};
+  P.beamsum = {P.pbeam1.e + P.pbeam2.e, P.pbeam1.x + P.pbeam2.x, P.pbeam1.y + P.pbeam2.y,
+               P.pbeam1.z + P.pbeam2.z};
+  P.s = P.beamsum.e * P.beamsum.e -
+        (P.beamsum.x * P.beamsum.x + P.beamsum.y * P.beamsum.y + P.beamsum.z * P.beamsum.z);
+  P.sqrt_s = msqrt(P.s);
+  P.rs = std::sqrt(P.s);
+  P.s32 = std::pow(P.s, 3.0 / 2.0);
+  const double ZERO_EPS = 1e-12;
+  P.lmin = std::log(c.fwd_pt_min + ZERO_EPS);
+  P.lmax = std::log(c.fwd_pt_max);
+  P.kt_min = c.kt_min;
+  P.kt_max = c.kt_max;
+  P.rap_min = c.rap_min;
+  P.rap_max = c.rap_max;
+  P.m_beam = c.m_beam;
+  P.m_central = c.m_central;
+  P.w_central = c.w_central;
+  P.m_daughter = c.m_daughter;
+  if (c.w_central > 0.0) {
+    // MProcess::GetOffShellMass + MRandom::RelativisticBWRandom constants
+    const double m0 = c.m_central, G = c.w_central, L = c.offshell;
+    const double dm = (c.m_daughter > 0.0) ? (0.0 + c.m_daughter) + c.m_daughter + 1e-5 : 0.0;
+    const double m2 = m0 * m0;
+    const double m2max = (m0 + L * G) * (m0 + L * G);
+    const double m2min = std::max(std::max(0.0, dm * dm), (m0 - L * G) * (m0 - L * G));
+    P.bw_a1 = std::atan2(m2min - m2, m0 * G);
+    P.bw_a2 = std::atan2(m2max - m2, m0 * G);
+    P.bw_dmin = dm;
+  }
+  P.fid_active = c.fid_active;
+  P.fid_eta_min = c.fid_eta_min;
+  P.fid_eta_max = c.fid_eta_max;
+  P.fid_rap_min = c.fid_rap_min;
+  P.fid_rap_max = c.fid_rap_max;
+  P.fid_pt_min = c.fid_pt_min;
+  P.fid_pt_max = c.fid_pt_max;
+  P.fid_et_min = c.fid_et_min;
+  P.fid_et_max = c.fid_et_max;
+  P.sys_M_min = c.sys_M_min;
+  P.sys_M_max = c.sys_M_max;
+  P.sys_Y_min = c.sys_Y_min;
+  P.sys_Y_max = c.sys_Y_max;
+  P.sys_Pt_min = c.sys_Pt_min;
+  P.sys_Pt_max = c.sys_Pt_max;
+  P.fwd_t_min = c.fwd_t_min;
+  P.fwd_t_max = c.fwd_t_max;
+  // BNPhaseSpaceWeight / BNIntegralVolume / ForwardVolume constants (Nf = 4, Kf = 2)
+  P.psw_c1 = 1.0 / std::pow(2, 2 * (4 - 2));
+  P.psw_c2 = 1.0 / std::pow(2.0 * GRA_PI, 3 * 4 - 4);
+  const double lspan = std::log(c.fwd_pt_max) - std::log(c.fwd_pt_min + ZERO_EPS);
+  P.lspan2 = lspan * lspan;
+  double KT_vol = std::pow(c.kt_max - c.kt_min, 1);
+  KT_vol *= std::pow(2.0 * GRA_PI, 1);
+  KT_vol *= std::pow(c.rap_max - c.rap_min, 2);
+  P.KT_vol = KT_vol;
+  P.PHI_vol = (2.0 * GRA_PI) * (2.0 * GRA_PI);
+  P.invS = 1.0 / c.S_factor;
+  P.GeV2barn = gev2barn();
+  P.flux = 2.0 * P.s;
+  // Regge
+  int on = 0, kk = -1;
+  for (int k = 0; k < 3; ++k) {
+    P.regge_on[k] = c.regge_on[k];
+    P.regge_a0[k] = c.regge_a0[k];
+    P.regge_ap[k] = c.regge_ap[k];
+    const hcx e = regge_eta(c.regge_a0[k], c.regge_sgn[k]);
+    P.eta0_re[k] = e.re;
+    P.eta0_im[k] = e.im;
+    if (c.regge_on[k]) {
+      ++on;
+      kk = k;
+    }
+  }
+  P.regge_k = (on == 1) ? kk : -1;
+  P.regge_s0 = c.regge_s0;
+  P.gN_P = c.gN_P;
+  P.fc1 = c.fc1;
+  P.fc2 = c.fc2;
+  P.gpp_P = c.regge_c[0] / c.gN_P;
+  P.m_prop2 = c.m_prop * c.m_prop;
+  P.offshellFF = c.offshellFF;
+  P.b_EXP = c.b_EXP;
+  P.a_OREAR = c.a_OREAR;
+  P.b_OREAR = c.b_OREAR;
+  P.b_POW = c.b_POW;
+  if (kk >= 0) {
+    P.ph_c = (-GRA_PI / 2.0) * c.regge_ap[kk];
+    const double e2 = P.eta0_re[kk] * P.eta0_re[kk] + P.eta0_im[kk] * P.eta0_im[kk];
+    const double g2 = (c.gN_P * P.gpp_P) * (c.gN_P * P.gpp_P);
+    P.amp_c2 = (e2 * e2) * (g2 * g2);
+  }
+  // screening loop
+  P.nphi = c.n_loop_phi;
+  P.nkt = c.n_loop_kt;
+  P.step_kt = (c.max_loop_kt - c.min_loop_kt) / c.n_loop_kt;
+  P.step_phi = (2.0 * GRA_PI - 0.0) / c.n_loop_phi;
+  P.min_loop_kt = c.min_loop_kt;
+  P.hfac = 9 * (P.step_phi * P.step_kt) / 64;
+  P.nrm = 1.0 / (8.0 * (GRA_PI * GRA_PI) * P.s);
+  P.kappa = P.nrm * P.hfac;
+  // tensor pomeron
+  P.tp_delta_P = c.tp_delta_P;
+  P.tp_ap_P = c.tp_ap_P;
+  P.tp_gPNN = c.tp_gPNN;
+  P.tp_gPvv0 = c.tp_gPvv[0];
+  P.tp_gPvv1 = c.tp_gPvv[1];
+  P.tp_mp_ff = c.tp_mp_ff;
+  P.m_rho = c.m_central;
+}
+
+// MEikonal::S3Constructor tables on the GPU, src/MEikonal.cc:46-132
+int build_eikonal(gra_handle* h) {
+  const gra_cfg& c = h->cfg;
+  const RunParams& P = h->P;
+  const int NF = c.n_fb;
+  std::vector<double> pre(2 * (size_t)(NF + 1));
+  DevBuf d_pre, d_out;
+  const int nrow_max = std::max(c.n_bt, c.n_kt2) + 1;
+  if (d_pre.ensure(sizeof(double) * pre.size()) || d_out.ensure(sizeof(double) * 2 * nrow_max))
+    return fail(h, GRA_E_NOMEM, "eikonal table buffers");
+  std::vector<double> out(2 * (size_t)nrow_max);
+  // Density Omega(bt): pre_q = SingleAmpElastic(s, -kt_q^2)
+  const double kmin = 1E-9, kmax = 30.0, kstep = (kmax - kmin) / NF;
+  for (int q = 0; q <= NF; ++q) {
+    const double kt = kmin + q * kstep;
+    const hcx a = single_amp_elastic(c, P.s, -(kt * kt));
+    pre[2 * q] = a.re;
+    pre[2 * q + 1] = a.im;
+  }
+  const double bmin = 1E-6, bmax = 10.0 / gev2fm(), bstep = (bmax - bmin) / c.n_bt;
+  GRA_HIP_CHECK(h, hipMemcpyAsync(d_pre.p, pre.data(), sizeof(double) * pre.size(),
+                                  hipMemcpyHostToDevice, h->stream));
+  GRA_HIP_CHECK(h, launch_fb_table(0, d_pre.as<double>(), NF, kmin, kstep, bmin, bstep, c.n_bt + 1,
+                                   1 / (2.0 * GRA_PI * P.s), d_out.as<double>(), h->stream));
+  GRA_HIP_CHECK(h, hipMemcpyAsync(out.data(), d_out.p, sizeof(double) * 2 * (c.n_bt + 1),
+                                  hipMemcpyDeviceToHost, h->stream));
+  GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  h->mbt.resize(3 * (size_t)(c.n_bt + 1));
+  for (int i = 0; i <= c.n_bt; ++i) {
+    h->mbt[3 * i] = round15(bmin + i * bstep);
+    h->mbt[3 * i + 1] = round15(out[2 * i]);
+    h->mbt[3 * i + 2] = round15(out[2 * i + 1]);
+  }
+  // Screening amplitude A_eik(kt^2): pre_q = i (1 - exp(i Omega(bt_q) / 2))
+  const double STEP = (bmax - bmin) / NF;
+  for (int q = 0; q <= NF; ++q) {
+    const double bt = bmin + q * STEP;
+    const hcx Om = interp_rows(h->mbt, c.n_bt, bmin, bstep, bt);
+    const hcx z = {(0.0 * Om.re - 1.0 * Om.im) / 2.0, (0.0 * Om.im + 1.0 * Om.re) / 2.0};
+    const hcx e = hexp(z);
+    const hcx om = {1.0 - e.re, -e.im};
+    pre[2 * q] = 0.0 * om.re - 1.0 * om.im;
+    pre[2 * q + 1] = 0.0 * om.im + 1.0 * om.re;
+  }
+  const double amin = 1E-6, amax = 25.0, astep = (amax - amin) / c.n_kt2;
+  GRA_HIP_CHECK(h, hipMemcpyAsync(d_pre.p, pre.data(), sizeof(double) * pre.size(),
+                                  hipMemcpyHostToDevice, h->stream));
+  GRA_HIP_CHECK(h, launch_fb_table(1, d_pre.as<double>(), NF, bmin, STEP, amin, astep, c.n_kt2 + 1,
+                                   (2.0 * P.s) * (2.0 * GRA_PI), d_out.as<double>(), h->stream));
+  GRA_HIP_CHECK(h, hipMemcpyAsync(out.data(), d_out.p, sizeof(double) * 2 * (c.n_kt2 + 1),
+                                  hipMemcpyDeviceToHost, h->stream));
+  GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  h->msa.resize(3 * (size_t)(c.n_kt2 + 1));
+  for (int i = 0; i <= c.n_kt2; ++i) {
+    h->msa[3 * i] = round15(amin + i * astep);
+    h->msa[3 * i + 1] = round15(out[2 * i]);
+    h->msa[3 * i + 2] = round15(out[2 * i + 1]);
+  }
+  d_pre.release();
+  d_out.release();
+  // Loop nodes (phi_i, kt_j), i-major as the reference's (i, j) loops
+  const int NP = c.n_loop_phi, NK = c.n_loop_kt;
+  h->nodes.resize((size_t)(NP + 1) * (NK + 1));
+  for (int i = 0; i <= NP; ++i) {
+    const double phi = 0.0 + i * P.step_phi;
+    for (int j = 0; j <= NK; ++j) {
+      const double kt = c.min_loop_kt + j * P.step_kt;
+      const hcx A = interp_rows(h->msa, c.n_kt2, amin, astep, kt * kt);
+      NodeConst& nd = h->nodes[(size_t)i * (NK + 1) + j];
+      nd.kx = kt * std::cos(phi);
+      nd.ky = kt * std::sin(phi);
+      nd.kt = kt;
+      nd.w = simpson38_w(i, NP) * simpson38_w(j, NK);
+      nd.aeik_re = A.re;
+      nd.aeik_im = A.im;
+      nd.cf_re = (float)(nd.w * kt * A.re);
+      nd.cf_im = (float)(nd.w * kt * A.im);
+    }
+  }
+  if (h->d_nodes.ensure(sizeof(NodeConst) * h->nodes.size()))
+    return fail(h, GRA_E_NOMEM, "node table");
+  GRA_HIP_CHECK(h, hipMemcpy(h->d_nodes.p, h->nodes.data(), sizeof(NodeConst) * h->nodes.size(),
+                             hipMemcpyHostToDevice));
+  return GRA_OK;
+}
+
+}  // namespace
+
+int ensure_point_buffers(gra_handle* h, uint64_t n) {
+  const int D = h->P.dim;
+  const size_t recb = (h->P.amp_mode == GRA_AMP_FP32) ? k32::loop_record_bytes()
+                                                      : k64::loop_record_bytes();
+  if (h->d_W.ensure(sizeof(double) * n) || h->d_amp2.ensure(sizeof(double) * n) ||
+      h->d_flags.ensure(n) || h->d_vw.ensure(sizeof(double) * n) ||
+      h->d_bins.ensure((size_t)D * n) || h->d_ncand.ensure(16) ||
+      (h->P.screening && h->d_rec.ensure(recb * n)))
+    return fail(h, GRA_E_NOMEM, "device buffers for %llu points", (unsigned long long)n);
+  return GRA_OK;
+}
+
+static void time_begin(gra_handle* h, int slot) {
+  (void)hipEventRecord(h->ev[2 * slot], h->stream);
+}
+static void time_end(gra_handle* h, int slot) {
+  (void)hipEventRecord(h->ev[2 * slot + 1], h->stream);
+}
+// After a stream synchronization: add the elapsed time of the recorded slots
+static void time_collect(gra_handle* h, bool tree, bool screen, bool accum) {
+  const bool on[3] = {tree, screen, accum};
+  for (int s = 0; s < 3; ++s) {
+    if (!on[s]) continue;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, h->ev[2 * s], h->ev[2 * s + 1]) == hipSuccess) {
+      h->time_ms[s] += ms;
+      h->launches[s] += 1;
+    }
+  }
+}
+
+int run_integrand(gra_handle* h, const double* d_grid, unsigned int bins, uint64_t seed,
+                  uint32_t iter, uint64_t first, const double* d_x, uint64_t n, double* d_W,
+                  double* d_amp2, uint8_t* d_flags, double* d_vw, uint8_t* d_bins) {
+  const RunParams& P = h->P;
+  const bool f32 = P.amp_mode == GRA_AMP_FP32;
+  uint32_t* ncand = h->d_ncand.as<uint32_t>();
+  GRA_HIP_CHECK(h, hipMemsetAsync(ncand, 0, sizeof(uint32_t), h->stream));
+  time_begin(h, TS_TREE);
+  GRA_HIP_CHECK(h, f32 ? k32::launch_tree(P, d_grid, bins, seed, iter, first, d_x, n, d_W, d_amp2,
+                                          d_flags, d_vw, d_bins, h->d_rec.p, ncand, h->stream)
+                       : k64::launch_tree(P, d_grid, bins, seed, iter, first, d_x, n, d_W, d_amp2,
+                                          d_flags, d_vw, d_bins, h->d_rec.p, ncand, h->stream));
+  time_end(h, TS_TREE);
+  if (P.screening) {
+    time_begin(h, TS_SCREEN);
+    GRA_HIP_CHECK(h, f32 ? k32::launch_screen(P, h->d_nodes.p, h->d_rec.p, ncand, n, d_W, d_amp2,
+                                              d_flags, h->stream)
+                         : k64::launch_screen(P, h->d_nodes.p, h->d_rec.p, ncand, n, d_W, d_amp2,
+                                              d_flags, h->stream));
+    time_end(h, TS_SCREEN);
+  }
+  return GRA_OK;
+}
+
+}  // namespace gra_hip
+
+using namespace gra_hip;
+
+// ---------------------------------------------------------------------------
+// Entry points
+// ---------------------------------------------------------------------------
+extern "C" const char* gra_hip_last_error(const gra_handle* h) {
+  return h ? h->err.c_str() : global_error();
+}
+
+extern "C" int gra_hip_create(const gra_cfg* cfg, gra_handle** out) {
+  if (!out) return set_global_error(GRA_E_ARG, "gra_hip_create: out is NULL");
+  *out = nullptr;
+  const int v = validate(cfg);
+  if (v != GRA_OK) return v;
+  gra_handle* h = new (std::nothrow) gra_handle();
+  if (!h) return set_global_error(GRA_E_NOMEM, "gra_hip_create: host allocation");
+  h->cfg = *cfg;
+  fill_params(h->cfg, h->P);
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    delete h;
+    return set_global_error(GRA_E_HIP, "gra_hip_create: no HIP device available");
+  }
+  h->device = dev;
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return set_global_error(GRA_E_HIP, "gra_hip_create: stream creation failed");
+  }
+  for (int k = 0; k < 2 * TS_N; ++k) {
+    if (hipEventCreate(&h->ev[k]) != hipSuccess) {
+      gra_hip_destroy(h);
+      return set_global_error(GRA_E_HIP, "gra_hip_create: event creation failed");
+    }
+  }
+  if (h->cfg.screening) {
+    const int r = build_eikonal(h);
+    if (r != GRA_OK) {
+      const std::string m = h->err;
+      gra_hip_destroy(h);
+      return set_global_error(r, "gra_hip_create: %s", m.c_str());
+    }
+  }
+  *out = h;
+  return GRA_OK;
+}
+
+extern "C" void gra_hip_destroy(gra_handle* h) {
+  if (!h) return;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->flow) flow_release(h->flow);
+  DevBuf* bufs[] = {&h->d_nodes, &h->d_x,    &h->d_W,         &h->d_amp2,      &h->d_flags,
+                    &h->d_vw,    &h->d_bins, &h->d_rec,       &h->d_ncand,     &h->d_grid,
+                    &h->d_part_hist, &h->d_part_scal, &h->d_stats, &h->d_aux};
+  for (DevBuf* b : bufs) b->release();
+  for (int k = 0; k < 2 * TS_N; ++k)
+    if (h->ev[k]) (void)hipEventDestroy(h->ev[k]);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+extern "C" int gra_hip_eval_points_device(gra_handle* h, const double* d_u, size_t n, double* d_W,
+                                          double* d_amp2, uint8_t* d_flags) {
+  if (!h) return set_global_error(GRA_E_ARG, "handle is NULL");
+  if (n == 0) return GRA_OK;
+  if (!d_u || !d_W || !d_flags) return fail(h, GRA_E_ARG, "eval_points_device: NULL buffer");
+  if (n > 0xFFFFFFFFull) return fail(h, GRA_E_ARG, "eval_points_device: n too large");
+  int r = ensure_point_buffers(h, n);
+  if (r) return r;
+  r = run_integrand(h, nullptr, 0, 0, 0, 0, d_u, n, d_W, d_amp2, d_flags, h->d_vw.as<double>(),
+                    h->d_bins.as<uint8_t>());
+  if (r) return r;
+  GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  time_collect(h, true, h->P.screening != 0, false);
+  return GRA_OK;
+}
+
+extern "C" int gra_hip_eval_points(gra_handle* h, const double* u, size_t n, double* W,
+                                   double* amp2, uint8_t* flags) {
+  if (!h) return set_global_error(GRA_E_ARG, "handle is NULL");
+  if (n == 0) return GRA_OK;
+  if (!u || !W) return fail(h, GRA_E_ARG, "eval_points: u and W must not be NULL");
+  if (n > 0xFFFFFFFFull) return fail(h, GRA_E_ARG, "eval_points: n too large");
+  const int D = h->P.dim;
+  int r = ensure_point_buffers(h, n);
+  if (r) return r;
+  if (h->d_x.ensure(sizeof(double) * D * n)) return fail(h, GRA_E_NOMEM, "input buffer");
+  GRA_HIP_CHECK(h, hipMemcpyAsync(h->d_x.p, u, sizeof(double) * D * n, hipMemcpyHostToDevice,
+                                  h->stream));
+  r = run_integrand(h, nullptr, 0, 0, 0, 0, h->d_x.as<double>(), n, h->d_W.as<double>(),
+                    h->d_amp2.as<double>(), h->d_flags.as<uint8_t>(), h->d_vw.as<double>(),
+                    h->d_bins.as<uint8_t>());
+  if (r) return r;
+  GRA_HIP_CHECK(h, hipMemcpyAsync(W, h->d_W.p, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+  if (amp2)
+    GRA_HIP_CHECK(h, hipMemcpyAsync(amp2, h->d_amp2.p, sizeof(double) * n, hipMemcpyDeviceToHost,
+                                    h->stream));
+  if (flags)
+    GRA_HIP_CHECK(h, hipMemcpyAsync(flags, h->d_flags.p, n, hipMemcpyDeviceToHost, h->stream));
+  GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  time_collect(h, true, h->P.screening != 0, false);
+  return GRA_OK;
+}
+
+static int check_grid(gra_handle* h, const gra_grid* g) {
+  if (!g) return fail(h, GRA_E_ARG, "grid is NULL");
+  if (g->dim != h->P.dim)
+    return fail(h, GRA_E_ARG, "grid dim %d != process dim %d", g->dim, h->P.dim);
+  if (g->bins < 2 || g->bins > GRA_VEGAS_BINS || (g->bins % 2))
+    return fail(h, GRA_E_ARG, "grid bins %d invalid", g->bins);
+  return GRA_OK;
+}
+
+extern "C" int gra_hip_vegas_iterate(gra_handle* h, const gra_grid* grid, uint64_t n_calls,
+                                     uint64_t seed, uint32_t iter, gra_iter_stats* out) {
+  if (!h) return set_global_error(GRA_E_ARG, "handle is NULL");
+  int r = check_grid(h, grid);
+  if (r) return r;
+  if (!out) return fail(h, GRA_E_ARG, "vegas_iterate: out is NULL");
+  if (n_calls == 0 || n_calls > 0xFFFFFFFFull)
+    return fail(h, GRA_E_ARG, "vegas_iterate: n_calls %llu outside [1, 2^32)",
+                (unsigned long long)n_calls);
+  const int D = h->P.dim;
+  r = ensure_point_buffers(h, n_calls);
+  if (r) return r;
+  const int nblk = (int)std::min<uint64_t>(2048, std::max<uint64_t>(1, (n_calls + 2047) / 2048));
+  const size_t H = 2 * (size_t)D * GRA_VEGAS_BINS;
+  if (h->d_grid.ensure(sizeof(double) * GRA_MAX_DIM * GRA_VEGAS_BINS) ||
+      h->d_part_hist.ensure(sizeof(double) * H * nblk) ||
+      h->d_part_scal.ensure(sizeof(double) * 10 * nblk) ||
+      h->d_stats.ensure(sizeof(double) * (10 + H)))
+    return fail(h, GRA_E_NOMEM, "VEGAS buffers");
+  GRA_HIP_CHECK(h, hipMemcpyAsync(h->d_grid.p, grid->xmat, sizeof(double) * D * GRA_VEGAS_BINS,
+                                  hipMemcpyHostToDevice, h->stream));
+  r = run_integrand(h, h->d_grid.as<double>(), (unsigned int)grid->bins, seed, iter, 0, nullptr,
+                    n_calls, h->d_W.as<double>(), nullptr, h->d_flags.as<uint8_t>(),
+                    h->d_vw.as<double>(), h->d_bins.as<uint8_t>());
+  if (r) return r;
+  time_begin(h, TS_ACCUM);
+  GRA_HIP_CHECK(h, launch_accumulate(h->d_W.as<double>(), h->d_vw.as<double>(),
+                                     h->d_bins.as<uint8_t>(), h->d_flags.as<uint8_t>(), n_calls, D,
+                                     nblk, h->d_part_hist.as<double>(),
+                                     h->d_part_scal.as<double>(), h->d_stats.as<double>(),
+                                     h->stream));
+  time_end(h, TS_ACCUM);
+  h->h_stats.resize(10 + H);
+  GRA_HIP_CHECK(h, hipMemcpyAsync(h->h_stats.data(), h->d_stats.p, sizeof(double) * (10 + H),
+                                  hipMemcpyDeviceToHost, h->stream));
+  GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  time_collect(h, true, h->P.screening != 0, true);
+  const double* s = h->h_stats.data();
+  std::memset(out, 0, sizeof(*out));
+  out->fsum = s[0] - s[1];
+  out->fsum_c = s[1];
+  out->f2sum = s[2] - s[3];
+  out->f2sum_c = s[3];
+  out->maxW = s[4];
+  out->maxf = s[5];
+  out->n_calls = n_calls;
+  out->n_kin_ok = (uint64_t)s[6];
+  out->n_fid_ok = (uint64_t)s[7];
+  out->n_valid = (uint64_t)s[8];
+  out->n_nan = (uint64_t)s[9];
+  for (int j = 0; j < D; ++j)
+    for (int b = 0; b < GRA_VEGAS_BINS; ++b) {
+      out->fmat[j * GRA_VEGAS_BINS + b] = s[10 + j * GRA_VEGAS_BINS + b];
+      out->f2mat[j * GRA_VEGAS_BINS + b] = s[10 + (D + j) * GRA_VEGAS_BINS + b];
+    }
+  return GRA_OK;
+}
+
+// Iteration combination of MGraniitti::VEGAS, src/MGraniitti.cc:1222-1251
+static void vegas_combine(double fsum, double f2sum, double calls, int iter, double* state,
+                          double* sigma, double* sigma_err, double* chi2) {
+  const double EPS = 1.0e-30;
+  fsum *= 1.0 / calls;
+  f2sum *= (1.0 / calls) * (1.0 / calls);
+  const double I_this = fsum;
+  f2sum = msqrt(f2sum * calls);
+  f2sum = (f2sum - fsum) * (f2sum + fsum);
+  if (f2sum <= 0.0) f2sum = EPS;
+  const double I2_this = f2sum * 1.0 / calls;
+  const double alpha = 1.0 / I2_this;
+  state[0] += alpha * I_this;
+  state[1] += alpha * (I_this * I_this);
+  state[2] += alpha;
+  *sigma = state[0] / state[2];
+  *sigma_err = msqrt(1.0 / state[2]);
+  double c2 = (state[1] - (state[0] * state[0]) / state[2]) / (iter + 1E-5);
+  *chi2 = (c2 < 0) ? 0.0 : c2;
+}
+
+extern "C" int gra_hip_vegas_run(gra_handle* h, const gra_vegas_cfg* vc, gra_grid* grid,
+                                 gra_vegas_result* res) {
+  if (!h) return set_global_error(GRA_E_ARG, "handle is NULL");
+  if (!vc || !grid || !res) return fail(h, GRA_E_ARG, "vegas_run: NULL argument");
+  if (vc->ncall == 0 || vc->iter < 1 || vc->burnin_iter < 0)
+    return fail(h, GRA_E_ARG, "vegas_run: ncall >= 1, iter >= 1, burnin_iter >= 0 required");
+  std::memset(res, 0, sizeof(*res));
+  res->seconds_to_precision = -1.0;
+  gra_iter_stats* st = new (std::nothrow) gra_iter_stats;
+  if (!st) return fail(h, GRA_E_NOMEM, "vegas_run: host stats");
+  const double kt0 = h->time_ms[TS_TREE] + h->time_ms[TS_SCREEN] + h->time_ms[TS_ACCUM];
+  const auto t0 = std::chrono::steady_clock::now();
+  auto since = [&t0]() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  };
+  int r = GRA_OK;
+  uint32_t counter = 0;
+  double state[3] = {0, 0, 0};
+  double sigma = 0, err = 0, chi2 = 0;
+  const int bins = vc->bins > 0 ? vc->bins : GRA_VEGAS_BINS;
+  // init = 0: fresh uniform grid (VEGASData::Init(0) -> ClearAll)
+  r = grid_init(grid, h->P.dim, bins);
+  if (r) {
+    delete st;
+    return fail(h, r, "%s", global_error());
+  }
+  for (int it = 0; it < vc->burnin_iter && r == GRA_OK; ++it) {
+    r = gra_hip_vegas_iterate(h, grid, vc->ncall, vc->seed, counter++, st);
+    if (r) break;
+    vegas_combine(st->fsum, st->f2sum, (double)vc->ncall, it, state, &sigma, &err, &chi2);
+    res->calls_total += vc->ncall;
+    grid_optimize(grid, st->f2mat, vc->lambda);
+  }
+  const double t_main = since();
+  // init = 1: keep the grid, clear the integral (VEGASData::ClearIntegral)
+  state[0] = state[1] = state[2] = 0;
+  int itermin = vc->iter;
+  const int itermax = std::max(vc->max_iter, vc->iter);
+  int done = 0;
+  for (int it = 0; it < itermin && r == GRA_OK; ++it) {
+    r = gra_hip_vegas_iterate(h, grid, vc->ncall, vc->seed, counter++, st);
+    if (r) break;
+    vegas_combine(st->fsum, st->f2sum, (double)vc->ncall, it, state, &sigma, &err, &chi2);
+    res->calls_total += vc->ncall;
+    res->maxW = std::max(res->maxW, st->maxW);
+    res->maxf = std::max(res->maxf, st->maxf);
+    ++done;
+    if (vc->precision > 0 && res->seconds_to_precision < 0 && sigma > 0 &&
+        err / sigma <= vc->precision)
+      res->seconds_to_precision = since();
+    if (it == itermin - 1 && itermin < itermax &&
+        ((vc->chi2max > 0 && chi2 > vc->chi2max) ||
+         (vc->precision > 0 && err / sigma > vc->precision)))
+      ++itermin;
+    grid_optimize(grid, st->f2mat, vc->lambda);
+  }
+  delete st;
+  if (r) return r;
+  res->sigma = sigma;
+  res->sigma_err = err;
+  res->chi2 = chi2;
+  res->iterations = done;
+  res->seconds_total = since();
+  res->seconds = res->seconds_total - t_main;
+  res->kernel_ms = h->time_ms[TS_TREE] + h->time_ms[TS_SCREEN] + h->time_ms[TS_ACCUM] - kt0;
+  if (!(std::isfinite(sigma)) || sigma < 1e-60)
+    return fail(h, GRA_E_NUMERIC, "VEGAS: integral %g is not finite or < 1e-60", sigma);
+  return GRA_OK;
+}
+
+extern "C" int gra_hip_vegas_map(gra_handle* h, const gra_grid* grid, const double* u, size_t n,
+                                 double* x, double* vw, int32_t* bins) {
+  if (!h) return set_global_error(GRA_E_ARG, "handle is NULL");
+  int r = check_grid(h, grid);
+  if (r) return r;
+  if (n == 0) return GRA_OK;
+  if (!u || !x || !vw || !bins) return fail(h, GRA_E_ARG, "vegas_map: NULL buffer");
+  const int D = grid->dim;
+  DevBuf du, dx, dv, db;
+  if (du.ensure(sizeof(double) * D * n) || dx.ensure(sizeof(double) * D * n) ||
+      dv.ensure(sizeof(double) * n) || db.ensure(sizeof(int32_t) * D * n) ||
+      h->d_grid.ensure(sizeof(double) * GRA_MAX_DIM * GRA_VEGAS_BINS))
+    return fail(h, GRA_E_NOMEM, "vegas_map buffers");
+  GRA_HIP_CHECK(h, hipMemcpyAsync(h->d_grid.p, grid->xmat, sizeof(double) * D * GRA_VEGAS_BINS,
+                                  hipMemcpyHostToDevice, h->stream));
+  GRA_HIP_CHECK(h, hipMemcpyAsync(du.p, u, sizeof(double) * D * n, hipMemcpyHostToDevice, h->stream));
+  GRA_HIP_CHECK(h, launch_map(h->d_grid.as<double>(), D, (unsigned int)grid->bins, du.as<double>(),
+                              n, dx.as<double>(), dv.as<double>(), db.as<int32_t>(), h->stream));
+  GRA_HIP_CHECK(h, hipMemcpyAsync(x, dx.p, sizeof(double) * D * n, hipMemcpyDeviceToHost, h->stream));
+  GRA_HIP_CHECK(h, hipMemcpyAsync(vw, dv.p, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+  GRA_HIP_CHECK(h, hipMemcpyAsync(bins, db.p, sizeof(int32_t) * D * n, hipMemcpyDeviceToHost,
+                                  h->stream));
+  GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  du.release();
+  dx.release();
+  dv.release();
+  db.release();
+  return GRA_OK;
+}
+
+extern "C" int gra_hip_unweight(gra_handle* h, const double* w, size_t n, double wmax,
+                                uint64_t seed, uint8_t* accept, uint64_t* n_accepted,
+                                uint64_t* n_overflow) {
+  if (!h) return set_global_error(GRA_E_ARG, "handle is NULL");
+  if (!w || !accept) return fail(h, GRA_E_ARG, "unweight: NULL buffer");
+  if (!(wmax > 0.0)) return fail(h, GRA_E_ARG, "unweight: wmax must be > 0");
+  if (n == 0) {
+    if (n_accepted) *n_accepted = 0;
+    if (n_overflow) *n_overflow = 0;
+    return GRA_OK;
+  }
+  DevBuf dw, da, dc;
+  if (dw.ensure(sizeof(double) * n) || da.ensure(n) || dc.ensure(2 * sizeof(unsigned long long)))
+    return fail(h, GRA_E_NOMEM, "unweight buffers");
+  GRA_HIP_CHECK(h, hipMemcpyAsync(dw.p, w, sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
+  GRA_HIP_CHECK(h, hipMemsetAsync(dc.p, 0, 2 * sizeof(unsigned long long), h->stream));
+  GRA_HIP_CHECK(h, launch_unweight(dw.as<double>(), n, wmax, seed, da.as<uint8_t>(),
+                                   dc.as<unsigned long long>(), h->stream));
+  unsigned long long c[2] = {0, 0};
+  GRA_HIP_CHECK(h, hipMemcpyAsync(accept, da.p, n, hipMemcpyDeviceToHost, h->stream));
+  GRA_HIP_CHECK(h, hipMemcpyAsync(c, dc.p, sizeof(c), hipMemcpyDeviceToHost, h->stream));
+  GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  if (n_accepted) *n_accepted = c[0];
+  if (n_overflow) *n_overflow = c[1];
+  return GRA_OK;
+}
+
+extern "C" int gra_hip_eikonal_tables(gra_handle* h, double* node_amp, double* mbt, double* msa) {
+  if (!h) return set_global_error(GRA_E_ARG, "handle is NULL");
+  if (!h->cfg.screening) return fail(h, GRA_E_STATE, "eikonal tables: screening is off");
+  if (node_amp) {
+    const int NK = h->cfg.n_loop_kt;
+    for (int j = 0; j <= NK; ++j) {
+      node_amp[2 * j] = h->nodes[j].aeik_re;
+      node_amp[2 * j + 1] = h->nodes[j].aeik_im;
+    }
+  }
+  if (mbt) std::memcpy(mbt, h->mbt.data(), sizeof(double) * h->mbt.size());
+  if (msa) std::memcpy(msa, h->msa.data(), sizeof(double) * h->msa.size());
+  return GRA_OK;
+}
+
+extern "C" int gra_hip_kernel_time(gra_handle* h, int32_t which, double* total_ms,
+                                   uint64_t* launches, int32_t reset) {
+  if (!h) return set_global_error(GRA_E_ARG, "handle is NULL");
+  if (which < 0 || which >= TS_N) return fail(h, GRA_E_ARG, "kernel_time: slot %d", which);
+  if (total_ms) *total_ms = h->time_ms[which];
+  if (launches) *launches = h->launches[which];
+  if (reset) {
+    h->time_ms[which] = 0;
+    h->launches[which] = 0;
+  }
+  return GRA_OK;
+}

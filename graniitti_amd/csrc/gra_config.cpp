@@ -1,0 +1,253 @@
+// Host-side configuration and VEGAS grid arithmetic of the C ABI.
+//
+// gra_hip_cfg_default fills the preset workloads with the TUNE0 model values
+// (modeldata/TUNE0/GENERAL.json), the PDG table (modeldata/mass_width_2018.mcd)
+// and the eikonal numerics (modeldata/NUMERICS.json, include/Graniitti/MEikonal.h:26-57).
+// The grid routines restate VEGASData (include/Graniitti/MVEGAS.h); they run on
+// the host once per iteration over BINS x D = 128 x 8 numbers.
+#include <cmath>
+#include <cstring>
+
+#include "gra_internal.h"
+
+namespace gra_hip {
+
+namespace {
+
+void set_common(gra_cfg* c) {
+  std::memset(c, 0, sizeof(*c));
+  c->abi_size = (int32_t)sizeof(gra_cfg);
+
+  // Beams: PROCESSPARAM BEAM ["p+","p+"], ENERGY [6500, 6500] (tests/processes/ALICECUTS.json)
+  c->beam_E1 = 6500.0;
+  c->beam_E2 = 6500.0;
+  c->m_beam = 0.938272081;  // p, mass_width_2018.mcd:169
+
+  // MProcess::OFFSHELL default (include/Graniitti/MProcess.h:363)
+  c->offshell = 5.0;
+
+  // GENCUT defaults and technical boundaries (src/MProcess.cc:1369-1386,
+  // src/MContinuum.cc:71-82 with Kf = 2)
+  c->fwd_pt_min = 0.0;
+  c->fwd_pt_max = 2.5;
+  c->kt_min = 0.0;
+  c->kt_max = 10.0 / 2.0;
+
+  // FIDCUT (ALICECUTS.json values where the presets state no other)
+  c->fid_active = 1;
+  c->fid_rap_min = -10.0;
+  c->fid_rap_max = 10.0;
+  c->fid_et_min = 0.0;
+  c->fid_et_max = 100000.0;
+  c->fid_pt_max = 100000.0;
+  c->sys_M_min = 0.0;
+  c->sys_M_max = 100000.0;
+  c->sys_Y_min = -10.0;
+  c->sys_Y_max = 10.0;
+  c->sys_Pt_min = 0.0;
+  c->sys_Pt_max = 100000.0;
+  c->fwd_t_min = 0.0;
+  c->fwd_t_max = 100000.0;
+  c->fwd_M_min = 0.0;
+  c->fwd_M_max = 100000.0;
+
+  // PARAM_SOFT
+  c->delta_P = 0.098;
+  c->alpha_P = 0.065;
+  c->gN_P = 8.20;
+  c->fc1 = 0.34;
+  c->fc2 = 5.0;
+  c->fc3 = 10.0;
+  c->mpi_const = 0.13957018;  // PDG::mpi, include/Graniitti/MPDG.h:56
+
+  // PARAM_REGGE
+  const double a0[3] = {1.088, 0.55, 0.55};
+  const double ap[3] = {0.20, 0.93, 0.93};
+  const double sg[3] = {1.0, -1.0, 1.0};
+  for (int k = 0; k < 3; ++k) {
+    c->regge_a0[k] = a0[k];
+    c->regge_ap[k] = ap[k];
+    c->regge_sgn[k] = sg[k];
+  }
+  c->regge_s0 = 1.0;
+  c->offshellFF = 2;
+  c->b_EXP = 0.944;
+  c->a_OREAR = 0.638;
+  c->b_OREAR = 1.886;
+  c->b_POW = 0.500;
+
+  // PARAM_TENSORPOM
+  c->tp_delta_P = 0.0808;
+  c->tp_ap_P = 0.25;
+  c->tp_gPNN = 1.87;
+  c->tp_gPvv[0] = 0.70;
+  c->tp_gPvv[1] = 6.20;
+  c->tp_mp_ff = 0.938272081;
+
+  // Eikonal numerics: NumberBT = NumberKT2 = 10000 (NUMERICS.json),
+  // FBIntegralN = 10000, MinLoopKT = 1e-4 (MEikonal.h:37-40)
+  c->n_bt = 10000;
+  c->n_kt2 = 10000;
+  c->n_fb = 10000;
+  c->min_loop_kt = 1e-4;
+  c->max_loop_kt = 3.0;
+  c->n_loop_phi = 15;
+  c->n_loop_kt = 30;
+}
+
+void set_pipi(gra_cfg* c) {
+  c->process = GRA_PROC_REGGE_CON;
+  c->dim = 8;  // 3 (Kf + 2) - 4, src/MContinuum.cc:85
+  c->m_central = 0.13957061;  // pi+-, mass_width_2018.mcd:57
+  c->w_central = 0.0;  // width 2.5284e-17 GeV moves the mass by < 1 ulp: pole mass
+  c->m_daughter = 0.0;
+  c->S_factor = 1.0;  // pi+ pi-
+  c->m_prop = 0.13957061;
+  // PARAM_PI couplings and active trajectories
+  c->regge_c[0] = 35.00184;
+  c->regge_c[1] = 81.63672;
+  c->regge_c[2] = 10.86264;
+  c->regge_on[0] = 1;
+  c->regge_on[1] = 0;
+  c->regge_on[2] = 0;
+  c->rap_min = -0.9;
+  c->rap_max = 0.9;
+  c->fid_eta_min = -0.9;
+  c->fid_eta_max = 0.9;
+  c->fid_pt_min = 0.15;
+}
+
+void set_rhorho(gra_cfg* c) {
+  c->process = GRA_PROC_TENSOR_CON;
+  c->dim = 14;  // 8 phase-space + 2 BW masses + 2 x (cos theta, phi) decay angles
+  c->m_central = 0.775;
+  c->w_central = 0.149;
+  c->m_daughter = 0.13957061;
+  c->S_factor = 2.0;  // rho0 rho0
+  c->m_prop = 0.775;
+  // PARAM_RHO couplings (ansatz), used only if the Regge amplitude is selected
+  c->regge_c[0] = 25.0;
+  c->regge_on[0] = 1;
+  c->rap_min = -2.5;
+  c->rap_max = 2.5;
+  c->fid_eta_min = -2.5;
+  c->fid_eta_max = 2.5;
+  c->fid_pt_min = 0.1;
+}
+
+}  // namespace
+
+}  // namespace gra_hip
+
+using namespace gra_hip;
+
+extern "C" int gra_hip_cfg_default(gra_cfg* cfg, int32_t preset) {
+  if (!cfg) return set_global_error(GRA_E_ARG, "gra_hip_cfg_default: cfg is NULL");
+  set_common(cfg);
+  switch (preset) {
+    case 1:
+      set_pipi(cfg);
+      cfg->screening = 0;
+      cfg->amp_precision = GRA_AMP_FP64;
+      break;
+    case 2:
+      set_pipi(cfg);
+      cfg->screening = 1;
+      cfg->amp_precision = GRA_AMP_FP32;
+      break;
+    case 3:
+      set_rhorho(cfg);
+      cfg->screening = 1;
+      cfg->amp_precision = GRA_AMP_FP32;
+      break;
+    default:
+      return set_global_error(GRA_E_ARG, "gra_hip_cfg_default: unknown preset %d", preset);
+  }
+  return GRA_OK;
+}
+
+namespace gra_hip {
+
+#define XM(g, i, j) ((g)->xmat[(j)*GRA_VEGAS_BINS + (i)])
+
+// VEGASData::Rebin, include/Graniitti/MVEGAS.h:87-104
+static void grid_rebin(gra_grid* g, double ac, int j, const double* rvec) {
+  const int B = g->bins;
+  unsigned int k = 0;
+  double dr = 0.0, zn = 0.0, zo = 0.0;
+  double xc[GRA_VEGAS_BINS];
+  for (int i = 0; i < B - 1; ++i) {
+    while (ac > dr) dr += rvec[(++k) - 1];
+    if (k > 1) zo = XM(g, k - 2, j);
+    zn = XM(g, k - 1, j);
+    dr -= ac;
+    xc[i] = zn - (zn - zo) * dr / rvec[k - 1];
+  }
+  for (int i = 0; i < B - 1; ++i) XM(g, i, j) = xc[i];
+  XM(g, B - 1, j) = 1.0;
+}
+
+int grid_init(gra_grid* g, int dim, int bins) {
+  if (!g) return set_global_error(GRA_E_ARG, "grid_init: grid is NULL");
+  if (dim < 1 || dim > GRA_MAX_DIM)
+    return set_global_error(GRA_E_ARG, "grid_init: dim %d outside [1, %d]", dim, GRA_MAX_DIM);
+  if (bins < 2 || bins > GRA_VEGAS_BINS || (bins % 2) != 0)
+    return set_global_error(GRA_E_ARG, "grid_init: bins %d must be even and <= %d", bins,
+                            GRA_VEGAS_BINS);
+  std::memset(g, 0, sizeof(*g));
+  g->dim = dim;
+  g->bins = bins;
+  for (int j = 0; j < dim; ++j) XM(g, 0, j) = 1.0;
+  double rvec[GRA_VEGAS_BINS];
+  for (int i = 0; i < GRA_VEGAS_BINS; ++i) rvec[i] = 1.0;
+  for (int j = 0; j < dim; ++j) grid_rebin(g, 1.0 / (double)bins, j, rvec);
+  return GRA_OK;
+}
+
+// VEGASData::OptimizeGrid, include/Graniitti/MVEGAS.h:107-142
+int grid_optimize(gra_grid* g, double* f2mat, double lambda) {
+  const int B = g->bins;
+  const double EPS = 1.0e-30;
+  double dcache[GRA_MAX_DIM];
+  double rvec[GRA_VEGAS_BINS];
+#define F2(i, j) f2mat[(j)*GRA_VEGAS_BINS + (i)]
+  for (int j = 0; j < g->dim; ++j) {
+    double zo = F2(0, j), zn = F2(1, j), ac;
+    F2(0, j) = (zo + zn) / 2.0;
+    dcache[j] = F2(0, j);
+    for (int i = 2; i < B; ++i) {
+      ac = zo + zn;
+      zo = zn;
+      zn = F2(i, j);
+      F2(i - 1, j) = (ac + zn) / 3.0;
+      dcache[j] += F2(i - 1, j);
+    }
+    F2(B - 1, j) = (zo + zn) / 2.0;
+    dcache[j] += F2(B - 1, j);
+  }
+  for (int j = 0; j < g->dim; ++j) {
+    double ac = 0.0;
+    for (int i = 0; i < B; ++i) {
+      F2(i, j) = (F2(i, j) < EPS) ? EPS : F2(i, j);
+      rvec[i] = std::pow((1.0 - F2(i, j) / dcache[j]) /
+                             (std::log(dcache[j]) - std::log(F2(i, j)) + EPS),
+                         lambda);
+      if (std::isnan(rvec[i]) || std::isinf(rvec[i])) rvec[i] = EPS;
+      ac += rvec[i];
+    }
+    grid_rebin(g, ac / B, j, rvec);
+  }
+#undef F2
+  return GRA_OK;
+}
+
+}  // namespace gra_hip
+
+extern "C" int gra_hip_grid_init(gra_grid* grid, int32_t dim, int32_t bins) {
+  return grid_init(grid, dim, bins);
+}
+
+extern "C" int gra_hip_grid_optimize(gra_grid* grid, gra_iter_stats* st, double lambda) {
+  if (!grid || !st) return set_global_error(GRA_E_ARG, "gra_hip_grid_optimize: NULL argument");
+  return grid_optimize(grid, st->f2mat, lambda);
+}

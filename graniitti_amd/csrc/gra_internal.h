@@ -1,0 +1,83 @@
+// Host-side internals of the C ABI: the handle, error reporting, buffers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "gra_hip.h"
+#include "gra_params.h"
+
+namespace gra_hip {
+
+// Global (handle-less) error text, for gra_hip_create failures and host-only calls
+int set_global_error(int code, const char* fmt, ...);
+const char* global_error();
+
+int grid_init(gra_grid* g, int dim, int bins);
+int grid_optimize(gra_grid* g, double* f2mat, double lambda);
+
+// Device buffer that grows on demand
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t need);
+  void release();
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+struct FlowState;  // normalizing-flow sampler (gra_flow.hip)
+void flow_release(FlowState* f);
+
+// Kernel timing slots
+enum TimeSlot { TS_TREE = 0, TS_SCREEN = 1, TS_ACCUM = 2, TS_FLOW = 3, TS_N = 4 };
+
+}  // namespace gra_hip
+
+struct gra_handle {
+  gra_cfg cfg;
+  RunParams P;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  std::string err;
+
+  // eikonal tables (x, re, im rows, host copies after the 15-digit round trip)
+  std::vector<double> mbt, msa;
+  std::vector<NodeConst> nodes;
+  gra_hip::DevBuf d_nodes;
+
+  // per-point buffers
+  gra_hip::DevBuf d_x, d_W, d_amp2, d_flags, d_vw, d_bins, d_rec, d_ncand, d_grid;
+  gra_hip::DevBuf d_part_hist, d_part_scal, d_stats, d_aux;
+  std::vector<double> h_stats;
+
+  // timing
+  hipEvent_t ev[2 * gra_hip::TS_N] = {};
+  double time_ms[gra_hip::TS_N] = {};
+  uint64_t launches[gra_hip::TS_N] = {};
+
+  gra_hip::FlowState* flow = nullptr;
+};
+
+namespace gra_hip {
+int fail(gra_handle* h, int code, const char* fmt, ...);
+// Integrand evaluation on device-resident inputs (x given) or a grid draw.
+int run_integrand(gra_handle* h, const double* d_grid, unsigned int bins, uint64_t seed,
+                  uint32_t iter, uint64_t first, const double* d_x, uint64_t n, double* d_W,
+                  double* d_amp2, uint8_t* d_flags, double* d_vw, uint8_t* d_bins);
+int ensure_point_buffers(gra_handle* h, uint64_t n);
+}  // namespace gra_hip
+
+#define GRA_HIP_CHECK(h, call)                                                        \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return gra_hip::fail((h), GRA_E_HIP, "%s: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                           __FILE__, __LINE__);                                       \
+  } while (0)

@@ -1,0 +1,42 @@
+// Launchers of the device kernels (one set per floating-point mode).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "gra_params.h"
+
+namespace gra_hip {
+
+#define GRA_DECLARE_INTEGRAND_LAUNCHERS(NS)                                                        \
+  namespace NS {                                                                                   \
+  size_t loop_record_bytes();                                                                      \
+  hipError_t launch_tree(const RunParams& P, const double* grid, unsigned int bins, uint64_t seed, \
+                         uint32_t iter, uint64_t first, const double* xin, uint64_t n, double* W,  \
+                         double* amp2, uint8_t* flags, double* vw, uint8_t* binsout, void* rec,    \
+                         uint32_t* n_cand, hipStream_t st);                                        \
+  hipError_t launch_screen(const RunParams& P, const void* nodes, const void* rec,                 \
+                           const uint32_t* n_cand, uint64_t n_max, double* W, double* amp2,        \
+                           uint8_t* flags, hipStream_t st);                                        \
+  }
+
+GRA_DECLARE_INTEGRAND_LAUNCHERS(k64)
+GRA_DECLARE_INTEGRAND_LAUNCHERS(k32)
+
+hipError_t launch_accumulate(const double* W, const double* vw, const uint8_t* bins,
+                             const uint8_t* flags, uint64_t n, int D, int nblk, double* part_hist,
+                             double* part_scal, double* out, hipStream_t st);
+hipError_t launch_map(const double* grid, int D, unsigned int bins, const double* u, uint64_t n,
+                      double* x, double* vw, int32_t* b, hipStream_t st);
+hipError_t launch_unweight(const double* w, uint64_t n, double wmax, uint64_t seed, uint8_t* acc,
+                           unsigned long long* counts, hipStream_t st);
+
+// Eikonal Fourier-Bessel tables: rows i = 0..nrows-1 at x_i = xmin + i xstep,
+// out_i = factor * CSIntegral_q[pre_q J0(m(x_i) q_v) q_v], q_v = qmin + q qstep,
+// m(x) = x (density, which = 0) or sqrt(max(x, 0)) (screening, which = 1).
+hipError_t launch_fb_table(int which, const double* pre, int nf, double qmin, double qstep,
+                           double xmin, double xstep, int nrows, double factor, double* out,
+                           hipStream_t st);
+
+}  // namespace gra_hip

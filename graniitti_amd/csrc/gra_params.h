@@ -1,0 +1,58 @@
+// Run constants shared by host and device code. Filled once per handle on the
+// host (so every transcendental constant comes from the host libm, exactly as
+// the reference computes it) and passed to kernels by value.
+#pragma once
+
+#include <stdint.h>
+
+#include "gra_hip.h"
+
+#define GRA_PI 3.141592653589793238462643383279502884197169399375105820974944
+
+struct GV4 {
+  double e, x, y, z;
+};
+
+struct RunParams {
+  int32_t process, amp_mode, dim, screening;
+  // beams and invariants
+  GV4 pbeam1, pbeam2, beamsum;
+  double s, sqrt_s, rs, s32;
+  // sampling map (BNRandomKin)
+  double lmin, lmax, kt_min, kt_max, rap_min, rap_max;
+  double m_beam, m_central, w_central, m_daughter;
+  double bw_a1, bw_a2, bw_dmin;  // Breit-Wigner inverse-CDF constants
+  // fiducial cuts
+  int32_t fid_active, pad0;
+  double fid_eta_min, fid_eta_max, fid_rap_min, fid_rap_max;
+  double fid_pt_min, fid_pt_max, fid_et_min, fid_et_max;
+  double sys_M_min, sys_M_max, sys_Y_min, sys_Y_max, sys_Pt_min, sys_Pt_max;
+  double fwd_t_min, fwd_t_max;
+  // weight constants
+  double psw_c1, psw_c2, lspan2, KT_vol, PHI_vol, invS, GeV2barn, flux;
+  // Regge amplitude
+  int32_t regge_on[3];
+  int32_t regge_k;  // index of the single active trajectory, -1 if several
+  double regge_a0[3], regge_ap[3], regge_s0;
+  double eta0_re[3], eta0_im[3];  // ReggeEta(alpha(0), sigma)
+  double ph_c;                    // -pi/2 alpha'_k of the single trajectory
+  double amp_c2;  // |eta0^2 gN_P^2 gpp_P^2|^2: squared common factor of the FP32 amplitude
+  double gN_P, fc1, fc2, gpp_P, m_prop2;
+  int32_t offshellFF, pad1;
+  double b_EXP, a_OREAR, b_OREAR, b_POW;
+  // screening loop
+  int32_t nphi, nkt;  // intervals
+  double step_kt, step_phi, min_loop_kt, hfac, nrm;
+  double kappa;  // nrm * hfac (FP32 loop sum normalisation)
+  // tensor pomeron
+  double tp_delta_P, tp_ap_P, tp_gPNN, tp_gPvv0, tp_gPvv1, tp_mp_ff, m_rho;
+};
+
+// Screening node constants: one entry per (phi_i, kt_j) node, i-major.
+struct NodeConst {
+  double kx, ky;   // kt cos(phi_i), kt sin(phi_i)
+  double kt;       // kt_j
+  double w;        // Simpson 3/8 weight U_i V_j
+  double aeik_re, aeik_im;  // A_eik(kt_j^2) from the interpolated table
+  float cf_re, cf_im;       // FP32: w * kt * A_eik (scaled by the run's table scale)
+};

@@ -1,0 +1,299 @@
+/*
+ * gra_hip.h -- C ABI of the MI355X (gfx950) event-generation inner loop.
+ *
+ * The library evaluates the per-point event weight W = |M|^2 x Jacobian of the
+ * GRANIITTI <C> continuum class (pp -> p X p, X = 2 central particles) on the
+ * GPU, runs VEGAS iterations over it, and trains / samples a rational-quadratic
+ * spline normalizing flow used as an alternative importance sampler.
+ *
+ * Conventions
+ *   - every entry point returns an int status: GRA_OK (0) or a negative
+ *     GRA_E_* code; the message of the last failure is in gra_hip_last_error();
+ *   - no C++ exception crosses this boundary;
+ *   - host arrays are owned by the caller and are only touched during the call;
+ *   - a handle owns all its device memory and one HIP stream, and is used by one
+ *     host thread at a time (the reference keeps one process copy per thread for
+ *     the same reason, src/MGraniitti.cc:278-309);
+ *   - calls are synchronous.
+ *
+ * Each declaration names the reference interface it replaces
+ * (mieskolainen/GRANIITTI v1.051, paths relative to the reference root).
+ */
+#ifndef GRA_HIP_H
+#define GRA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gra_handle gra_handle; /* opaque: device state of one process */
+
+/* ------------------------------------------------------------------------- */
+/* Status codes                                                              */
+/* ------------------------------------------------------------------------- */
+#define GRA_OK 0
+#define GRA_E_ARG (-1)     /* invalid argument (reference: std::invalid_argument) */
+#define GRA_E_HIP (-2)     /* HIP runtime failure */
+#define GRA_E_NOMEM (-3)   /* device allocation failed */
+#define GRA_E_STATE (-4)   /* call not valid in the handle's state */
+#define GRA_E_NUMERIC (-5) /* too many NaN/Inf weights (src/MProcess.h:317-340) */
+
+/* ------------------------------------------------------------------------- */
+/* Limits                                                                    */
+/* ------------------------------------------------------------------------- */
+#define GRA_MAX_DIM 16     /* hypercube dimension limit (2->4: 8, rho rho -> 4 pi: 14) */
+#define GRA_VEGAS_BINS 128 /* VEGASPARAM::BINS default, include/Graniitti/MVEGAS.h:19 */
+
+/* Processes (sub-process registry, include/Graniitti/MSubProc.h) */
+#define GRA_PROC_REGGE_CON 15  /* PROC_15 "PP[CON]" 2-body: MRegge::ME4, include/Graniitti/MSubProc.h:361-380 */
+#define GRA_PROC_TENSOR_CON 12 /* PROC_12 "PP[CONTENSOR]" 2xV: MTensorPomeron::ME4, MSubProc.h:288-300 */
+
+/* Amplitude arithmetic */
+#define GRA_AMP_FP64 0 /* everything FP64, operation order of the reference */
+#define GRA_AMP_FP32 1 /* FP64 kinematics/invariants/Jacobian, FP32 complex amplitude + compensated sums */
+
+/* Per-point flag bits (AuxIntData, include/Graniitti/MProcess.h:37-52) */
+#define GRA_FLAG_KIN_OK 0x01u /* kinematics_ok */
+#define GRA_FLAG_FID_OK 0x02u /* fidcuts_ok   */
+#define GRA_FLAG_VETO_OK 0x04u /* vetocuts_ok */
+#define GRA_FLAG_AMP_OK 0x08u /* amplitude_ok (finite weight, CheckInfNan) */
+
+/* ------------------------------------------------------------------------- */
+/* Configuration (plain old data, passed by value through the ABI)           */
+/* ------------------------------------------------------------------------- */
+typedef struct gra_cfg {
+  int32_t abi_size; /* = sizeof(gra_cfg), checked by every entry point */
+  int32_t process;  /* GRA_PROC_* */
+  int32_t amp_precision; /* GRA_AMP_* */
+  int32_t dim;      /* hypercube dimension (output of gra_hip_cfg_default) */
+
+  /* Beams (MProcess::SetBeamEnergies, src/MProcess.cc:313-334) */
+  double beam_E1, beam_E2; /* GeV */
+  double m_beam;           /* proton mass, modeldata/mass_width_2018.mcd:169 */
+
+  /* Central particles: first-level decaytree legs */
+  double m_central;     /* pole mass of each central particle */
+  double w_central;     /* full width (0 => fixed mass, GetOffShellMass src/MProcess.cc:755) */
+  double offshell;      /* BW window in widths, MProcess::OFFSHELL (src/MProcess.h:363) */
+  double m_daughter;    /* decay daughter mass (rho -> pi pi); 0 if no decay */
+  double S_factor;      /* identical-particle factor (CalculateSymmetryFactor) */
+
+  /* Generation cuts (GENCUT, include/Graniitti/MKinematics.h:1554-1576) */
+  double fwd_pt_min, fwd_pt_max; /* forward leg pt; defaults 0, 2.5 (src/MProcess.cc:1369-1386) */
+  double kt_min, kt_max;         /* intermediate kt; default 0, 10/Kf (src/MContinuum.cc:71-82) */
+  double rap_min, rap_max;       /* central final state rapidity */
+
+  /* Fiducial cuts (FIDCUT, include/Graniitti/MKinematics.h:1579-1611) */
+  int32_t fid_active;
+  int32_t pad0;
+  double fid_eta_min, fid_eta_max, fid_rap_min, fid_rap_max;
+  double fid_pt_min, fid_pt_max, fid_et_min, fid_et_max;
+  double sys_M_min, sys_M_max, sys_Y_min, sys_Y_max, sys_Pt_min, sys_Pt_max;
+  double fwd_t_min, fwd_t_max, fwd_M_min, fwd_M_max;
+
+  /* Soft model (PARAM_SOFT, modeldata/TUNE0/GENERAL.json) */
+  double delta_P, alpha_P, gN_P, fc1, fc2, fc3;
+  double mpi_const; /* PDG::mpi used in the pion loop of S3PomAlpha, include/Graniitti/MPDG.h:56 */
+
+  /* Regge continuum (PARAM_REGGE + PARAM_PI) */
+  double regge_a0[3], regge_ap[3], regge_sgn[3], regge_c[3];
+  int32_t regge_on[3];
+  int32_t offshellFF; /* 0 = exp, 1 = orear, 2 = pow (src/MRegge.cc:1155-1174) */
+  double regge_s0, b_EXP, a_OREAR, b_OREAR, b_POW;
+  double m_prop; /* off-shell propagator nominal mass, decaytree[0].p.mass (src/MRegge.cc:396) */
+
+  /* Tensor pomeron (PARAM_TENSORPOM) */
+  double tp_delta_P, tp_ap_P, tp_gPNN, tp_gPvv[2];
+  double tp_mp_ff; /* proton mass used by the Kelly/F1 form factor */
+
+  /* Eikonal screening (MEikonalNumerics, include/Graniitti/MEikonal.h:26-57) */
+  int32_t screening;
+  int32_t n_loop_phi; /* intervals, multiple of 3 */
+  int32_t n_loop_kt;  /* intervals, multiple of 3 */
+  int32_t n_bt;       /* NumberBT: density table intervals (modeldata/NUMERICS.json) */
+  int32_t n_kt2;      /* NumberKT2: screening table intervals */
+  int32_t n_fb;       /* FBIntegralN: Fourier-Bessel quadrature intervals (even) */
+  double min_loop_kt, max_loop_kt;
+} gra_cfg;
+
+/* ------------------------------------------------------------------------- */
+/* VEGAS                                                                     */
+/* ------------------------------------------------------------------------- */
+/* Grid: bin upper edges, dimension-major: xmat[j * GRA_VEGAS_BINS + i] is the
+ * reference's VD.xmat[i][j] (include/Graniitti/MVEGAS.h:212-214). */
+typedef struct gra_grid {
+  int32_t dim;
+  int32_t bins;
+  double xmat[GRA_MAX_DIM * GRA_VEGAS_BINS];
+} gra_grid;
+
+/* One VEGAS iteration's accumulators (the locked section of
+ * MGraniitti::VEGASMultiThread, src/MGraniitti.cc:1389-1419). Sums are kept as
+ * (value, compensation) pairs; fmat/f2mat are dimension-major like xmat. */
+typedef struct gra_iter_stats {
+  double fsum, fsum_c, f2sum, f2sum_c;
+  double maxW, maxf;
+  uint64_t n_calls, n_kin_ok, n_fid_ok, n_valid, n_nan;
+  double fmat[GRA_MAX_DIM * GRA_VEGAS_BINS];
+  double f2mat[GRA_MAX_DIM * GRA_VEGAS_BINS];
+} gra_iter_stats;
+
+/* Fixed, clock-independent schedule for MGraniitti::SampleVegas
+ * (src/MGraniitti.cc:1074-1135). */
+typedef struct gra_vegas_cfg {
+  uint64_t ncall;        /* calls per iteration */
+  uint64_t seed;
+  int32_t burnin_iter;   /* init = 0 iterations (grid adaptation, integral discarded) */
+  int32_t iter;          /* init = 1 iterations (integral accumulated) */
+  int32_t max_iter;      /* cap when extending iterations for precision (>= iter) */
+  int32_t bins;          /* <= GRA_VEGAS_BINS, even */
+  double lambda;         /* VEGASPARAM::LAMBDA */
+  double precision;      /* stop when sigma_err/sigma <= precision (0 = fixed schedule) */
+  double chi2max;        /* VEGASPARAM::CHI2MAX */
+} gra_vegas_cfg;
+
+typedef struct gra_vegas_result {
+  double sigma, sigma_err, chi2;   /* barn */
+  double maxW, maxf;
+  double seconds;                  /* wall time of the init=1 phase (incl. host grid updates) */
+  double seconds_total;            /* burn-in + main */
+  double seconds_to_precision;     /* wall time until sigma_err/sigma <= precision, or -1 */
+  uint64_t calls_total;
+  int32_t iterations;              /* init = 1 iterations done */
+  int32_t pad0;
+  double kernel_ms;                /* sum of integrand-kernel durations (HIP events) */
+} gra_vegas_result;
+
+/* ------------------------------------------------------------------------- */
+/* Normalizing flow (rational-quadratic spline coupling layers)              */
+/* ------------------------------------------------------------------------- */
+typedef struct gra_flow_cfg {
+  int32_t dim;       /* = integrand dimension */
+  int32_t layers;    /* coupling layers */
+  int32_t bins;      /* spline bins K */
+  int32_t hidden;    /* MLP width (multiple of 32) */
+  int32_t depth;     /* hidden layers per MLP (>= 1) */
+  int32_t steps;     /* Adam steps */
+  uint64_t batch;    /* points per training step */
+  double lr;         /* Adam learning rate */
+  double beta1, beta2, eps;
+  uint64_t seed;
+  double ess_fallback; /* ESS fraction below which sampling falls back to VEGAS */
+} gra_flow_cfg;
+
+typedef struct gra_flow_stats {
+  double sigma, sigma_err;  /* plain mean of w = f/q and its standard error (barn) */
+  double cv2;               /* Var(w)/<w>^2 */
+  double ess;               /* (sum w)^2 / (N sum w^2) */
+  double maxw;
+  double loss;              /* last training-loss value */
+  double seconds;           /* wall time of the call */
+  uint64_t n;               /* points */
+  int32_t fallback;         /* 1 if ESS < ess_fallback */
+  int32_t pad0;
+} gra_flow_stats;
+
+/* ------------------------------------------------------------------------- */
+/* Entry points                                                              */
+/* ------------------------------------------------------------------------- */
+
+/* Fill a configuration with one of the preset workloads:
+ *   1: PP[CON]<C> -> pi+ pi-, 13 TeV, |eta|<0.9, pT>0.15, no screening, FP64
+ *   2: preset 1 + eikonal screening (16 phi x 31 kt nodes, MaxLoopKT 3), FP32 amplitude
+ *   3: PP[CONTENSOR]<C> -> rho0 rho0 -> 4 pi, |eta|<2.5, pT>0.1, screening, FP32
+ * Replaces the steering-card reader MGraniitti::ReadInput (src/MGraniitti.cc:318-329)
+ * for the keys these workloads use, with TUNE0 values (modeldata/TUNE0/GENERAL.json). */
+int gra_hip_cfg_default(gra_cfg* cfg, int32_t preset);
+
+/* Create a handle: uploads parameters, builds the eikonal tables on the GPU if
+ * screening is on (MEikonal::S3Constructor, src/MEikonal.cc:46-132), and sets
+ * up the process (MContinuum ctor + post_Constructor, src/MContinuum.cc:43-89). */
+int gra_hip_create(const gra_cfg* cfg, gra_handle** out);
+
+/* Release every device resource of the handle. */
+void gra_hip_destroy(gra_handle* h);
+
+/* Message of the last failed call on h (or of the last failed create when h is NULL). */
+const char* gra_hip_last_error(const gra_handle* h);
+
+/* Batched MContinuum::EventWeight(randvec, aux) (src/MContinuum.cc:164-193;
+ * include/Graniitti/MProcess.h:83-84): u is n x dim row-major in [0,1); W[i] is
+ * the weight in barn, amp2[i] (nullable) the screened |M|^2, flags[i] (nullable)
+ * the AuxIntData bits. Host pointers. */
+int gra_hip_eval_points(gra_handle* h, const double* u, size_t n, double* W, double* amp2,
+                        uint8_t* flags);
+
+/* Same as gra_hip_eval_points with device pointers, on the handle's stream. */
+int gra_hip_eval_points_device(gra_handle* h, const double* d_u, size_t n, double* d_W,
+                               double* d_amp2, uint8_t* d_flags);
+
+/* Initialize a uniform grid (VEGASData::Init(0) + InitGridDependent,
+ * include/Graniitti/MVEGAS.h:38-84). */
+int gra_hip_grid_init(gra_grid* grid, int32_t dim, int32_t bins);
+
+/* Grid update from one iteration's histograms (VEGASData::OptimizeGrid,
+ * include/Graniitti/MVEGAS.h:107-142). f2mat is consumed (smoothed in place). */
+int gra_hip_grid_optimize(gra_grid* grid, gra_iter_stats* st, double lambda);
+
+/* One VEGAS iteration on the GPU: n_calls points drawn with Philox4x32-10 keyed
+ * by (seed, iter, point), mapped through the grid, weighted and accumulated
+ * (MGraniitti::VEGASMultiThread, src/MGraniitti.cc:1341-1443). */
+int gra_hip_vegas_iterate(gra_handle* h, const gra_grid* grid, uint64_t n_calls, uint64_t seed,
+                          uint32_t iter, gra_iter_stats* out);
+
+/* Full VEGAS integration with a fixed schedule (MGraniitti::SampleVegas + VEGAS,
+ * src/MGraniitti.cc:1074-1338). grid is in/out. */
+int gra_hip_vegas_run(gra_handle* h, const gra_vegas_cfg* vc, gra_grid* grid,
+                      gra_vegas_result* res);
+
+/* Points drawn from a grid (no weights): u -> (x, vegas weight, bin) for tests
+ * of the map (src/MGraniitti.cc:1357-1375). Host pointers; x is n x dim, vw n,
+ * bins n x dim (int32, 1-based as in the reference). */
+int gra_hip_vegas_map(gra_handle* h, const gra_grid* grid, const double* u, size_t n, double* x,
+                      double* vw, int32_t* bins);
+
+/* Hit-or-miss unweighting of n weights against wmax (MGraniitti::SaveEvent,
+ * src/MGraniitti.cc:1651-1663): accept[i] = U_i < w_i / wmax with U from
+ * Philox keyed by (seed, i). Returns accepted count and overflow count. */
+int gra_hip_unweight(gra_handle* h, const double* w, size_t n, double wmax, uint64_t seed,
+                     uint8_t* accept, uint64_t* n_accepted, uint64_t* n_overflow);
+
+/* Copy the screening-node eikonal amplitudes A_eik(kt_j^2) (complex, interleaved
+ * re,im; n_loop_kt+1 values) and the density / amplitude tables (MBT, MSA of
+ * src/MEikonal.cc:46-132; x, re, im triplets, n+1 rows) to host. */
+int gra_hip_eikonal_tables(gra_handle* h, double* node_amp, double* mbt, double* msa);
+
+/* Create / train the flow sampler in the NEURO integrator slot
+ * (MGraniitti::CallIntegrator, src/MGraniitti.cc:1056-1071; SampleNeuro 1527-1648). */
+int gra_hip_flow_init(gra_handle* h, const gra_flow_cfg* fc);
+int gra_hip_flow_train(gra_handle* h, int32_t steps, gra_flow_stats* last);
+/* Sample n points from the flow, evaluate the integrand and report sigma, CV^2,
+ * ESS; W (nullable, device size n) receives w = f/q. */
+int gra_hip_flow_sample(gra_handle* h, uint64_t n, uint64_t seed, gra_flow_stats* st);
+
+/* Parameter access for tests: host copy in/out of all flow parameters (FP32,
+ * packed per layer: W1,b1,...,Wout,bout) and of their gradients. */
+int gra_hip_flow_param_count(gra_handle* h, uint64_t* count);
+int gra_hip_flow_get_params(gra_handle* h, float* params);
+int gra_hip_flow_set_params(gra_handle* h, const float* params);
+/* Forward of the flow in the sampling direction on caller points: z (n x dim)
+ * -> x (n x dim), logq (n). Host pointers. */
+int gra_hip_flow_forward(gra_handle* h, const float* z, size_t n, float* x, float* logq);
+/* Gradient of L = -sum_i c_i log q(x_i) with respect to every parameter, at
+ * caller points x and coefficients c. Host pointers. */
+int gra_hip_flow_grad(gra_handle* h, const float* x, const float* c, size_t n, float* grad,
+                      float* logq);
+
+/* Accumulated duration of the integrand kernels (HIP events on the handle's
+ * stream) since creation or the last reset, and the number of launches. */
+int gra_hip_kernel_time(gra_handle* h, int32_t which, double* total_ms, uint64_t* launches,
+                        int32_t reset);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GRA_HIP_H */

@@ -1,0 +1,1290 @@
+/*
+ * gra_oracle.c -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Plain-C FP64 CPU restatement of the GRANIITTI (v1.051) event-generation inner
+ * loop, used as the parity checker of the HIP library and as the CPU baseline of
+ * the benchmark. It is never linked into, loaded by, or called from the product
+ * path. Every function cites the reference file:line it follows (paths relative
+ * to the reference root). Floating-point operations follow the reference's order
+ * and are compiled without FMA contraction (the reference is built for baseline
+ * x86-64, Makefile:196-204), so that the FP64 restatement reproduces the
+ * reference's rounding wherever libm agrees.
+ *
+ * Pinning status of each part (what fixes it beyond this text) is listed in the
+ * test files under tests/ and summarised here:
+ *   philox4x32_10          Random123 known-answer vectors (tests/test_oracle_rng.py)
+ *   vegas grid/rebin/opt   golden vectors from the reference's own MVEGAS.h compiled
+ *                          from source (oracle/_ref, tests/golden/vegas_*.json)
+ *   vegas map              exact integration of polynomials through the map
+ *   simpson 3/8, CS rule,  golden vectors from the reference's MMath.h/extmath.hpp
+ *   CheckEMC, BESSJ0       (oracle/_ref) + exactness on polynomials + scipy j0
+ *   SolvePz                residuals of the conservation equations + mpmath root
+ *   kinematics/scalars     4-momentum conservation, on-shell, invariants vs boosts
+ *   phase-space weight     FLATAMP=4 volume vs analytic 2->4 phase space (brute force)
+ *   Regge ME4              ReggeEtaLinear vs ReggeEta (reference testbench3 5% check),
+ *                          |P| = |eta| (s/s0)^alpha, t<->u symmetry, s-scaling
+ *   eikonal Fourier-Bessel Hankel pair of a Gaussian (closed form), alpha_P(0)=1+Delta
+ *   screening loop         Simpson exactness on polynomial node functions;
+ *                          zero-table limit equals the bare amplitude
+ *   sigma_tot/sigma_el      ballpark of TOTEM 13 TeV (sanity, not a pin)
+ *   tensor-pomeron ME4     see the tensor section (Lorentz invariance, completeness)
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/gra_hip.h"
+
+#define ORA_EXPORT __attribute__((visibility("default")))
+
+static const double PI = 3.141592653589793238462643383279502884197169399375105820974944;
+
+/* ------------------------------------------------------------------------- */
+/* Complex numbers with the arithmetic of std::complex<double> (libstdc++)   */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+  double re, im;
+} cx;
+
+static inline cx cx_make(double re, double im) {
+  cx z = {re, im};
+  return z;
+}
+static inline cx cx_add(cx a, cx b) { return cx_make(a.re + b.re, a.im + b.im); }
+static inline cx cx_sub(cx a, cx b) { return cx_make(a.re - b.re, a.im - b.im); }
+/* complex * complex, finite case of __muldc3 */
+static inline cx cx_mul(cx a, cx b) {
+  return cx_make(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+/* complex * real and real * complex are component-wise in libstdc++ */
+static inline cx cx_scale(cx a, double d) { return cx_make(a.re * d, a.im * d); }
+static inline cx cx_div_real(cx a, double d) { return cx_make(a.re / d, a.im / d); }
+/* std::exp(complex) -> glibc cexp, finite arguments */
+static inline cx cx_exp(cx a) {
+  const double e = exp(a.re);
+  return cx_make(e * cos(a.im), e * sin(a.im));
+}
+/* gra::math::abs2 = pow2(std::abs(z)), include/Graniitti/MMath.h:253 */
+static inline double cx_abs2(cx a) {
+  const double h = hypot(a.re, a.im);
+  return h * h;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), counter-based RNG      */
+/* ------------------------------------------------------------------------- */
+ORA_EXPORT void ora_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2],
+                                  uint32_t out[4]) {
+  uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+  uint32_t k0 = key_in[0], k1 = key_in[1];
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    c0 = hi1 ^ c1 ^ k0;
+    c1 = lo1;
+    c2 = hi0 ^ c3 ^ k1;
+    c3 = lo0;
+  }
+  out[0] = c0;
+  out[1] = c1;
+  out[2] = c2;
+  out[3] = c3;
+}
+
+/* 53-bit uniform in [0,1) from two 32-bit words */
+static inline double u53(uint32_t hi, uint32_t lo) {
+  const uint64_t x = (((uint64_t)hi << 32) | lo) >> 11;
+  return (double)x * (1.0 / 9007199254740992.0);
+}
+
+/* Uniforms of point `idx` in iteration `iter`: counter = (block, idx_lo, idx_hi, iter),
+ * key = seed; each Philox block yields two doubles. */
+ORA_EXPORT void ora_point_uniforms(uint64_t seed, uint32_t iter, uint64_t idx, int dim,
+                                   double* u) {
+  const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  for (int b = 0; 2 * b < dim; ++b) {
+    const uint32_t ctr[4] = {(uint32_t)b, (uint32_t)idx, (uint32_t)(idx >> 32), iter};
+    uint32_t o[4];
+    ora_philox4x32_10(ctr, key, o);
+    u[2 * b] = u53(o[0], o[1]);
+    if (2 * b + 1 < dim) u[2 * b + 1] = u53(o[2], o[3]);
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* VEGAS grid (include/Graniitti/MVEGAS.h)                                   */
+/* ------------------------------------------------------------------------- */
+#define XM(g, i, j) ((g)->xmat[(j)*GRA_VEGAS_BINS + (i)])
+
+/* VEGASData::Rebin, include/Graniitti/MVEGAS.h:87-104 */
+static void vg_rebin(gra_grid* g, double ac, int j, const double* rvec) {
+  const int BINS = g->bins;
+  unsigned int k = 0;
+  double dr = 0.0, zn = 0.0, zo = 0.0;
+  double xcache[GRA_VEGAS_BINS];
+  for (int i = 0; i < BINS - 1; ++i) {
+    while (ac > dr) { dr += rvec[(++k) - 1]; }
+    if (k > 1) { zo = XM(g, k - 2, j); }
+    zn = XM(g, k - 1, j);
+    dr -= ac;
+    xcache[i] = zn - (zn - zo) * dr / rvec[k - 1];
+  }
+  for (int i = 0; i < BINS - 1; ++i) { XM(g, i, j) = xcache[i]; }
+  XM(g, BINS - 1, j) = 1.0;
+}
+
+/* VEGASData::ClearAll + InitGridDependent with BINS_prev = 1,
+ * include/Graniitti/MVEGAS.h:72-84, 163-187 */
+ORA_EXPORT int ora_grid_init(gra_grid* g, int dim, int bins) {
+  if (dim < 1 || dim > GRA_MAX_DIM || bins < 2 || bins > GRA_VEGAS_BINS || (bins % 2)) return -1;
+  memset(g, 0, sizeof(*g));
+  g->dim = dim;
+  g->bins = bins;
+  for (int j = 0; j < dim; ++j) XM(g, 0, j) = 1.0;
+  double rvec[GRA_VEGAS_BINS];
+  for (int i = 0; i < GRA_VEGAS_BINS; ++i) rvec[i] = 1.0;
+  for (int j = 0; j < dim; ++j) vg_rebin(g, 1.0 / (double)bins, j, rvec);
+  return 0;
+}
+
+/* VEGASData::OptimizeGrid, include/Graniitti/MVEGAS.h:107-142 (f2mat consumed) */
+ORA_EXPORT void ora_grid_optimize(gra_grid* g, double* f2mat, double lambda) {
+  const int BINS = g->bins;
+  const double EPS = 1.0e-30;
+  double dcache[GRA_MAX_DIM];
+  double rvec[GRA_VEGAS_BINS];
+#define F2(i, j) f2mat[(j)*GRA_VEGAS_BINS + (i)]
+  for (int j = 0; j < g->dim; ++j) {
+    double zo = F2(0, j), zn = F2(1, j), ac;
+    F2(0, j) = (zo + zn) / 2.0;
+    dcache[j] = F2(0, j);
+    for (int i = 2; i < BINS; ++i) {
+      ac = zo + zn;
+      zo = zn;
+      zn = F2(i, j);
+      F2(i - 1, j) = (ac + zn) / 3.0;
+      dcache[j] += F2(i - 1, j);
+    }
+    F2(BINS - 1, j) = (zo + zn) / 2.0;
+    dcache[j] += F2(BINS - 1, j);
+  }
+  for (int j = 0; j < g->dim; ++j) {
+    double ac = 0.0;
+    for (int i = 0; i < BINS; ++i) {
+      F2(i, j) = (F2(i, j) < EPS) ? EPS : F2(i, j);
+      rvec[i] = pow((1.0 - F2(i, j) / dcache[j]) / (log(dcache[j]) - log(F2(i, j)) + EPS), lambda);
+      if (isnan(rvec[i]) || isinf(rvec[i])) { rvec[i] = EPS; }
+      ac += rvec[i];
+    }
+    vg_rebin(g, ac / BINS, j, rvec);
+  }
+#undef F2
+}
+
+/* Map of one point through the grid, MGraniitti::VEGASMultiThread,
+ * src/MGraniitti.cc:1355-1375. bins are 1-based like indvec. */
+ORA_EXPORT void ora_vegas_map(const gra_grid* g, const double* u, double* x, double* vw_out,
+                              int32_t* bin) {
+  const unsigned int BINS = (unsigned int)g->bins;
+  double vw = 1.0;
+  for (int j = 0; j < g->dim; ++j) {
+    const double zn = u[j] * BINS + 1.0;
+    unsigned int k = (unsigned int)zn;
+    if (k > BINS) k = BINS;
+    if (k < 1) k = 1;
+    double zo, ac;
+    if (k > 1) {
+      zo = XM(g, k - 1, j) - XM(g, k - 2, j);
+      ac = XM(g, k - 2, j) + (zn - k) * zo;
+    } else {
+      zo = XM(g, k - 1, j);
+      ac = (zn - k) * zo;
+    }
+    x[j] = 0.0 + ac * 1.0; /* region[j] + ac * dxvec[j] on [0,1]^D */
+    vw *= zo * BINS;
+    bin[j] = (int32_t)k;
+  }
+  *vw_out = vw;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Lorentz 4-vectors, include/Graniitti/M4Vec.h (storage order E, x, y, z)   */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+  double e, x, y, z;
+} v4;
+
+static inline double msqrt(double x) { return sqrt(x > 0.0 ? x : 0.0); }
+static inline v4 v4_make(double x, double y, double z, double e) {
+  v4 v = {e, x, y, z};
+  return v;
+}
+static inline v4 v4_add(v4 a, v4 b) { return v4_make(a.x + b.x, a.y + b.y, a.z + b.z, a.e + b.e); }
+static inline v4 v4_sub(v4 a, v4 b) { return v4_make(a.x - b.x, a.y - b.y, a.z - b.z, a.e - b.e); }
+static inline v4 v4_scale(v4 a, double d) { return v4_make(a.x * d, a.y * d, a.z * d, a.e * d); }
+static inline double v4_p3mod2(v4 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+static inline double v4_m2(v4 a) { return a.e * a.e - v4_p3mod2(a); }
+static inline double v4_m(v4 a) {
+  const double m2 = v4_m2(a);
+  return (m2 > 0.0) ? msqrt(m2) : -msqrt(-m2);
+}
+static inline double v4_pt2(v4 a) { return a.x * a.x + a.y * a.y; }
+static inline double v4_pt(v4 a) { return msqrt(v4_pt2(a)); }
+static inline double v4_p3mod(v4 a) { return msqrt(v4_p3mod2(a)); }
+static inline double v4_mt(v4 a) { return msqrt(v4_m2(a) + v4_pt2(a)); }
+static inline double v4_eta(v4 a) {
+  return 0.5 * log((v4_p3mod(a) + a.z) / (v4_p3mod(a) - a.z));
+}
+static inline double v4_rap(v4 a) { return 0.5 * log((a.e + a.z) / (a.e - a.z)); }
+static inline double v4_dot(v4 a, v4 b) { return a.e * b.e - (a.x * b.x + a.y * b.y + a.z * b.z); }
+
+/* gra::math::CheckEMC, include/Graniitti/MMath.h:191-198 */
+static inline int check_emc(v4 d) {
+  const double eps = 1e-6;
+  if (fabs(d.x) > eps || fabs(d.y) > eps || fabs(d.z) > eps || d.e > eps) return 0;
+  return 1;
+}
+ORA_EXPORT int ora_check_emc(const double d[4]) {
+  v4 v = {d[0], d[1], d[2], d[3]};
+  return check_emc(v);
+}
+
+/* kinematics::SolvePz3_A (m3 = m4), include/Graniitti/MKinematics.h:33-64 */
+static double solvepz_a(double m3, double pt3, double pt4, double pz5, double E5, double s) {
+  const double a2 = E5 * E5, a3 = pt3 * pt3, a4 = pt4 * pt4, a5 = pz5 * pz5, a6 = m3 * m3;
+  const double rs = sqrt(s);
+  const double c8 = s * a2 * 6.0;
+  const double s32 = pow(s, 3.0 / 2.0);
+  const double a10 = a2 * a2, a11 = a3 * a3, a12 = a4 * a4, a13 = a5 * a5, a14 = s * s;
+  const double c15 = a5 * a6 * 4.0, c16 = a3 * a5 * 2.0, c17 = a4 * a5 * 2.0;
+  const double c18 = E5 * a6 * rs * 8.0, c19 = E5 * a3 * rs * 4.0, c20 = E5 * a4 * rs * 4.0;
+  const double c21 = E5 * a5 * rs * 4.0;
+  const double disc = c8 + a10 + a11 + a12 + a13 + a14 + c15 + c16 + c17 + c18 + c19 + c20 + c21 -
+                      E5 * s32 * 4.0 - s * a3 * 2.0 - s * a4 * 2.0 - s * a5 * 2.0 - s * a6 * 4.0 -
+                      a2 * a3 * 2.0 - a2 * a4 * 2.0 - a2 * a5 * 2.0 - a3 * a4 * 2.0 -
+                      a2 * a6 * 4.0 - E5 * a2 * rs * 4.0;
+  const double sq = sqrt(disc);
+  return pz5 * (-1.0 / 2.0) -
+         (E5 * sq * (1.0 / 2.0) - rs * sq * (1.0 / 2.0) + pz5 * (a3 * (1.0 / 2.0) - a4 * (1.0 / 2.0))) /
+             (s + a2 - a5 - E5 * rs * 2.0);
+}
+
+/* kinematics::SolvePz3_B (general masses), include/Graniitti/MKinematics.h:117-159 */
+static double solvepz_b(double m3, double m4, double pt3, double pt4, double pz5, double E5,
+                        double s) {
+  const double a2 = E5 * E5, a3 = m3 * m3, a4 = m4 * m4, a5 = pt3 * pt3, a6 = pt4 * pt4,
+               a7 = pz5 * pz5;
+  const double rs = sqrt(s);
+  const double c9 = s * a2 * 6.0;
+  const double s32 = pow(s, 3.0 / 2.0);
+  const double a11 = a2 * a2, a12 = a3 * a3, a13 = a4 * a4, a14 = a5 * a5, a15 = a6 * a6,
+               a16 = a7 * a7, a17 = s * s;
+  const double c18 = a3 * a5 * 2.0, c19 = a4 * a6 * 2.0, c20 = a3 * a7 * 2.0, c21 = a4 * a7 * 2.0,
+               c22 = a5 * a7 * 2.0, c23 = a6 * a7 * 2.0;
+  const double c24 = E5 * a3 * rs * 4.0, c25 = E5 * a4 * rs * 4.0, c26 = E5 * a5 * rs * 4.0,
+               c27 = E5 * a6 * rs * 4.0, c28 = E5 * a7 * rs * 4.0;
+  const double disc = c9 + a11 + a12 + a13 + a14 + a15 + a16 + a17 + c18 + c19 + c20 + c21 + c22 +
+                      c23 + c24 + c25 + c26 + c27 + c28 - E5 * s32 * 4.0 - s * a3 * 2.0 -
+                      s * a4 * 2.0 - s * a5 * 2.0 - s * a6 * 2.0 - s * a7 * 2.0 - a2 * a3 * 2.0 -
+                      a2 * a4 * 2.0 - a2 * a5 * 2.0 - a3 * a4 * 2.0 - a2 * a6 * 2.0 -
+                      a2 * a7 * 2.0 - a3 * a6 * 2.0 - a4 * a5 * 2.0 - a5 * a6 * 2.0 -
+                      E5 * a2 * rs * 4.0;
+  const double sq = sqrt(disc);
+  return pz5 * (-1.0 / 2.0) -
+         (E5 * sq * (1.0 / 2.0) - rs * sq * (1.0 / 2.0) +
+          pz5 * (a3 * (1.0 / 2.0) - a4 * (1.0 / 2.0) + a5 * (1.0 / 2.0) - a6 * (1.0 / 2.0))) /
+             (s + a2 - a7 - E5 * rs * 2.0);
+}
+
+/* kinematics::SolvePz, include/Graniitti/MKinematics.h:220-229 */
+ORA_EXPORT double ora_solvepz(double m3, double m4, double pt3, double pt4, double pz5, double E5,
+                              double s) {
+  if (fabs(m3 - m4) < 1e-9) return solvepz_a(m3, pt3, pt4, pz5, E5, s);
+  return solvepz_b(m3, m4, pt3, pt4, pz5, E5, s);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Run constants derived from the configuration                              */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+  gra_cfg c;
+  v4 pbeam1, pbeam2, beamsum;
+  double s, sqrt_s;
+  double GeV2barn;
+  /* screening node tables */
+  int nphi, nkt;
+  double step_kt, step_phi;
+  /* eikonal tables */
+  int n_bt, n_kt2, n_fb;
+  double mbt_min, mbt_max, mbt_step, msa_min, msa_max, msa_step;
+  double *mbt_x, *msa_x; /* n+1 */
+  cx *mbt_y, *msa_y;
+  int tables_ready;
+} ora_run;
+
+/* MParticle masses and units, include/Graniitti/MPDG.h:54-78 */
+static double gev2barn(void) {
+  const double c = 2.99792458E8, hbar = 6.582119514E-25;
+  const double GeV2m = hbar * c;
+  const double GeV2m2 = GeV2m * GeV2m;
+  return GeV2m2 / 1E-28;
+}
+static double gev2fm(void) {
+  const double c = 2.99792458E8, hbar = 6.582119514E-25;
+  return hbar * c * 1E15;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Form factors and Regge factors (src/MForm.cc, src/MRegge.cc)              */
+/* ------------------------------------------------------------------------- */
+/* form::ReggeEta, src/MForm.cc:382-385 */
+static cx regge_eta(double alpha_t, double sigma) {
+  const double denom = sin(PI * alpha_t);
+  /* -zi * PI * alpha_t = (-0, -PI*alpha_t) */
+  const cx e = cx_exp(cx_make(-0.0 * PI * alpha_t, -PI * alpha_t));
+  const cx num = cx_make(1.0 + sigma * e.re, sigma * e.im);
+  return cx_div_real(cx_make(-num.re, -num.im), denom);
+}
+/* form::ReggeEtaLinear, src/MForm.cc:394-396 */
+static cx regge_eta_linear(double t, double a0, double ap, double sigma) {
+  /* -zi * PI / 2.0 * ap * t */
+  const double th = ((-PI / 2.0) * ap) * t;
+  const cx ph = cx_exp(cx_make(((-0.0 * PI / 2.0) * ap) * t, th));
+  return cx_mul(regge_eta(a0, sigma), ph);
+}
+ORA_EXPORT void ora_regge_eta(double alpha_t, double sigma, double* out) {
+  const cx z = regge_eta(alpha_t, sigma);
+  out[0] = z.re;
+  out[1] = z.im;
+}
+ORA_EXPORT void ora_regge_eta_linear(double t, double a0, double ap, double sigma, double* out) {
+  const cx z = regge_eta_linear(t, a0, ap, sigma);
+  out[0] = z.re;
+  out[1] = z.im;
+}
+
+/* form::S3F, src/MForm.cc:433-435 */
+static double s3f(const gra_cfg* c, double t) {
+  return (1.0 / (1 - t / c->fc1)) * (1.0 / (1 - t / c->fc2));
+}
+/* form::S3HPL, src/MForm.cc:416-426 */
+static double s3hpl(const gra_cfg* c, double tau, double t) {
+  const double m = 1.0;
+  const double sqrtau = msqrt(1 + tau);
+  const double F_pi = 1.0 / (1.0 - t / c->fc3);
+  const double mpi = c->mpi_const;
+  return (4.0 / tau) * (F_pi * F_pi) *
+         (2.0 * tau - pow(1.0 + tau, 3.0 / 2.0) * log((sqrtau + 1.0) / (sqrtau - 1.0)) +
+          log((m * m) / (mpi * mpi)));
+}
+/* form::S3PomAlpha, src/MForm.cc:402-413 */
+ORA_EXPORT double ora_pom_alpha(const gra_cfg* c, double t) {
+  const double mpi = c->mpi_const;
+  const double BETApi = (2.0 / 3.0) * c->gN_P;
+  const double PIC = ((BETApi * BETApi) * (mpi * mpi)) / (32.0 * (PI * (PI * PI)));
+  const double h = PIC * s3hpl(c, 4.0 * (mpi * mpi) / fabs(t), t);
+  return 1.0 + c->delta_P + c->alpha_P * t - h;
+}
+
+/* MRegge::PropOnly, src/MRegge.cc:888-906 */
+static cx regge_prop(const gra_cfg* c, double s, double t) {
+  cx M = cx_make(0, 0);
+  for (int k = 0; k < 3; ++k) {
+    if (c->regge_on[k]) {
+      const double alpha = c->regge_a0[k] + c->regge_ap[k] * t;
+      const cx eta = regge_eta_linear(t, c->regge_a0[k], c->regge_ap[k], c->regge_sgn[k]);
+      M = cx_add(M, cx_scale(eta, pow(s / c->regge_s0, alpha)));
+    }
+  }
+  return M;
+}
+ORA_EXPORT void ora_regge_prop(const gra_cfg* c, double s, double t, double* out) {
+  const cx z = regge_prop(c, s, t);
+  out[0] = z.re;
+  out[1] = z.im;
+}
+/* PARAM_REGGE::Meson_FF / Meson_prop, src/MRegge.cc:1155-1207 */
+static double meson_ff(const gra_cfg* c, double t_hat, double M2) {
+  const double tprime = t_hat - M2;
+  switch (c->offshellFF) {
+    case 0: return exp(-c->b_EXP * fabs(tprime));
+    case 1:
+      return exp(-c->b_OREAR * msqrt(fabs(tprime) + c->a_OREAR * c->a_OREAR) +
+                 c->a_OREAR * c->b_OREAR);
+    default: return 1.0 / (1.0 - tprime / c->b_POW);
+  }
+}
+static double meson_prop(double t_hat, double M2) { return 1.0 / (t_hat - M2); }
+
+/* ------------------------------------------------------------------------- */
+/* Kinematic state (LORENTZSCALAR subset, include/Graniitti/MKinematics.h)   */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+  v4 pf[5];      /* pfinal: 0 system, 1,2 forward, 3,4 central */
+  v4 pf_orig[5]; /* pfinal_orig */
+  v4 dt[2];      /* decaytree[i].p4 (tree level) */
+  v4 legs[2][2]; /* decay daughters of decaytree[i] (rho -> pi pi) */
+  v4 pkt;        /* pkt_[0] */
+  v4 q1, q2;
+  double ss[5][5];
+  double s1, s2, t1, t2, t_hat, u_hat, m2, Y, Pt;
+  double m_off[2];
+} kin_t;
+
+/* MProcess::GetLorentzScalars(Nf=4), src/MProcess.cc:1499-1584 (fields used by
+ * the amplitudes and the cuts) */
+static int lorentz_scalars(const ora_run* R, kin_t* K) {
+  for (int i = 0; i <= 4; ++i) {
+    for (int j = 0; j <= 4; ++j) {
+      if (i < j) K->ss[i][j] = v4_m2(v4_add(K->pf[i], K->pf[j]));
+      if (K->ss[i][j] < 0) return 0;
+    }
+  }
+  for (int i = 0; i <= 4; ++i)
+    for (int j = i; j <= 4; ++j)
+      if (i != j) K->ss[j][i] = K->ss[i][j];
+  K->s1 = K->ss[1][0];
+  K->s2 = K->ss[2][0];
+  if (K->s1 < 0 || K->s2 < 0) return 0;
+  K->q1 = v4_sub(R->pbeam1, K->pf[1]);
+  K->q2 = v4_sub(R->pbeam2, K->pf[2]);
+  K->t1 = v4_m2(K->q1);
+  K->t2 = v4_m2(K->q2);
+  if (K->t1 > 0 || K->t2 > 0) return 0;
+  K->t_hat = v4_m2(v4_sub(K->q1, K->dt[0]));
+  K->u_hat = v4_m2(v4_sub(K->q1, K->dt[1]));
+  K->m2 = v4_m2(K->pf[0]);
+  K->Y = v4_rap(K->pf[0]);
+  K->Pt = v4_pt(K->pf[0]);
+  return 1;
+}
+
+/* MContinuum::BLinearSystem for Kf = 2, src/MContinuum.cc:438-550:
+ * A = [[1/2, 0], [0, 1/2]], b0 = q0 - w, b1 = -q0 - w, w = p1f + p2f */
+static void blinear2(v4 p[2], v4 q, v4 p1f, v4 p2f) {
+  const v4 w = v4_add(p1f, p2f);
+  const v4 b0 = v4_sub(q, w);
+  const v4 b1 = v4_sub(v4_scale(q, -1.0), w);
+  const double A[2][2] = {{1.0 / 2.0, 0.0}, {0.0, 1.0 / 2.0}};
+  for (int i = 0; i < 2; ++i) {
+    v4 acc = v4_make(0, 0, 0, 0);
+    acc = v4_add(acc, v4_scale(b0, A[i][0]));
+    acc = v4_add(acc, v4_scale(b1, A[i][1]));
+    p[i] = acc;
+  }
+}
+
+/* kinematics::LorentzBoost(mother, M0, p, sign=+1) for the rho -> pi pi decay,
+ * include/Graniitti/MKinematics.h:384-401 */
+static v4 lorentz_boost(v4 boost, double M0, v4 p) {
+  const double bx = boost.x / boost.e, by = boost.y / boost.e, bz = boost.z / boost.e;
+  const double gamma = boost.e / M0;
+  const double kappa1 = bx * p.x + by * p.y + bz * p.z;
+  const double kappa2 = gamma * (gamma * kappa1 / (1.0 + gamma) + p.e);
+  return v4_make(p.x + kappa2 * bx, p.y + kappa2 * by, p.z + kappa2 * bz, gamma * (p.e + kappa1));
+}
+
+/* MContinuum::BNRandomKin + BNBuildKin for Kf = 2 central particles,
+ * src/MContinuum.cc:276-435. Hidden variables of the rho decays (config 3) are
+ * taken from x[8..13] (see the tensor section). Returns kinematics_ok. */
+static int build_kin(const ora_run* R, const double* x, kin_t* K) {
+  const gra_cfg* c = &R->c;
+  const double ZERO_EPS = 1e-12;
+  const double lmin = log(c->fwd_pt_min + ZERO_EPS), lmax = log(c->fwd_pt_max);
+  const double u1 = lmin + (lmax - lmin) * x[0];
+  const double u2 = lmin + (lmax - lmin) * x[1];
+  const double pt1 = exp(u1), pt2 = exp(u2);
+  const double phi1 = 2.0 * PI * x[2], phi2 = 2.0 * PI * x[3];
+
+  /* GetOffShellMass, src/MProcess.cc:753-812, with the BW draw lifted to x[8], x[9]
+   * (RelativisticBWRandom, src/MRandom.cc:82-104) */
+  for (int i = 0; i < 2; ++i) {
+    if (c->w_central <= 0.0) {
+      K->m_off[i] = c->m_central;
+    } else {
+      const double m0 = c->m_central, G = c->w_central, L = c->offshell;
+      const double dmin = 2.0 * c->m_daughter + 1e-5; /* daughters at pole + safe margin */
+      const double m2 = m0 * m0;
+      const double m2max = (m0 + L * G) * (m0 + L * G);
+      double lo = dmin * dmin;
+      if (lo < 0.0) lo = 0.0;
+      const double m2min = fmax(lo, (m0 - L * G) * (m0 - L * G));
+      const double a1 = atan2(m2min - m2, m0 * G), a2 = atan2(m2max - m2, m0 * G);
+      const double m2val = m2 + m0 * G * tan(a1 + x[8 + i] * (a2 - a1));
+      K->m_off[i] = fmax(dmin, msqrt(m2val));
+    }
+  }
+  const double kt = c->kt_min + (c->kt_max - c->kt_min) * x[4];
+  const double phi = 2.0 * PI * x[5];
+  const double y[2] = {c->rap_min + (c->rap_max - c->rap_min) * x[6],
+                       c->rap_min + (c->rap_max - c->rap_min) * x[7]};
+  const double m1 = c->m_beam, m2 = c->m_beam;
+
+  v4 p1 = v4_make(pt1 * cos(phi1), pt1 * sin(phi1), 0, 0);
+  v4 p2 = v4_make(pt2 * cos(phi2), pt2 * sin(phi2), 0, 0);
+  K->pkt = v4_make(kt * cos(phi), kt * sin(phi), 0, 0);
+  v4 p[2];
+  blinear2(p, K->pkt, p1, p2);
+  v4 sumP = v4_make(0, 0, 0, 0);
+  for (int i = 0; i < 2; ++i) {
+    const double m = K->m_off[i];
+    const double mt = msqrt(m * m + v4_pt2(p[i]));
+    p[i].z = mt * sinh(y[i]);
+    p[i].e = mt * cosh(y[i]);
+    sumP = v4_add(sumP, p[i]);
+  }
+  if (sumP.e > R->sqrt_s) return 0;
+  const double p1z = ora_solvepz(m1, m2, pt1, pt2, sumP.z, sumP.e, R->s);
+  const double p2z = -(sumP.z + p1z);
+  if (p1z < 0 || p2z > 0) return 0;
+  p1.z = p1z;
+  p1.e = msqrt(m1 * m1 + pt1 * pt1 + p1z * p1z);
+  p2.z = p2z;
+  p2.e = msqrt(m2 * m2 + pt2 * pt2 + p2z * p2z);
+  K->pf[1] = p1;
+  K->pf[2] = p2;
+  K->pf[0] = sumP;
+  double sumM = 0;
+  for (int i = 0; i < 2; ++i) {
+    K->dt[i] = p[i];
+    K->pf[3 + i] = p[i];
+    sumM += v4_m(p[i]);
+  }
+  if (v4_m(sumP) < sumM) return 0;
+  if (!check_emc(v4_sub(R->beamsum, v4_add(v4_add(K->pf[1], K->pf[2]), K->pf[0])))) return 0;
+
+  /* ConstructDecayKinematics, src/MProcess.cc:1231-1283: rho -> pi+ pi-,
+   * daughters at pole mass, isotropic in the rest frame
+   * (TwoBodyPhaseSpace, include/Graniitti/MKinematics.h:405-459), angles lifted to
+   * x[10..13] = (cos theta_1, phi_1, cos theta_2, phi_2) */
+  if (c->m_daughter > 0.0) {
+    for (int i = 0; i < 2; ++i) {
+      const double M0 = v4_m(K->dt[i]);
+      const double md = c->m_daughter;
+      if (M0 < 2.0 * md) return 0;
+      const double pn = msqrt((M0 * M0 - (md + md) * (md + md)) * (M0 * M0 - (md - md) * (md - md))) /
+                        (2.0 * M0);
+      const double ct = 2.0 * x[10 + 2 * i] - 1.0;
+      const double ph = 2.0 * PI * x[11 + 2 * i];
+      const double st = msqrt(1.0 - ct * ct);
+      const double px = pn * st * cos(ph), py = pn * st * sin(ph), pz = pn * ct;
+      const double ed = msqrt(md * md + pn * pn);
+      K->legs[i][0] = lorentz_boost(K->dt[i], M0, v4_make(px, py, pz, ed));
+      K->legs[i][1] = lorentz_boost(K->dt[i], M0, v4_make(-px, -py, -pz, ed));
+    }
+  }
+  return lorentz_scalars(R, K);
+}
+
+/* Fiducial check of one final-state leg, MProcess::FindDecayCuts, src/MProcess.cc:1138-1154 */
+static int leg_cuts(const gra_cfg* c, v4 p) {
+  const double pt = v4_pt(p), et = v4_mt(p), eta = v4_eta(p), rap = v4_rap(p);
+  return (pt >= c->fid_pt_min && pt <= c->fid_pt_max && et >= c->fid_et_min && et <= c->fid_et_max &&
+          eta >= c->fid_eta_min && eta <= c->fid_eta_max && rap >= c->fid_rap_min &&
+          rap <= c->fid_rap_max);
+}
+
+/* MProcess::CommonCuts, src/MProcess.cc:1085-1135 (USERCUTS = 0, elastic legs) */
+static int fiducial_cuts(const ora_run* R, const kin_t* K) {
+  const gra_cfg* c = &R->c;
+  if (!c->fid_active) return 1;
+  if (!(fabs(K->t1) >= c->fwd_t_min && fabs(K->t1) <= c->fwd_t_max && fabs(K->t2) >= c->fwd_t_min &&
+        fabs(K->t2) <= c->fwd_t_max))
+    return 0;
+  const double M = msqrt(K->m2);
+  if (!(M >= c->sys_M_min && M <= c->sys_M_max && K->Y >= c->sys_Y_min && K->Y <= c->sys_Y_max &&
+        K->Pt >= c->sys_Pt_min && K->Pt <= c->sys_Pt_max))
+    return 0;
+  int ok = 1;
+  for (int i = 0; i < 2; ++i) {
+    if (c->m_daughter > 0.0) {
+      if (!leg_cuts(c, K->legs[i][0])) ok = 0;
+      if (!leg_cuts(c, K->legs[i][1])) ok = 0;
+    } else {
+      if (!leg_cuts(c, K->dt[i])) ok = 0;
+    }
+  }
+  return ok;
+}
+
+/* MContinuum::LoopKinematics for Kf = 2, src/MContinuum.cc:94-161 */
+static int loop_kinematics(const ora_run* R, kin_t* K, const double p1p[2], const double p2p[2]) {
+  K->pf[1].x = p1p[0];
+  K->pf[1].y = p1p[1];
+  K->pf[2].x = p2p[0];
+  K->pf[2].y = p2p[1];
+  v4 p[2];
+  blinear2(p, K->pkt, K->pf[1], K->pf[2]);
+  v4 sumP = v4_make(0, 0, 0, 0);
+  for (int i = 0; i < 2; ++i) {
+    const double y = v4_rap(K->pf_orig[i + 3]);
+    const double m = v4_m(K->pf_orig[i + 3]);
+    K->pf[i + 3].x = p[i].x;
+    K->pf[i + 3].y = p[i].y;
+    const double mt = msqrt(m * m + v4_pt2(K->pf[i + 3]));
+    K->pf[i + 3].z = mt * sinh(y);
+    K->pf[i + 3].e = mt * cosh(y);
+    sumP = v4_add(sumP, K->pf[i + 3]);
+  }
+  const double pt1 = v4_pt(K->pf[1]), pt2 = v4_pt(K->pf[2]);
+  const double m1 = v4_m(K->pf_orig[1]), m2 = v4_m(K->pf_orig[2]);
+  const double p1z = ora_solvepz(m1, m2, pt1, pt2, sumP.z, sumP.e, R->s);
+  const double p2z = -(sumP.z + p1z);
+  if (p1z < 0 || p2z > 0) return 0;
+  K->pf[1].z = p1z;
+  K->pf[1].e = msqrt(m1 * m1 + pt1 * pt1 + p1z * p1z);
+  K->pf[2].z = p2z;
+  K->pf[2].e = msqrt(m2 * m2 + pt2 * pt2 + p2z * p2z);
+  K->pf[0] = sumP;
+  if (!check_emc(v4_sub(R->beamsum, v4_add(v4_add(K->pf[1], K->pf[2]), K->pf[0])))) return 0;
+  return lorentz_scalars(R, K);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Amplitudes                                                                */
+/* ------------------------------------------------------------------------- */
+/* MRegge::ME4(lts, sign = +1) with PomPomProtonVertex (elastic legs),
+ * src/MRegge.cc:394-438, 637-644 */
+static cx regge_me4(const ora_run* R, const kin_t* K) {
+  const gra_cfg* c = &R->c;
+  const double M2_ = c->m_prop * c->m_prop;
+  const double FF_A = c->gN_P * s3f(c, K->t1);
+  const double FF_B = c->gN_P * s3f(c, K->t2);
+  const double gpp_P = c->regge_c[0] / c->gN_P;
+  cx A_t = regge_prop(c, K->ss[1][3], K->t1);
+  A_t = cx_scale(A_t, FF_A);
+  A_t = cx_scale(A_t, meson_ff(c, K->t_hat, M2_));
+  A_t = cx_scale(A_t, gpp_P);
+  A_t = cx_scale(A_t, meson_prop(K->t_hat, M2_));
+  A_t = cx_scale(A_t, meson_ff(c, K->t_hat, M2_));
+  A_t = cx_scale(A_t, gpp_P);
+  A_t = cx_mul(A_t, regge_prop(c, K->ss[2][4], K->t2));
+  A_t = cx_scale(A_t, FF_B);
+  cx A_u = cx_scale(regge_prop(c, K->ss[1][4], K->t1), 1.0);
+  A_u = cx_scale(A_u, FF_A);
+  A_u = cx_scale(A_u, meson_ff(c, K->u_hat, M2_));
+  A_u = cx_scale(A_u, gpp_P);
+  A_u = cx_scale(A_u, meson_prop(K->u_hat, M2_));
+  A_u = cx_scale(A_u, meson_ff(c, K->u_hat, M2_));
+  A_u = cx_scale(A_u, gpp_P);
+  A_u = cx_mul(A_u, regge_prop(c, K->ss[2][3], K->t2));
+  A_u = cx_scale(A_u, FF_B);
+  return cx_add(A_t, A_u);
+}
+
+/* Tensor-pomeron ME4 (defined further below). Writes nh helicity amplitudes. */
+static int tensor_me4(const ora_run* R, const kin_t* K, cx* hamp);
+
+/* Helicity amplitudes of the bare process; returns their count. */
+static int bare_amplitudes(const ora_run* R, const kin_t* K, cx* hamp) {
+  if (R->c.process == GRA_PROC_TENSOR_CON) return tensor_me4(R, K, hamp);
+  hamp[0] = regge_me4(R, K);
+  return 1;
+}
+
+/* |M|^2 of the bare process as MSubProc returns it (PROC_15: abs2(A); PROC_12:
+ * sum |A_h|^2 / 4) */
+static double bare_amp2(int nh, const cx* hamp) {
+  if (nh == 1) return cx_abs2(hamp[0]);
+  double a = 0.0;
+  for (int h = 0; h < nh; ++h) a += cx_abs2(hamp[h]);
+  return a / 4;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Eikonal tables (src/MEikonal.cc)                                          */
+/* ------------------------------------------------------------------------- */
+/* BESSJ0, libs/extmath.hpp:12-49 (Abramowitz-Stegun / Clenshaw rational fits) */
+ORA_EXPORT double ora_bessj0(double X) {
+  const double P1 = 1.0, P2 = -0.1098628627E-2, P3 = 0.2734510407E-4, P4 = -0.2073370639E-5,
+               P5 = 0.2093887211E-6, Q1 = -0.1562499995E-1, Q2 = 0.1430488765E-3,
+               Q3 = -0.6911147651E-5, Q4 = 0.7621095161E-6, Q5 = -0.9349451520E-7,
+               R1 = 57568490574.0, R2 = -13362590354.0, R3 = 651619640.7, R4 = -11214424.18,
+               R5 = 77392.33017, R6 = -184.9052456, S1 = 57568490411.0, S2 = 1029532985.0,
+               S3 = 9494680.718, S4 = 59272.64853, S5 = 267.8532712, S6 = 1.0;
+  if (X == 0.0) return 1.0;
+  const double AX = fabs(X);
+  if (AX < 8.0) {
+    const double Y = X * X;
+    const double FR = R1 + Y * (R2 + Y * (R3 + Y * (R4 + Y * (R5 + Y * R6))));
+    const double FS = S1 + Y * (S2 + Y * (S3 + Y * (S4 + Y * (S5 + Y * S6))));
+    return FR / FS;
+  }
+  const double Z = 8.0 / AX;
+  const double Y = Z * Z;
+  const double XX = AX - 0.785398164;
+  const double FP = P1 + Y * (P2 + Y * (P3 + Y * (P4 + Y * P5)));
+  const double FQ = Q1 + Y * (Q2 + Y * (Q3 + Y * (Q4 + Y * Q5)));
+  return sqrt(0.636619772 / AX) * (FP * cos(XX) - Z * FQ * sin(XX));
+}
+
+/* gra::math::CSIntegral, include/Graniitti/MMath.h:378-394 (complex) */
+static cx cs_integral(const cx* f, int N, double h) {
+  cx Ie = cx_make(0, 0), Io = cx_make(0, 0);
+  for (int j = 1; j <= N / 2 - 1; ++j) Ie = cx_add(Ie, f[2 * j]);
+  for (int j = 1; j <= N / 2; ++j) Io = cx_add(Io, f[2 * j - 1]);
+  const cx sum = cx_add(cx_add(cx_add(f[0], cx_scale(Ie, 2.0)), cx_scale(Io, 4.0)), f[N]);
+  return cx_div_real(cx_scale(sum, h), 3.0);
+}
+ORA_EXPORT void ora_cs_integral(const double* f_reim, int N, double h, double* out) {
+  cx* f = (cx*)malloc(sizeof(cx) * (N + 1));
+  for (int i = 0; i <= N; ++i) f[i] = cx_make(f_reim[2 * i], f_reim[2 * i + 1]);
+  const cx r = cs_integral(f, N, h);
+  free(f);
+  out[0] = r.re;
+  out[1] = r.im;
+}
+
+/* Hankel transform (2 pi) ∫ g(k) J0(b k) k dk on the Fourier-Bessel grid of
+ * S3Density (k in [kmin, kmax], N intervals) for a real test function given on
+ * the grid; used to pin the quadrature against closed forms. */
+ORA_EXPORT double ora_fb_transform(const double* g, int N, double kmin, double kmax, double b) {
+  const double h = (kmax - kmin) / N;
+  cx* f = (cx*)malloc(sizeof(cx) * (N + 1));
+  for (int i = 0; i <= N; ++i) {
+    const double k = kmin + i * h;
+    f[i] = cx_make(g[i] * ora_bessj0(b * k) * k, 0.0);
+  }
+  const cx r = cs_integral(f, N, h);
+  free(f);
+  return r.re;
+}
+
+/* MEikonal::SingleAmpElastic(s, t, +1), src/MEikonal.cc:139-154 */
+static cx single_amp_elastic(const ora_run* R, double t) {
+  const gra_cfg* c = &R->c;
+  const double F = s3f(c, t);
+  const double aP = ora_pom_alpha(c, t);
+  const double s0 = 1.0;
+  const cx eta = regge_eta(aP, 1);
+  const double pre = ((c->gN_P * c->gN_P) * F) * F;
+  return cx_scale(cx_scale(eta, pre), pow(R->s / s0, aP));
+}
+
+/* IArray1D::Interpolate1D (linear), src/MEikonal.cc:489-512 */
+static cx interp1d(const double* X, const cx* Y, int N, double MIN, double MAX, double STEP,
+                   double a) {
+  if (a < MIN) a = MIN;
+  (void)MAX;
+  int i = (int)floor((a - MIN) / STEP);
+  if (i < 0) i = 0;
+  if (i >= N) i = N - 1;
+  const cx dy = cx_sub(Y[i + 1], Y[i]);
+  const double dx = X[i + 1] - X[i];
+  return cx_add(Y[i], cx_scale(cx_div_real(dy, dx), a - X[i]));
+}
+
+/* WriteArray/ReadArray round trip: std::setprecision(15) then std::stod,
+ * src/MEikonal.cc:404-485 */
+static double round15(double v) {
+  char buf[64];
+  snprintf(buf, sizeof(buf), "%.15g", v);
+  return strtod(buf, NULL);
+}
+
+typedef struct {
+  ora_run* R;
+  int which; /* 0 = MBT density, 1 = MSA screening */
+  int i0, i1;
+  const cx* pre; /* hoisted per-quadrature-node factors */
+} eik_job;
+
+static void* eik_worker(void* arg) {
+  eik_job* J = (eik_job*)arg;
+  ora_run* R = J->R;
+  const int NF = R->n_fb;
+  cx* f = (cx*)malloc(sizeof(cx) * (NF + 1));
+  for (int i = J->i0; i < J->i1; ++i) {
+    if (J->which == 0) {
+      /* MEikonal::S3Density(bt), src/MEikonal.cc:177-216 */
+      const double bt = R->mbt_min + i * R->mbt_step;
+      const double kmin = 1E-9, kmax = 30.0;
+      const double kstep = (kmax - kmin) / NF;
+      for (int q = 0; q <= NF; ++q) {
+        const double kt = kmin + q * kstep;
+        f[q] = cx_scale(cx_scale(J->pre[q], ora_bessj0(bt * kt)), kt);
+      }
+      const double FACTOR = 1 / (2.0 * PI * R->s);
+      R->mbt_y[i] = cx_scale(cs_integral(f, NF, kstep), FACTOR);
+    } else {
+      /* MEikonal::S3Screening(kt2), src/MEikonal.cc:222-245 */
+      const double kt2 = R->msa_min + i * R->msa_step;
+      const double STEP = (R->mbt_max - R->mbt_min) / NF;
+      const double kt = msqrt(kt2);
+      for (int q = 0; q <= NF; ++q) {
+        const double bt = R->mbt_min + q * STEP;
+        f[q] = cx_scale(cx_scale(J->pre[q], ora_bessj0(bt * kt)), bt);
+      }
+      const double FACTOR = 2.0 * PI;
+      R->msa_y[i] = cx_scale(cs_integral(f, NF, STEP), (2.0 * R->s) * FACTOR);
+    }
+  }
+  free(f);
+  return NULL;
+}
+
+static void run_eik_jobs(ora_run* R, int which, int n, const cx* pre, int nthreads) {
+  if (nthreads < 1) nthreads = 1;
+  pthread_t th[64];
+  eik_job jobs[64];
+  if (nthreads > 64) nthreads = 64;
+  for (int t = 0; t < nthreads; ++t) {
+    jobs[t].R = R;
+    jobs[t].which = which;
+    jobs[t].i0 = (int)((long long)n * t / nthreads);
+    jobs[t].i1 = (int)((long long)n * (t + 1) / nthreads);
+    jobs[t].pre = pre;
+    pthread_create(&th[t], NULL, eik_worker, &jobs[t]);
+  }
+  for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+}
+
+/* MEikonal::S3Constructor tables, src/MEikonal.cc:46-132 */
+static void build_tables(ora_run* R, int nthreads) {
+  const int NF = R->n_fb;
+  cx* pre = (cx*)malloc(sizeof(cx) * (NF + 1));
+  /* MBT: f = SingleAmpElastic(s, -kt^2) * J0(bt kt) * kt; the amplitude does not
+   * depend on bt and is evaluated once per quadrature node. */
+  {
+    const double kmin = 1E-9, kmax = 30.0, kstep = (kmax - kmin) / NF;
+    for (int q = 0; q <= NF; ++q) {
+      const double kt = kmin + q * kstep;
+      pre[q] = single_amp_elastic(R, -(kt * kt));
+    }
+  }
+  run_eik_jobs(R, 0, R->n_bt + 1, pre, nthreads);
+  for (int i = 0; i <= R->n_bt; ++i) {
+    R->mbt_x[i] = round15(R->mbt_min + i * R->mbt_step);
+    R->mbt_y[i] = cx_make(round15(R->mbt_y[i].re), round15(R->mbt_y[i].im));
+  }
+  /* MSA: f = i (1 - exp(i Omega(bt) / 2)) * J0(bt kt) * bt */
+  {
+    const double STEP = (R->mbt_max - R->mbt_min) / NF;
+    for (int q = 0; q <= NF; ++q) {
+      const double bt = R->mbt_min + q * STEP;
+      const cx Om = interp1d(R->mbt_x, R->mbt_y, R->n_bt, R->mbt_min, R->mbt_max, R->mbt_step, bt);
+      /* zi * Omega / 2.0 = (-Om.im, Om.re) / 2 */
+      const cx e = cx_exp(cx_div_real(cx_make(0.0 * Om.re - 1.0 * Om.im, 0.0 * Om.im + 1.0 * Om.re), 2.0));
+      const cx one_m = cx_make(1.0 - e.re, -e.im);
+      pre[q] = cx_make(0.0 * one_m.re - 1.0 * one_m.im, 0.0 * one_m.im + 1.0 * one_m.re);
+    }
+  }
+  run_eik_jobs(R, 1, R->n_kt2 + 1, pre, nthreads);
+  for (int i = 0; i <= R->n_kt2; ++i) {
+    R->msa_x[i] = round15(R->msa_min + i * R->msa_step);
+    R->msa_y[i] = cx_make(round15(R->msa_y[i].re), round15(R->msa_y[i].im));
+  }
+  free(pre);
+  R->tables_ready = 1;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Screened amplitude and event weight                                       */
+/* ------------------------------------------------------------------------- */
+#define MAX_HAMP 64
+
+/* Simpson 3/8 weight vector, include/Graniitti/MMath.h:502-516 */
+static double simpson38_w(int i, int N) {
+  if (i == 0 || i == N) return 1.0;
+  return (i % 3 == 0) ? 2.0 : 3.0;
+}
+ORA_EXPORT double ora_simpson38_weight(int i, int N) { return simpson38_w(i, N); }
+
+/* MProcess::S3ScreenedAmp2, src/MProcess.cc:157-288 (non-gg branch) */
+static double screened_amp2(ora_run* R, kin_t* K) {
+  cx hamp0[MAX_HAMP], h[MAX_HAMP];
+  const int nh = bare_amplitudes(R, K, hamp0);
+  if (!R->c.screening) return bare_amp2(nh, hamp0);
+
+  const double p1T[2] = {K->pf[1].x, K->pf[1].y};
+  const double p2T[2] = {K->pf[2].x, K->pf[2].y};
+  for (int i = 0; i < 5; ++i) K->pf_orig[i] = K->pf[i];
+  const int NP = R->nphi, NK = R->nkt;
+  const double StepKT = R->step_kt, StepPhi = R->step_phi;
+  cx I[MAX_HAMP];
+  for (int q = 0; q < nh; ++q) I[q] = cx_make(0, 0);
+  /* The reference stores f[h][i][j] and sums W[i][j] * f[i][j] in (i, j) order;
+   * accumulating in the same order gives the same sums. */
+  for (int i = 0; i <= NP; ++i) {
+    const double phi = 0.0 + i * StepPhi;
+    for (int j = 0; j <= NK; ++j) {
+      const double kt = R->c.min_loop_kt + j * StepKT;
+      const double kt2 = kt * kt;
+      const cx A_eik =
+          interp1d(R->msa_x, R->msa_y, R->n_kt2, R->msa_min, R->msa_max, R->msa_step, kt2);
+      const double ktx = kt * cos(phi), kty = kt * sin(phi);
+      const double p1p[2] = {p1T[0] - ktx, p1T[1] - kty};
+      const double p2p[2] = {p2T[0] + ktx, p2T[1] + kty};
+      const double w = simpson38_w(i, NP) * simpson38_w(j, NK);
+      if (!loop_kinematics(R, K, p1p, p2p)) {
+        continue; /* f[i][j] = 0 still enters the sum as W * 0 */
+      }
+      bare_amplitudes(R, K, h);
+      for (int q = 0; q < nh; ++q) {
+        const cx f = cx_scale(cx_mul(A_eik, h[q]), kt);
+        I[q] = cx_add(I[q], cx_scale(f, w));
+      }
+    }
+  }
+  const double nrm = 1.0 / (8.0 * (PI * PI) * R->s);
+  const double hfac = 9 * (StepPhi * StepKT) / 64;
+  loop_kinematics(R, K, p1T, p2T); /* back to tree level */
+  double amp2 = 0.0;
+  for (int q = 0; q < nh; ++q) {
+    const cx Iq = cx_scale(I[q], hfac);
+    const cx loop = cx_make(0.0 * Iq.re - nrm * Iq.im, 0.0 * Iq.im + nrm * Iq.re);
+    amp2 += cx_abs2(cx_add(hamp0[q], loop));
+  }
+  if (nh != 1) amp2 /= 4;
+  return amp2;
+}
+
+/* MContinuum::EventWeight, src/MContinuum.cc:164-193, with BNPhaseSpaceWeight
+ * (639-656), BNIntegralVolume (619-636), ForwardVolume (src/MProcess.cc:1390-1415),
+ * MollerFlux = 2 s (src/MProcess.h:214) and CheckInfNan (src/MProcess.h:317-340). */
+static double event_weight(ora_run* R, const double* x, double* amp2_out, uint8_t* flags) {
+  const gra_cfg* c = &R->c;
+  kin_t K;
+  memset(&K, 0, sizeof(K));
+  uint8_t fl = 0;
+  double W = 0.0, amp2 = 0.0;
+  const int kin_ok = build_kin(R, x, &K);
+  if (kin_ok) fl |= GRA_FLAG_KIN_OK;
+  const int fid_ok = kin_ok ? fiducial_cuts(R, &K) : 0;
+  if (fid_ok) fl |= GRA_FLAG_FID_OK;
+  fl |= GRA_FLAG_VETO_OK;
+  if (kin_ok && fid_ok) {
+    amp2 = screened_amp2(R, &K);
+    const double C_space = 1.0;
+    const double J = 1.0 / fabs(K.pf[1].z / K.pf[1].e - K.pf[2].z / K.pf[2].e);
+    const double PROD = 1.0 * v4_pt(K.pkt);
+    const double Nf = 4;
+    const double psw = (1.0 / pow(2, 2 * (Nf - 2))) * (1.0 / pow(2.0 * PI, 3 * Nf - 4)) *
+                       (v4_pt(K.pf[1]) / (2.0 * K.pf[1].e)) * (v4_pt(K.pf[2]) / (2.0 * K.pf[2].e)) *
+                       J * PROD;
+    const double ZERO_EPS = 1e-12;
+    const double PHI_vol = (2.0 * PI) * (2.0 * PI);
+    const double J_pt = v4_pt(K.pf[1]) * v4_pt(K.pf[2]);
+    const double lspan = log(c->fwd_pt_max) - log(c->fwd_pt_min + ZERO_EPS);
+    const double PT_vol = J_pt * (lspan * lspan);
+    double KT_vol = pow(c->kt_max - c->kt_min, 1);
+    KT_vol *= pow(2.0 * PI, 1);
+    KT_vol *= pow(c->rap_max - c->rap_min, 2);
+    const double vol = KT_vol * (PHI_vol * PT_vol);
+    W = C_space * (1.0 / c->S_factor) * psw * vol * amp2 * R->GeV2barn / (2.0 * R->s);
+  }
+  if (isnan(W) || isinf(W)) {
+    W = 0;
+  } else {
+    fl |= GRA_FLAG_AMP_OK;
+  }
+  if (amp2_out) *amp2_out = amp2;
+  if (flags) *flags = fl;
+  return W;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Run object                                                                */
+/* ------------------------------------------------------------------------- */
+ORA_EXPORT ora_run* ora_create(const gra_cfg* cfg, int nthreads) {
+  if (cfg->abi_size != (int32_t)sizeof(gra_cfg)) return NULL;
+  ora_run* R = (ora_run*)calloc(1, sizeof(ora_run));
+  R->c = *cfg;
+  const double E1 = cfg->beam_E1, E2 = cfg->beam_E2, m = cfg->m_beam;
+  R->pbeam1 = v4_make(0, 0, msqrt(E1 * E1 - m * m), E1);
+  R->pbeam2 = v4_make(0, 0, -msqrt(E2 * E2 - m * m), E2);
+  R->beamsum = v4_add(R->pbeam1, R->pbeam2);
+  R->s = v4_m2(R->beamsum);
+  R->sqrt_s = msqrt(R->s);
+  R->GeV2barn = gev2barn();
+  R->nphi = cfg->n_loop_phi;
+  R->nkt = cfg->n_loop_kt;
+  R->step_kt = (cfg->max_loop_kt - cfg->min_loop_kt) / cfg->n_loop_kt;
+  R->step_phi = (2.0 * PI - 0.0) / cfg->n_loop_phi;
+  if (cfg->screening) {
+    R->n_bt = cfg->n_bt;
+    R->n_kt2 = cfg->n_kt2;
+    R->n_fb = cfg->n_fb;
+    R->mbt_min = 1E-6;
+    R->mbt_max = 10.0 / gev2fm();
+    R->mbt_step = (R->mbt_max - R->mbt_min) / R->n_bt;
+    R->msa_min = 1E-6;
+    R->msa_max = 25.0;
+    R->msa_step = (R->msa_max - R->msa_min) / R->n_kt2;
+    R->mbt_x = (double*)malloc(sizeof(double) * (R->n_bt + 1));
+    R->msa_x = (double*)malloc(sizeof(double) * (R->n_kt2 + 1));
+    R->mbt_y = (cx*)malloc(sizeof(cx) * (R->n_bt + 1));
+    R->msa_y = (cx*)malloc(sizeof(cx) * (R->n_kt2 + 1));
+    build_tables(R, nthreads);
+  }
+  return R;
+}
+
+ORA_EXPORT void ora_destroy(ora_run* R) {
+  if (!R) return;
+  free(R->mbt_x);
+  free(R->msa_x);
+  free(R->mbt_y);
+  free(R->msa_y);
+  free(R);
+}
+
+/* Table access: rows of (x, re, im) */
+ORA_EXPORT int ora_tables(const ora_run* R, double* mbt, double* msa) {
+  if (!R->tables_ready) return -1;
+  for (int i = 0; i <= R->n_bt; ++i) {
+    mbt[3 * i] = R->mbt_x[i];
+    mbt[3 * i + 1] = R->mbt_y[i].re;
+    mbt[3 * i + 2] = R->mbt_y[i].im;
+  }
+  for (int i = 0; i <= R->n_kt2; ++i) {
+    msa[3 * i] = R->msa_x[i];
+    msa[3 * i + 1] = R->msa_y[i].re;
+    msa[3 * i + 2] = R->msa_y[i].im;
+  }
+  return 0;
+}
+
+/* Interpolated screening amplitude A_eik(kt2) */
+ORA_EXPORT void ora_msa(const ora_run* R, double kt2, double* out) {
+  const cx z = interp1d(R->msa_x, R->msa_y, R->n_kt2, R->msa_min, R->msa_max, R->msa_step, kt2);
+  out[0] = z.re;
+  out[1] = z.im;
+}
+ORA_EXPORT void ora_mbt(const ora_run* R, double bt, double* out) {
+  const cx z = interp1d(R->mbt_x, R->mbt_y, R->n_bt, R->mbt_min, R->mbt_max, R->mbt_step, bt);
+  out[0] = z.re;
+  out[1] = z.im;
+}
+
+/* Batched EventWeight on caller points (n x dim row-major) */
+ORA_EXPORT void ora_eval_points(ora_run* R, const double* u, int64_t n, double* W, double* amp2,
+                                uint8_t* flags) {
+  const int D = R->c.dim;
+  for (int64_t i = 0; i < n; ++i) {
+    W[i] = event_weight(R, u + i * D, amp2 ? amp2 + i : NULL, flags ? flags + i : NULL);
+  }
+}
+
+/* Kinematic state of one point for tests: pfinal[0..4], decaytree legs and the
+ * scalars (t1, t2, t_hat, u_hat, s13, s24, s14, s23). Returns kinematics_ok. */
+ORA_EXPORT int ora_kinematics(ora_run* R, const double* x, double* pf /* 5x4 E,x,y,z */,
+                              double* legs /* 4x4 */, double* sc /* 8 */) {
+  kin_t K;
+  memset(&K, 0, sizeof(K));
+  const int ok = build_kin(R, x, &K);
+  for (int i = 0; i < 5; ++i) {
+    pf[4 * i] = K.pf[i].e;
+    pf[4 * i + 1] = K.pf[i].x;
+    pf[4 * i + 2] = K.pf[i].y;
+    pf[4 * i + 3] = K.pf[i].z;
+  }
+  for (int i = 0; i < 2; ++i)
+    for (int k = 0; k < 2; ++k) {
+      const v4 v = K.legs[i][k];
+      legs[8 * i + 4 * k] = v.e;
+      legs[8 * i + 4 * k + 1] = v.x;
+      legs[8 * i + 4 * k + 2] = v.y;
+      legs[8 * i + 4 * k + 3] = v.z;
+    }
+  sc[0] = K.t1;
+  sc[1] = K.t2;
+  sc[2] = K.t_hat;
+  sc[3] = K.u_hat;
+  sc[4] = K.ss[1][3];
+  sc[5] = K.ss[2][4];
+  sc[6] = K.ss[1][4];
+  sc[7] = K.ss[2][3];
+  return ok;
+}
+
+/* Bare (unscreened) helicity amplitudes at one point: returns their count, or 0
+ * if the kinematics fail. */
+ORA_EXPORT int ora_bare_amplitudes(ora_run* R, const double* x, double* hamp_reim) {
+  kin_t K;
+  memset(&K, 0, sizeof(K));
+  if (!build_kin(R, x, &K)) return 0;
+  cx h[MAX_HAMP];
+  const int nh = bare_amplitudes(R, &K, h);
+  for (int q = 0; q < nh; ++q) {
+    hamp_reim[2 * q] = h[q].re;
+    hamp_reim[2 * q + 1] = h[q].im;
+  }
+  return nh;
+}
+
+/* ------------------------------------------------------------------------- */
+/* VEGAS iterations: multithreaded CPU run (reference threading model)        */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+  ora_run* R;
+  const gra_grid* g;
+  uint64_t seed;
+  uint32_t iter;
+  uint64_t i0, i1;
+  int use_mutex;
+  pthread_mutex_t* mtx;
+  gra_iter_stats* shared; /* mutex mode */
+  gra_iter_stats local;   /* per-thread mode */
+} vg_job;
+
+static inline void kahan_add(double* s, double* c, double v) {
+  const double y = v - *c;
+  const double t = *s + y;
+  *c = (t - *s) - y;
+  *s = t;
+}
+
+static void accumulate_point(gra_iter_stats* st, const gra_grid* g, double W, double vw,
+                             const int32_t* bin, uint8_t fl) {
+  const double f = W * vw;
+  const double f2 = f * f;
+  kahan_add(&st->fsum, &st->fsum_c, f);
+  kahan_add(&st->f2sum, &st->f2sum_c, f2);
+  for (int j = 0; j < g->dim; ++j) {
+    st->fmat[j * GRA_VEGAS_BINS + bin[j] - 1] += f;
+    st->f2mat[j * GRA_VEGAS_BINS + bin[j] - 1] += f2;
+  }
+  if (W > st->maxW) st->maxW = W;
+  if (f > st->maxf) st->maxf = f;
+  st->n_calls += 1;
+  if (fl & GRA_FLAG_KIN_OK) st->n_kin_ok += 1;
+  if (fl & GRA_FLAG_FID_OK) st->n_fid_ok += 1;
+  if ((fl & GRA_FLAG_KIN_OK) && (fl & GRA_FLAG_FID_OK)) st->n_valid += 1;
+  if (!(fl & GRA_FLAG_AMP_OK)) st->n_nan += 1;
+}
+
+static void* vg_worker(void* arg) {
+  vg_job* J = (vg_job*)arg;
+  const int D = J->g->dim;
+  double u[GRA_MAX_DIM], x[GRA_MAX_DIM];
+  int32_t bin[GRA_MAX_DIM];
+  for (uint64_t i = J->i0; i < J->i1; ++i) {
+    ora_point_uniforms(J->seed, J->iter, i, D, u);
+    double vw;
+    ora_vegas_map(J->g, u, x, &vw, bin);
+    uint8_t fl;
+    const double W = event_weight(J->R, x, NULL, &fl);
+    if (J->use_mutex) {
+      pthread_mutex_lock(J->mtx);
+      accumulate_point(J->shared, J->g, W, vw, bin, fl);
+      pthread_mutex_unlock(J->mtx);
+    } else {
+      accumulate_point(&J->local, J->g, W, vw, bin, fl);
+    }
+  }
+  return NULL;
+}
+
+/* One VEGAS iteration over points [first, first + n) of the Philox stream
+ * (first > 0 lets the benchmark time a bounded sample of a larger iteration). */
+ORA_EXPORT void ora_vegas_iterate(ora_run* R, const gra_grid* g, uint64_t first, uint64_t n,
+                                  uint64_t seed, uint32_t iter, int nthreads, int use_mutex,
+                                  gra_iter_stats* out) {
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > 256) nthreads = 256;
+  memset(out, 0, sizeof(*out));
+  vg_job* jobs = (vg_job*)calloc(nthreads, sizeof(vg_job));
+  pthread_t* th = (pthread_t*)calloc(nthreads, sizeof(pthread_t));
+  pthread_mutex_t mtx = PTHREAD_MUTEX_INITIALIZER;
+  for (int t = 0; t < nthreads; ++t) {
+    jobs[t].R = R;
+    jobs[t].g = g;
+    jobs[t].seed = seed;
+    jobs[t].iter = iter;
+    jobs[t].i0 = first + n * t / nthreads;
+    jobs[t].i1 = first + n * (t + 1) / nthreads;
+    jobs[t].use_mutex = use_mutex;
+    jobs[t].mtx = &mtx;
+    jobs[t].shared = out;
+  }
+  /* Each thread needs its own process state: the run object is read-only during
+   * evaluation (all mutable state lives on the stack). */
+  for (int t = 0; t < nthreads; ++t) pthread_create(&th[t], NULL, vg_worker, &jobs[t]);
+  for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+  if (!use_mutex) {
+    for (int t = 0; t < nthreads; ++t) {
+      const gra_iter_stats* L = &jobs[t].local;
+      kahan_add(&out->fsum, &out->fsum_c, L->fsum - L->fsum_c);
+      kahan_add(&out->f2sum, &out->f2sum_c, L->f2sum - L->f2sum_c);
+      for (int k = 0; k < GRA_MAX_DIM * GRA_VEGAS_BINS; ++k) {
+        out->fmat[k] += L->fmat[k];
+        out->f2mat[k] += L->f2mat[k];
+      }
+      if (L->maxW > out->maxW) out->maxW = L->maxW;
+      if (L->maxf > out->maxf) out->maxf = L->maxf;
+      out->n_calls += L->n_calls;
+      out->n_kin_ok += L->n_kin_ok;
+      out->n_fid_ok += L->n_fid_ok;
+      out->n_valid += L->n_valid;
+      out->n_nan += L->n_nan;
+    }
+  }
+  free(jobs);
+  free(th);
+}
+
+/* Iteration combination of MGraniitti::VEGAS, src/MGraniitti.cc:1222-1251.
+ * state = {sumdata, sumchi2, sweight}; out = {I_this, I2_this, sigma, sigma_err, chi2}. */
+ORA_EXPORT void ora_vegas_combine(double fsum, double f2sum, double calls, int iter, double* state,
+                                  double* out) {
+  const double EPS = 1.0e-30;
+  fsum *= 1.0 / calls;
+  f2sum *= (1.0 / calls) * (1.0 / calls);
+  const double I_this = fsum;
+  f2sum = msqrt(f2sum * calls);
+  f2sum = (f2sum - fsum) * (f2sum + fsum);
+  if (f2sum <= 0.0) f2sum = EPS;
+  const double I2_this = f2sum * 1.0 / calls;
+  const double alpha = 1.0 / I2_this;
+  state[0] += alpha * I_this;
+  state[1] += alpha * (I_this * I_this);
+  state[2] += alpha;
+  const double sigma = state[0] / state[2];
+  const double sigma_err = msqrt(1.0 / state[2]);
+  double chi2 = (state[1] - (state[0] * state[0]) / state[2]) / (iter + 1E-5);
+  chi2 = (chi2 < 0) ? 0.0 : chi2;
+  out[0] = I_this;
+  out[1] = I2_this;
+  out[2] = sigma;
+  out[3] = sigma_err;
+  out[4] = chi2;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Tensor pomeron (placeholder until the tensor section is restated)         */
+/* ------------------------------------------------------------------------- */
+static int tensor_me4(const ora_run* R, const kin_t* K, cx* hamp) {
+  (void)R;
+  (void)K;
+  hamp[0] = cx_make(0, 0);
+  return 1;
+}
+
+/* Phase-space part of the weight, BNPhaseSpaceWeight x BNIntegralVolume
+ * (src/MContinuum.cc:619-656), at tree-level kinematics, without cuts; 0 if the
+ * kinematics fail. */
+ORA_EXPORT double ora_phase_space_weight(ora_run* R, const double* x) {
+  const gra_cfg* c = &R->c;
+  kin_t K;
+  memset(&K, 0, sizeof(K));
+  if (!build_kin(R, x, &K)) return 0.0;
+  const double J = 1.0 / fabs(K.pf[1].z / K.pf[1].e - K.pf[2].z / K.pf[2].e);
+  const double psw = (1.0 / pow(2, 4)) * (1.0 / pow(2.0 * PI, 8)) *
+                     (v4_pt(K.pf[1]) / (2.0 * K.pf[1].e)) * (v4_pt(K.pf[2]) / (2.0 * K.pf[2].e)) *
+                     J * (1.0 * v4_pt(K.pkt));
+  const double lspan = log(c->fwd_pt_max) - log(c->fwd_pt_min + 1e-12);
+  const double vol = pow(c->kt_max - c->kt_min, 1) * pow(2.0 * PI, 1) *
+                     pow(c->rap_max - c->rap_min, 2) *
+                     ((2.0 * PI) * (2.0 * PI) * (v4_pt(K.pf[1]) * v4_pt(K.pf[2]) * (lspan * lspan)));
+  return psw * vol;
+}

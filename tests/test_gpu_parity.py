@@ -1,0 +1,206 @@
+"""GPU (HIP, gfx950) against the FP64 oracle on identical phase-space points.
+
+Tolerances: FP64 reference-order path <= 1e-10 relative per point (the same
+operations as the oracle; only libm ulps differ); FP32-amplitude path: the
+per-point |M|^2 error distribution is asserted and printed.
+"""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+from graniitti_amd import abi
+from graniitti_amd.process import MContinuum, MGraniitti, vegas_cfg
+import oracle_lib as ol
+from conftest import small_tables
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def points(n, dim, seed):
+    rng = np.random.default_rng(seed)
+    u = rng.random((n, dim))
+    # edge values of the hypercube
+    edge = np.array([0.0, 1e-300, 0.5, 1.0 - 2.0 ** -53])
+    for k in range(min(n, 64)):
+        u[k, k % dim] = edge[k % 4]
+    return u
+
+
+def compare(Wg, Wo, fg, fo, rtol, label, max_flag_mismatch=2):
+    mism = np.nonzero(fg != fo)[0]
+    assert len(mism) <= max_flag_mismatch, f"{label}: flag mismatches at {mism[:10]}"
+    ok = (fg == fo) & (Wo != 0)
+    rel = np.abs(Wg[ok] - Wo[ok]) / np.abs(Wo[ok])
+    say(f"{label}: n={len(Wg)} nonzero={ok.sum()} max rel={rel.max():.3e} "
+        f"median={np.median(rel):.3e}")
+    assert np.all(Wg[(fg == fo) & (Wo == 0)] == 0)
+    assert rel.max() <= rtol, f"{label}: max rel {rel.max()}"
+    return rel
+
+
+def test_config1_fp64_pointwise():
+    cfg = abi.cfg_default(1)
+    say("case: preset 1 FP64, 20000 points, seed 1")
+    u = points(20000, cfg.dim, 1)
+    with MContinuum(cfg) as P:
+        Wg, ag, fg = P.eval(u)
+    Wo, ao, fo = ol.OracleRun(cfg).eval(u)
+    compare(Wg, Wo, fg, fo, 1e-10, "cfg1 fp64")
+
+
+def test_config1_fp32_amplitude_error():
+    cfg = abi.cfg_default(1)
+    cfg.amp_precision = abi.GRA_AMP_FP32
+    say("case: preset 1 FP32 amplitude, 20000 points, seed 2")
+    u = points(20000, cfg.dim, 2)
+    with MContinuum(cfg) as P:
+        Wg, ag, fg = P.eval(u)
+    Wo, ao, fo = ol.OracleRun(abi.cfg_default(1)).eval(u)
+    rel = compare(Wg, Wo, fg, fo, 1e-3, "cfg1 fp32")
+    assert np.median(rel) < 1e-5
+
+
+def screened_cfg(amp):
+    cfg = small_tables(abi.cfg_default(2), 1000)
+    cfg.amp_precision = amp
+    return cfg
+
+
+def test_eikonal_tables_match_oracle():
+    cfg = screened_cfg(abi.GRA_AMP_FP64)
+    say("case: eikonal tables n=1000")
+    with MContinuum(cfg) as P:
+        node, mbt, msa = P.eikonal_tables()
+    R = ol.OracleRun(cfg)
+    ombt, omsa = R.tables()
+    np.testing.assert_array_equal(mbt[:, 0], ombt[:, 0])
+    np.testing.assert_array_equal(msa[:, 0], omsa[:, 0])
+    for got, want in [(mbt, ombt), (msa, omsa)]:
+        scale = np.abs(want[:, 1] + 1j * want[:, 2]).max()
+        err = np.abs((got[:, 1] + 1j * got[:, 2]) - (want[:, 1] + 1j * want[:, 2])).max()
+        assert err <= 1e-12 * scale
+    kt = cfg.min_loop_kt + (cfg.max_loop_kt - cfg.min_loop_kt) / cfg.n_loop_kt * np.arange(
+        cfg.n_loop_kt + 1)
+    want = np.array([R.msa(k * k) for k in kt])
+    assert np.abs(node - want).max() <= 1e-12 * np.abs(want).max()
+
+
+def test_config2_fp64_pointwise_screened():
+    cfg = screened_cfg(abi.GRA_AMP_FP64)
+    say("case: screened FP64, 2000 points, seed 3")
+    u = points(2000, cfg.dim, 3)
+    with MContinuum(cfg) as P:
+        Wg, ag, fg = P.eval(u)
+    Wo, ao, fo = ol.OracleRun(cfg).eval(u)
+    compare(Wg, Wo, fg, fo, 1e-10, "cfg2 fp64 screened")
+
+
+def test_config2_fp32_screened_error():
+    cfg = screened_cfg(abi.GRA_AMP_FP32)
+    say("case: screened FP32, 4000 points, seed 4")
+    u = points(4000, cfg.dim, 4)
+    with MContinuum(cfg) as P:
+        Wg, ag, fg = P.eval(u)
+    Wo, ao, fo = ol.OracleRun(screened_cfg(abi.GRA_AMP_FP64)).eval(u)
+    rel = compare(Wg, Wo, fg, fo, 1e-2, "cfg2 fp32 screened")
+    say(f"cfg2 fp32 |M|^2 rel err: median {np.median(rel):.2e} p99 {np.quantile(rel, 0.99):.2e}")
+    assert np.median(rel) < 1e-4
+
+
+def test_vegas_map_matches_oracle():
+    cfg = abi.cfg_default(1)
+    say("case: vegas map on an adapted grid")
+    g = abi.GraGrid()
+    abi.check(abi.load().gra_hip_grid_init(C.byref(g), 8, 128))
+    rng = np.random.default_rng(5)
+    st = abi.GraIterStats()
+    f2 = np.ctypeslib.as_array(st.f2mat).reshape(abi.GRA_MAX_DIM, abi.GRA_VEGAS_BINS)
+    f2[:8, :] = rng.lognormal(0, 2, (8, 128))
+    abi.check(abi.load().gra_hip_grid_optimize(C.byref(g), C.byref(st), 1.5))
+    u = rng.random((5000, 8))
+    with MContinuum(cfg) as P:
+        x, vw, b = P.vegas_map(g, u)
+    L = ol.oracle()
+    for i in range(0, 5000, 7):
+        xo = np.zeros(8)
+        vo = np.zeros(1)
+        bo = np.zeros(8, dtype=np.int32)
+        L.ora_vegas_map(C.byref(g), abi.dptr(np.ascontiguousarray(u[i])), abi.dptr(xo),
+                        abi.dptr(vo), bo.ctypes.data_as(C.POINTER(C.c_int32)))
+        np.testing.assert_array_equal(x[i], xo)
+        assert vw[i] == vo[0]
+        np.testing.assert_array_equal(b[i], bo)
+
+
+def test_vegas_iteration_matches_oracle():
+    cfg = abi.cfg_default(1)
+    say("case: one VEGAS iteration, 200000 calls, FP64")
+    g = abi.GraGrid()
+    abi.check(abi.load().gra_hip_grid_init(C.byref(g), 8, 128))
+    with MContinuum(cfg) as P:
+        sg = P.vegas_iterate(g, 200000, 777, 3)
+        sg2 = P.vegas_iterate(g, 200000, 777, 3)
+    so = ol.OracleRun(cfg).vegas_iterate(g, 200000, 777, 3)
+    # deterministic on the GPU
+    assert sg.fsum == sg2.fsum and sg.f2sum == sg2.f2sum
+    assert np.array_equal(np.ctypeslib.as_array(sg.fmat), np.ctypeslib.as_array(sg2.fmat))
+    for k in ("n_calls", "n_kin_ok", "n_fid_ok", "n_valid", "n_nan"):
+        assert abs(getattr(sg, k) - getattr(so, k)) <= 1, k
+    assert abs(sg.fsum - so.fsum) <= 1e-10 * so.fsum
+    assert abs(sg.f2sum - so.f2sum) <= 1e-10 * so.f2sum
+    assert abs(sg.maxf - so.maxf) <= 1e-10 * so.maxf
+    fm_g = np.ctypeslib.as_array(sg.fmat)
+    fm_o = np.ctypeslib.as_array(so.fmat)
+    assert np.abs(fm_g - fm_o).max() <= 1e-10 * fm_o.max()
+
+
+def test_vegas_run_sigma_matches_oracle_run():
+    cfg = abi.cfg_default(1)
+    say("case: VEGAS run 3+5 x 100000, FP64, GPU vs oracle")
+    vc = vegas_cfg(100000, 5, burnin=3, seed=12345678)
+    with MContinuum(cfg) as P:
+        G = MGraniitti(P)
+        res = G.Integrate(vc)
+    # oracle run with the same schedule
+    R = ol.OracleRun(cfg)
+    L = ol.oracle()
+    g = abi.GraGrid()
+    L.ora_grid_init(C.byref(g), 8, 128)
+    state = np.zeros(3)
+    out = np.zeros(5)
+    cnt = 0
+    for it in range(3):
+        st = R.vegas_iterate(g, 100000, 12345678, cnt)
+        cnt += 1
+        L.ora_vegas_combine(st.fsum, st.f2sum, 100000.0, it, abi.dptr(state), abi.dptr(out))
+        L.ora_grid_optimize(C.byref(g), abi.dptr(np.ctypeslib.as_array(st.f2mat)), 1.5)
+    state[:] = 0
+    for it in range(5):
+        st = R.vegas_iterate(g, 100000, 12345678, cnt)
+        cnt += 1
+        L.ora_vegas_combine(st.fsum, st.f2sum, 100000.0, it, abi.dptr(state), abi.dptr(out))
+        L.ora_grid_optimize(C.byref(g), abi.dptr(np.ctypeslib.as_array(st.f2mat)), 1.5)
+    say(f"sigma GPU {res.sigma:.10e} +- {res.sigma_err:.3e}  oracle {out[2]:.10e} +- {out[3]:.3e}")
+    assert abs(res.sigma - out[2]) <= 1e-8 * out[2]
+    assert abs(res.sigma_err - out[3]) <= 1e-6 * out[3]
+
+
+def test_unweighting_efficiency():
+    cfg = abi.cfg_default(1)
+    say("case: unweighting 1e6 weights")
+    rng = np.random.default_rng(9)
+    w = rng.exponential(1.0, 1_000_000)
+    wmax = np.quantile(w, 0.999)
+    with MContinuum(cfg) as P:
+        acc, na, no = P.unweight(w, wmax, 42)
+    assert na == acc.sum()
+    assert no == (w > wmax).sum()
+    expect = np.minimum(w, 0).sum() + (w[w <= wmax] / wmax).sum()
+    assert abs(na - expect) < 5 * np.sqrt(expect)
