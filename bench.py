@@ -1,0 +1,184 @@
+"""Benchmark of the screened 2->4 continuum inner loop on one MI355X.
+
+Workload (default): pp -> p pi+ pi- p, sqrt(s) = 13 TeV, |eta| < 0.9, pT > 0.15 GeV,
+single-channel eikonal screening (16 phi x 31 kt loop nodes, MaxLoopKT 3 GeV,
+10001-node eikonal tables), FP64 kinematics and Jacobian, FP32 amplitude with
+compensated FP32 loop sums, VEGAS with 2^20 points per iteration.
+
+A step is one VEGAS iteration: 2^20 points drawn (Philox), mapped through the
+grid, weighted (tree kernel + screening kernel), accumulated, and the grid
+updated. W untimed warm-up iterations adapt the grid; K iterations are timed.
+
+value = unweighted events/s = points/s x unweighting efficiency <f>/max f of
+the timed iterations (MGraniitti::SaveEvent hit-or-miss, src/MGraniitti.cc:1651-1663).
+Extra fields: points/s, time to 0.1% relative sigma error (a separate VEGAS run
+to that precision, grid adaptation included), sigma.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from graniitti_amd import abi  # noqa: E402
+from graniitti_amd.process import MContinuum, MGraniitti, vegas_cfg  # noqa: E402
+
+# Algorithmic work of one screening-loop node, counted on this build's kernel
+# formulation (loop kinematics + Lorentz scalars in FP64, reduced Regge
+# amplitude in FP32): see the work-count table of the design notes.
+FP64_FLOP_PER_NODE = 333
+FP32_FLOP_PER_NODE = 60
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(cfg, grid, eff, n_sample, threads):
+    """The oracle's multithreaded FP64 restatement on a bounded sample of the
+    same workload (same grid), per-thread accumulators."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+
+    cfg64 = abi.GraCfg.from_buffer_copy(cfg)
+    cfg64.amp_precision = abi.GRA_AMP_FP64
+    t0 = time.time()
+    R = ol.OracleRun(cfg64, threads)
+    t_tables = time.time() - t0
+    t0 = time.time()
+    st = R.vegas_iterate(grid, n_sample, 987654321, 0, nthreads=threads)
+    dt = time.time() - t0
+    R.close()
+    pts = n_sample / dt
+    return {
+        "value": pts * eff, "unit": "unweighted events/s", "cores": threads, "kind": "port",
+        "points_per_s": pts,
+        "sample": f"{n_sample} points of the same workload on the same adapted grid, "
+                  f"FP64 oracle, {threads} threads ({dt:.1f} s; eikonal tables {t_tables:.1f} s "
+                  f"not included); value = points/s x the GPU run's unweighting efficiency",
+        "n_valid": int(st.n_valid),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--points", type=int, default=1 << 20)
+    ap.add_argument("--preset", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-precision-run", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=1 << 17)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: every box here has one MI355X; "
+                          "the multi-GPU path is a design note only (bench.py --gpus 1)"}))
+        sys.exit(2)
+
+    cfg = abi.cfg_default(args.preset)
+    N = args.points
+    P = MContinuum(cfg)
+    lib = P.lib
+    grid = abi.GraGrid()
+    abi.check(lib.gra_hip_grid_init(C.byref(grid), cfg.dim, 128))
+    seed = 12345678
+    it = 0
+    log(f"workload preset {args.preset}: {N} points/iteration, warmup {args.warmup}, "
+        f"steps {args.steps}")
+    for _ in range(args.warmup):
+        st = P.vegas_iterate(grid, N, seed, it)
+        it += 1
+        abi.check(lib.gra_hip_grid_optimize(C.byref(grid), C.byref(st), 1.5))
+    for s in range(4):
+        P.kernel_time(s, reset=True)
+    fsum = f2sum = 0.0
+    maxf = 0.0
+    n_valid = 0
+    state = [0.0, 0.0, 0.0]
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = P.vegas_iterate(grid, N, seed, it)
+        it += 1
+        abi.check(lib.gra_hip_grid_optimize(C.byref(grid), C.byref(st), 1.5))
+        fsum += st.fsum
+        f2sum += st.f2sum
+        maxf = max(maxf, st.maxf)
+        n_valid += st.n_valid
+        # iteration combination (src/MGraniitti.cc:1222-1246) for the sigma report
+        I = st.fsum / N
+        v = max((st.f2sum / N - I * I) / N, 1e-300)
+        state[0] += I / v
+        state[1] += 1.0 / v
+    t1 = time.perf_counter()
+    dt = t1 - t0
+    ms_tree, n_tree = P.kernel_time(0)
+    ms_scr, n_scr = P.kernel_time(1)
+    ms_acc, n_acc = P.kernel_time(2)
+    pts_per_s = N * args.steps / dt
+    mean_f = fsum / (N * args.steps)
+    eff = mean_f / maxf if maxf > 0 else 0.0
+    value = pts_per_s * eff
+    sigma = state[0] / state[1]
+    sigma_err = 1.0 / np.sqrt(state[1])
+    log(f"timed: {dt:.3f} s, {pts_per_s:.3e} points/s, eff {eff:.4f}, sigma {sigma:.5e} +- "
+        f"{sigma_err:.2e} barn")
+
+    roofline = None
+    if cfg.screening and n_scr > 0:
+        nodes = (cfg.n_loop_phi + 1) * (cfg.n_loop_kt + 1)
+        cand_per_launch = n_valid / n_scr
+        avg_ms = ms_scr / n_scr
+        flop = FP64_FLOP_PER_NODE * nodes * cand_per_launch
+        achieved = flop / (avg_ms * 1e-3) / 1e12
+        roofline = {
+            "bound": "valu", "kernel": "k_screen (screening loop)",
+            "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+            "avg_launch_ms": avg_ms, "launches": n_scr,
+            "fp64_flop_per_node": FP64_FLOP_PER_NODE, "fp32_flop_per_node": FP32_FLOP_PER_NODE,
+            "nodes_per_point": nodes, "screened_points_per_launch": cand_per_launch,
+        }
+    # time to 0.1% relative sigma error with plain VEGAS (GPU)
+    t_prec = None
+    prec_res = None
+    if not args.no_precision_run:
+        G = MGraniitti(P)
+        vc = vegas_cfg(N, 1, burnin=args.warmup, seed=424242, precision=1e-3, max_iter=60)
+        res = G.Integrate(vc)
+        t_prec = res.seconds_to_precision if res.seconds_to_precision > 0 else None
+        prec_res = {"sigma": res.sigma, "sigma_err": res.sigma_err, "iterations": res.iterations,
+                    "seconds_total": res.seconds_total, "calls_total": int(res.calls_total)}
+        log(f"precision run: {prec_res}, t(0.1%) = {t_prec}")
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(cfg, grid, eff, args.cpu_sample, args.cpu_threads)
+        log(f"cpu baseline: {cpu}")
+    P.close()
+    line = {
+        "metric": "time to 0.1% sigma rel. error & unweighted events/s, flow vs VEGAS, 1 MI355X",
+        "value": value, "unit": "unweighted events/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
+        "config": {"workload": "pp->p pi+ pi- p 13 TeV, screened (16x31 loop nodes), "
+                               "VEGAS 2^20 points/iteration, FP32 amplitude",
+                   "preset": args.preset, "points_per_step": N, "integrator": "VEGAS"},
+        "points_per_s": pts_per_s, "unweighting_efficiency": eff,
+        "time_to_0p1pct_s": t_prec, "precision_run": prec_res,
+        "sigma_barn": sigma, "sigma_err_barn": sigma_err,
+        "kernel_ms": {"tree": ms_tree, "screen": ms_scr, "accumulate": ms_acc},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

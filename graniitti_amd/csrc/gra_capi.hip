@@ -582,10 +582,11 @@ extern "C" int gra_hip_vegas_iterate(gra_handle* h, const gra_grid* grid, uint64
   const int D = h->P.dim;
   r = ensure_point_buffers(h, n_calls);
   if (r) return r;
-  const int nblk = (int)std::min<uint64_t>(2048, std::max<uint64_t>(1, (n_calls + 2047) / 2048));
+  // one wave per 512 points, at most 8192 blocks; +32 rows of stage-1 reduction space
+  const int nblk = (int)std::min<uint64_t>(8192, std::max<uint64_t>(1, (n_calls + 511) / 512));
   const size_t H = 2 * (size_t)D * GRA_VEGAS_BINS;
   if (h->d_grid.ensure(sizeof(double) * GRA_MAX_DIM * GRA_VEGAS_BINS) ||
-      h->d_part_hist.ensure(sizeof(double) * H * nblk) ||
+      h->d_part_hist.ensure(sizeof(double) * (H * (nblk + 32) + 32 * 10)) ||
       h->d_part_scal.ensure(sizeof(double) * 10 * nblk) ||
       h->d_stats.ensure(sizeof(double) * (10 + H)))
     return fail(h, GRA_E_NOMEM, "VEGAS buffers");

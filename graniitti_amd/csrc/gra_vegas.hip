@@ -6,10 +6,21 @@
 
 namespace {
 
-// Deterministic accumulation of one iteration: each block (one wave) owns a
-// contiguous range of points; lane l owns histogram bins l and l + 64; lanes
-// read each point's (bin, f) with wave broadcasts in point order.
+// Deterministic accumulation of one iteration. Each block (one wave) owns a
+// contiguous range of points. Lane l owns histogram bins l and l + 64 of every
+// dimension and keeps them in registers; the wave walks its non-zero points in
+// point order and broadcasts each point's (f, bins) with v_readlane, so every
+// bin is summed in the same order on every run without atomics.
 constexpr int kAccWave = 64;
+
+__device__ __forceinline__ double readlane_d(double v, int src) {
+  const uint64_t u = __double_as_longlong(v);
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, src);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), src);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+template <int DT>
 __global__ __launch_bounds__(kAccWave) void k_accumulate(const double* __restrict__ W,
                                                          const double* __restrict__ vw,
                                                          const uint8_t* __restrict__ bins,
@@ -17,10 +28,10 @@ __global__ __launch_bounds__(kAccWave) void k_accumulate(const double* __restric
                                                          uint64_t n, uint64_t chunk, int D,
                                                          double* __restrict__ part_hist,
                                                          double* __restrict__ part_scal) {
-  extern __shared__ double hist[];  // [2][D][128]
   const int lane = threadIdx.x;
-  for (int k = lane; k < 2 * D * GRA_VEGAS_BINS; k += kAccWave) hist[k] = 0.0;
-  __syncthreads();
+  double hlo[DT], hhi[DT], h2lo[DT], h2hi[DT];
+#pragma unroll
+  for (int j = 0; j < DT; ++j) hlo[j] = hhi[j] = h2lo[j] = h2hi[j] = 0.0;
   const uint64_t p0 = (uint64_t)blockIdx.x * chunk;
   uint64_t p1 = p0 + chunk;
   if (p1 > n) p1 = n;
@@ -29,6 +40,9 @@ __global__ __launch_bounds__(kAccWave) void k_accumulate(const double* __restric
   for (uint64_t base = p0; base < p1; base += kAccWave) {
     const uint64_t i = base + lane;
     double f = 0.0;
+    int bb[DT];
+#pragma unroll
+    for (int j = 0; j < DT; ++j) bb[j] = 0;
     if (i < p1) {
       const double Wi = W[i];
       f = Wi * vw[i];
@@ -47,26 +61,39 @@ __global__ __launch_bounds__(kAccWave) void k_accumulate(const double* __restric
       c_fid += (fl & GRA_FLAG_FID_OK) ? 1 : 0;
       c_val += ((fl & GRA_FLAG_KIN_OK) && (fl & GRA_FLAG_FID_OK)) ? 1 : 0;
       c_nan += (fl & GRA_FLAG_AMP_OK) ? 0 : 1;
+#pragma unroll
+      for (int j = 0; j < DT; ++j)
+        if (j < D) bb[j] = bins[i * D + j];
     }
-    // Non-zero contributions of this tile, in point order
     uint64_t m = __ballot(f != 0.0);
     while (m) {
       const int src = __ffsll((unsigned long long)m) - 1;
       m &= m - 1;
-      const double fv = __shfl(f, src);
-      const uint64_t ip = base + src;
-      for (int j = 0; j < D; ++j) {
-        const int b = bins[ip * D + j];
-        if ((b & 63) == lane) {
-          hist[j * GRA_VEGAS_BINS + b] += fv;
-          hist[(D + j) * GRA_VEGAS_BINS + b] += fv * fv;
-        }
+      const double fv = readlane_d(f, src);
+      const double fv2 = fv * fv;
+#pragma unroll
+      for (int j = 0; j < DT; ++j) {
+        const int b = __builtin_amdgcn_readlane(bb[j], src);
+        // f >= 0, so adding +0.0 to the bins a lane does not own leaves them unchanged
+        const bool lo = b == lane, hi = b == lane + 64;
+        hlo[j] += lo ? fv : 0.0;
+        h2lo[j] += lo ? fv2 : 0.0;
+        hhi[j] += hi ? fv : 0.0;
+        h2hi[j] += hi ? fv2 : 0.0;
       }
     }
   }
-  __syncthreads();
-  for (int k = lane; k < 2 * D * GRA_VEGAS_BINS; k += kAccWave)
-    part_hist[(uint64_t)blockIdx.x * 2 * D * GRA_VEGAS_BINS + k] = hist[k];
+  const uint64_t H = 2 * (uint64_t)D * GRA_VEGAS_BINS;
+  double* ph = part_hist + (uint64_t)blockIdx.x * H;
+#pragma unroll
+  for (int j = 0; j < DT; ++j) {
+    if (j < D) {
+      ph[j * GRA_VEGAS_BINS + lane] = hlo[j];
+      ph[j * GRA_VEGAS_BINS + lane + 64] = hhi[j];
+      ph[(D + j) * GRA_VEGAS_BINS + lane] = h2lo[j];
+      ph[(D + j) * GRA_VEGAS_BINS + lane + 64] = h2hi[j];
+    }
+  }
   // Wave reduction of the scalars in a fixed lane order
   for (int off = 32; off > 0; off >>= 1) {
     const double ofs = __shfl_down(fs, off), ofc = __shfl_down(fc, off);
@@ -103,44 +130,53 @@ __global__ __launch_bounds__(kAccWave) void k_accumulate(const double* __restric
   }
 }
 
-// Fixed-order reduction of the block partials into gra_iter_stats layout:
-// out[0..9] scalars, out[10 ..] hist [2][D][128].
-__global__ void k_reduce(const double* __restrict__ part_hist, const double* __restrict__ part_scal,
-                         int nblk, int D, double* __restrict__ out) {
+// Fixed-order two-level reduction of the block partials. Stage 1: group g sums
+// blocks [g nblk / G, (g + 1) nblk / G) in order; stage 2 sums the groups in
+// order into gra_iter_stats layout: out[0..9] scalars, out[10 ..] hist [2][D][128].
+constexpr int kRedGroups = 32;
+
+// Kahan-add of one block's (sum, compensation) pair and the other scalars
+__device__ __forceinline__ void scal_add(double* a, const double* p) {
+  double y = (p[0] - p[1]) - a[1], tt = a[0] + y;
+  a[1] = (tt - a[0]) - y;
+  a[0] = tt;
+  y = (p[2] - p[3]) - a[3];
+  tt = a[2] + y;
+  a[3] = (tt - a[2]) - y;
+  a[2] = tt;
+  a[4] = p[4] > a[4] ? p[4] : a[4];
+  a[5] = p[5] > a[5] ? p[5] : a[5];
+  for (int k = 6; k < 10; ++k) a[k] += p[k];
+}
+
+__global__ void k_reduce1(const double* __restrict__ part_hist, const double* __restrict__ part_scal,
+                          int nblk, int H, double* __restrict__ tmp, double* __restrict__ tmp_scal) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int H = 2 * D * GRA_VEGAS_BINS;
+  const int g = blockIdx.y;
+  const int b0 = (int)((long long)nblk * g / kRedGroups);
+  const int b1 = (int)((long long)nblk * (g + 1) / kRedGroups);
   if (t < H) {
     double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += part_hist[(uint64_t)b * H + t];
+    for (int b = b0; b < b1; ++b) s += part_hist[(uint64_t)b * H + t];
+    tmp[(uint64_t)g * H + t] = s;
+  } else if (t == H) {
+    double a[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int b = b0; b < b1; ++b) scal_add(a, part_scal + (uint64_t)b * 10);
+    for (int k = 0; k < 10; ++k) tmp_scal[g * 10 + k] = a[k];
+  }
+}
+
+__global__ void k_reduce2(const double* __restrict__ tmp, const double* __restrict__ tmp_scal, int H,
+                          double* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < H) {
+    double s = 0.0;
+    for (int g = 0; g < kRedGroups; ++g) s += tmp[(uint64_t)g * H + t];
     out[10 + t] = s;
   } else if (t == H) {
-    double fs = 0, fc = 0, f2s = 0, f2c = 0, mW = 0, mf = 0, ck = 0, cf_ = 0, cv = 0, cn = 0;
-    for (int b = 0; b < nblk; ++b) {
-      const double* p = part_scal + (uint64_t)b * 10;
-      double y = (p[0] - p[1]) - fc, tt = fs + y;
-      fc = (tt - fs) - y;
-      fs = tt;
-      y = (p[2] - p[3]) - f2c;
-      tt = f2s + y;
-      f2c = (tt - f2s) - y;
-      f2s = tt;
-      mW = p[4] > mW ? p[4] : mW;
-      mf = p[5] > mf ? p[5] : mf;
-      ck += p[6];
-      cf_ += p[7];
-      cv += p[8];
-      cn += p[9];
-    }
-    out[0] = fs;
-    out[1] = fc;
-    out[2] = f2s;
-    out[3] = f2c;
-    out[4] = mW;
-    out[5] = mf;
-    out[6] = ck;
-    out[7] = cf_;
-    out[8] = cv;
-    out[9] = cn;
+    double a[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int g = 0; g < kRedGroups; ++g) scal_add(a, tmp_scal + g * 10);
+    for (int k = 0; k < 10; ++k) out[k] = a[k];
   }
 }
 
@@ -194,12 +230,22 @@ hipError_t launch_accumulate(const double* W, const double* vw, const uint8_t* b
                              const uint8_t* flags, uint64_t n, int D, int nblk, double* part_hist,
                              double* part_scal, double* out, hipStream_t st) {
   const uint64_t chunk = (n + nblk - 1) / nblk;
-  const size_t lds = sizeof(double) * 2 * D * GRA_VEGAS_BINS;
-  hipLaunchKernelGGL(k_accumulate, dim3(nblk), dim3(kAccWave), lds, st, W, vw, bins, flags, n,
-                     chunk, D, part_hist, part_scal);
-  const int H = 2 * D * GRA_VEGAS_BINS + 1;
-  hipLaunchKernelGGL(k_reduce, dim3((H + 255) / 256), dim3(256), 0, st, part_hist, part_scal, nblk,
-                     D, out);
+  if (D <= 8)
+    hipLaunchKernelGGL(k_accumulate<8>, dim3(nblk), dim3(kAccWave), 0, st, W, vw, bins, flags, n,
+                       chunk, D, part_hist, part_scal);
+  else if (D <= 14)
+    hipLaunchKernelGGL(k_accumulate<14>, dim3(nblk), dim3(kAccWave), 0, st, W, vw, bins, flags, n,
+                       chunk, D, part_hist, part_scal);
+  else
+    hipLaunchKernelGGL(k_accumulate<16>, dim3(nblk), dim3(kAccWave), 0, st, W, vw, bins, flags, n,
+                       chunk, D, part_hist, part_scal);
+  const int H = 2 * D * GRA_VEGAS_BINS;
+  // stage-1 partial sums use the tail of part_hist's allocation: [G][H] then [G][10]
+  double* tmp = part_hist + (uint64_t)nblk * H;
+  double* tmp_scal = tmp + (uint64_t)kRedGroups * H;
+  hipLaunchKernelGGL(k_reduce1, dim3((H + 1 + 255) / 256, kRedGroups), dim3(256), 0, st,
+                     part_hist, part_scal, nblk, H, tmp, tmp_scal);
+  hipLaunchKernelGGL(k_reduce2, dim3((H + 1 + 255) / 256), dim3(256), 0, st, tmp, tmp_scal, H, out);
   return hipGetLastError();
 }
 
