@@ -210,6 +210,8 @@ void fill_params(const gra_cfg& c, RunParams& P) {
   P.sqrt_s = msqrt(P.s);
   P.rs = std::sqrt(P.s);
   P.s32 = std::pow(P.s, 3.0 / 2.0);
+  P.c_beam1 = (m * m) / (P.pbeam1.e + P.pbeam1.z);
+  P.c_beam2 = (m * m) / (P.pbeam2.e - P.pbeam2.z);
   const double ZERO_EPS = 1e-12;
   P.lmin = std::log(c.fwd_pt_min + ZERO_EPS);
   P.lmax = std::log(c.fwd_pt_max);

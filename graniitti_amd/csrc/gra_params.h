@@ -18,6 +18,7 @@ struct RunParams {
   // beams and invariants
   GV4 pbeam1, pbeam2, beamsum;
   double s, sqrt_s, rs, s32;
+  double c_beam1, c_beam2;  // E - |p_z| of each beam, m^2 / (E + |p_z|)
   // sampling map (BNRandomKin)
   double lmin, lmax, kt_min, kt_max, rap_min, rap_max;
   double m_beam, m_central, w_central, m_daughter;
