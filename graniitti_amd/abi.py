@@ -112,6 +112,7 @@ SIGNATURES = {
     "gra_hip_unweight": (C.c_int, [VP, P(D), C.c_size_t, D, U64, P(C.c_uint8), P(U64), P(U64)]),
     "gra_hip_eikonal_tables": (C.c_int, [VP, P(D), P(D), P(D)]),
     "gra_hip_flow_init": (C.c_int, [VP, P(GraFlowCfg)]),
+    "gra_hip_flow_set_grid": (C.c_int, [VP, P(GraGrid)]),
     "gra_hip_flow_train": (C.c_int, [VP, I32, P(GraFlowStats)]),
     "gra_hip_flow_sample": (C.c_int, [VP, U64, U64, P(GraFlowStats)]),
     "gra_hip_flow_param_count": (C.c_int, [VP, P(U64)]),

@@ -1,33 +1,1150 @@
-// Normalizing-flow sampler entry points (rational-quadratic spline coupling
-// layers). This unit is filled in by the flow implementation.
+// Normalizing-flow importance sampler: rational-quadratic spline (RQS) coupling
+// layers (Durkan, Bekasov, Murray, Papamakarios 2019) over the unit hypercube of
+// the VEGAS grid, trained by minimising the variance of w = f / q with the
+// score-function gradient  grad Var_q(w) = -E_q[w^2 grad log q]
+// (Mueller et al. 2019, neural importance sampling). It occupies the NEURO
+// integrator slot of the reference (MGraniitti::SampleNeuro, src/MGraniitti.cc:
+// 1527-1648; include/Graniitti/MNeuroJacobian.h).
+//
+// Direction convention: F = F_L o ... o F_1 maps a point u of the VEGAS unit
+// cube to z; log q(u) = sum_l log dF_l. Sampling draws z ~ U(0,1)^D and inverts
+// the layers (analytic RQS inverse). Training re-uses the sampling pass's stored
+// layer inputs and MLP activations and back-propagates -sum_b c_b log q(u_b),
+// c_b = w_b^2 / sum w^2, through the forward direction.
+//
+// MLP: input (FP32, VALU) -> depth hidden ReLU layers -> output, the hidden and
+// output products as BF16 MFMA (v_mfma_f32_16x16x32_bf16) with FP32 accumulation;
+// a pure FP32 path (same kernels, VALU FMA) serves gradient checks.
 #include <hip/hip_runtime.h>
 
-#include "gra_internal.h"
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
 
+#include "gra_device.h"
+#include "gra_internal.h"
+#include "gra_kernels.h"
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+
+namespace {
+
+constexpr float kEpsW = 1e-3f;  // minimum bin width / height
+constexpr float kEpsD = 1e-3f;  // minimum knot derivative
+constexpr int kMaxK = 32;       // spline bins
+constexpr int kMaxDB = 8;       // transformed dimensions per layer
+
+__device__ __forceinline__ float softplusf(float x) {
+  return x > 20.f ? x : log1pf(__expf(x));
+}
+__device__ __forceinline__ float sigmoidf(float x) { return 1.f / (1.f + __expf(-x)); }
+
+// Spline knots of one transformed dimension from its raw parameters
+// p[0..K) widths, p[K..2K) heights, p[2K..3K-1) interior derivatives.
+struct Knots {
+  float sw[kMaxK], sh[kMaxK];  // softmax of widths / heights
+};
+__device__ __forceinline__ void softmax_k(const float* a, int K, float* out) {
+  float mx = a[0];
+  for (int k = 1; k < K; ++k) mx = fmaxf(mx, a[k]);
+  float sum = 0.f;
+  for (int k = 0; k < K; ++k) {
+    out[k] = __expf(a[k] - mx);
+    sum += out[k];
+  }
+  const float inv = 1.f / sum;
+  for (int k = 0; k < K; ++k) out[k] *= inv;
+}
+
+struct SplineEval {
+  int k;
+  float X, w, Y, h, dk, dk1, xi, s;
+};
+
+// Bin search in the cumulative widths; returns the bin and its parameters
+__device__ __forceinline__ void locate(const float* sw, const float* sh, const float* c, int K,
+                                       float v, bool by_y, SplineEval& e) {
+  const float cw = 1.f - K * kEpsW;
+  float X = 0.f, Y = 0.f;
+  int k = 0;
+  for (; k < K - 1; ++k) {
+    const float w = kEpsW + cw * sw[k], h = kEpsW + cw * sh[k];
+    if ((by_y ? Y + h : X + w) > v) break;
+    X += w;
+    Y += h;
+  }
+  e.k = k;
+  e.X = X;
+  e.Y = Y;
+  e.w = kEpsW + cw * sw[k];
+  e.h = kEpsW + cw * sh[k];
+  e.dk = (k == 0) ? 1.f : kEpsD + softplusf(c[k - 1]);
+  e.dk1 = (k == K - 1) ? 1.f : kEpsD + softplusf(c[k]);
+  e.s = e.h / e.w;
+}
+
+// Forward value and log-derivative at xi
+__device__ __forceinline__ void rqs_value(const SplineEval& e, float& z, float& logJ) {
+  const float xi = e.xi, t = xi * (1.f - xi);
+  const float Dn = e.s + (e.dk1 + e.dk - 2.f * e.s) * t;
+  const float N = e.h * (e.s * xi * xi + e.dk * t);
+  z = e.Y + N / Dn;
+  const float Pn = e.dk1 * xi * xi + 2.f * e.s * t + e.dk * (1.f - xi) * (1.f - xi);
+  logJ = 2.f * __logf(e.s) + __logf(Pn) - 2.f * __logf(Dn);
+}
+
+// ---------------------------------------------------------------------------
+// RQS kernels
+// ---------------------------------------------------------------------------
+struct LayerIdx {
+  int dA, dB;
+  int A[16], B[16];
+};
+
+// Sampling direction: u_{l-1}[B] = RQS^{-1}(u_l[B]), u_{l-1}[A] = u_l[A];
+// ld[b] += log dF_l at the pre-image.
+__global__ void k_rqs_inverse(const float* __restrict__ prm, int ldp, const float* __restrict__ u_in,
+                              float* __restrict__ u_out, float* __restrict__ ld, int n, int D,
+                              LayerIdx L, int K) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  const float* uin = u_in + (size_t)b * D;
+  float* uo = u_out + (size_t)b * D;
+  for (int i = 0; i < L.dA; ++i) uo[L.A[i]] = uin[L.A[i]];
+  float acc = 0.f;
+  const int np = 3 * K - 1;
+  for (int j = 0; j < L.dB; ++j) {
+    const float* p = prm + (size_t)b * ldp + j * np;
+    float sw[kMaxK], sh[kMaxK];
+    softmax_k(p, K, sw);
+    softmax_k(p + K, K, sh);
+    float zv = fminf(fmaxf(uin[L.B[j]], 0.f), 1.f);
+    SplineEval e;
+    locate(sw, sh, p + 2 * K, K, zv, true, e);
+    const float zeta = zv - e.Y;
+    const float Dl = e.dk1 + e.dk - 2.f * e.s;
+    const float qa = e.h * (e.s - e.dk) + zeta * Dl;
+    const float qb = e.h * e.dk - zeta * Dl;
+    const float qc = -e.s * zeta;
+    const float disc = fmaxf(qb * qb - 4.f * qa * qc, 0.f);
+    float xi = (2.f * qc) / (-qb - sqrtf(disc));
+    xi = fminf(fmaxf(xi, 0.f), 1.f);
+    e.xi = xi;
+    float zz, lj;
+    rqs_value(e, zz, lj);
+    const float x = fminf(fmaxf(e.X + xi * e.w, 0.f), 0.99999994f);
+    uo[L.B[j]] = x;
+    acc += lj;
+  }
+  ld[b] += acc;
+}
+
+// Training direction, backward of layer l: from g = dL/du_l and the loss
+// coefficient c_b (dL/dlog q = -c_b) compute dL/d(raw spline params) (fp32 and
+// bf16 copies) and dL/du_{l-1} for the transformed dimensions; the conditioning
+// dimensions pass g through (their MLP-input gradient is added later).
+__global__ void k_rqs_backward(const float* __restrict__ prm, int ldp, const float* __restrict__ u_prev,
+                               const float* __restrict__ g_in, const float* __restrict__ coef,
+                               float* __restrict__ g_out, float* __restrict__ gp,
+                               __bf16* __restrict__ gp16, int n, int D, LayerIdx L, int K,
+                               float* __restrict__ logq_chk) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  const float* up = u_prev + (size_t)b * D;
+  const float* gi = g_in + (size_t)b * D;
+  float* go = g_out + (size_t)b * D;
+  for (int i = 0; i < L.dA; ++i) go[L.A[i]] = gi[L.A[i]];
+  const float gld = -coef[b];
+  const int np = 3 * K - 1;
+  float ljsum = 0.f;
+  for (int j = 0; j < L.dB; ++j) {
+    const float* p = prm + (size_t)b * ldp + j * np;
+    float* g = gp + (size_t)b * ldp + j * np;
+    __bf16* g16 = gp16 + (size_t)b * ldp + j * np;
+    float sw[kMaxK], sh[kMaxK];
+    softmax_k(p, K, sw);
+    softmax_k(p + K, K, sh);
+    const float x = fminf(fmaxf(up[L.B[j]], 0.f), 1.f);
+    SplineEval e;
+    locate(sw, sh, p + 2 * K, K, x, false, e);
+    const float xi = fminf(fmaxf((x - e.X) / e.w, 0.f), 1.f);
+    e.xi = xi;
+    const float t = xi * (1.f - xi), om = 1.f - xi;
+    const float Dl = e.dk1 + e.dk - 2.f * e.s;
+    const float Dn = e.s + Dl * t;
+    const float N = e.h * (e.s * xi * xi + e.dk * t);
+    const float Pn = e.dk1 * xi * xi + 2.f * e.s * t + e.dk * om * om;
+    ljsum += 2.f * __logf(e.s) + __logf(Pn) - 2.f * __logf(Dn);
+    const float gu = gi[L.B[j]];
+    // partials of N, Dn, Pn with respect to (xi, s, h, dk, dk1)
+    const float N_xi = e.h * (2.f * e.s * xi + e.dk * (1.f - 2.f * xi));
+    const float N_s = e.h * xi * xi, N_h = e.s * xi * xi + e.dk * t, N_dk = e.h * t;
+    const float D_xi = Dl * (1.f - 2.f * xi), D_s = 1.f - 2.f * t, D_dk = t, D_dk1 = t;
+    const float P_xi = 2.f * e.dk1 * xi + 2.f * e.s * (1.f - 2.f * xi) - 2.f * e.dk * om;
+    const float P_s = 2.f * t, P_dk = om * om, P_dk1 = xi * xi;
+    const float iD = 1.f / Dn, iD2 = iD * iD, iP = 1.f / Pn;
+    // G_q = gu dz/dq + gld dlogJ/dq
+    const float G_xi = gu * (N_xi * Dn - N * D_xi) * iD2 + gld * (P_xi * iP - 2.f * D_xi * iD);
+    const float G_s = gu * (N_s * Dn - N * D_s) * iD2 + gld * (2.f / e.s + P_s * iP - 2.f * D_s * iD);
+    const float G_h0 = gu * N_h * iD;
+    const float G_dk = gu * (N_dk * Dn - N * D_dk) * iD2 + gld * (P_dk * iP - 2.f * D_dk * iD);
+    const float G_dk1 = gu * (-N * D_dk1) * iD2 + gld * (P_dk1 * iP - 2.f * D_dk1 * iD);
+    const float G_Y = gu;
+    // chain through xi = (x - X) / w and s = h / w
+    const float iw = 1.f / e.w;
+    const float G_x = G_xi * iw;
+    const float G_X = -G_xi * iw;
+    const float G_w = -G_xi * xi * iw - G_s * e.s * iw;
+    const float G_h = G_h0 + G_s * iw;
+    go[L.B[j]] = G_x;
+    // widths: w_m = eps + cw softmax(a)_m, X = sum_{m<k} w_m
+    const float cw = 1.f - K * kEpsW;
+    float sdotw = sw[e.k] * G_w, sdoth = sh[e.k] * G_h;
+    for (int m = 0; m < e.k; ++m) {
+      sdotw += sw[m] * G_X;
+      sdoth += sh[m] * G_Y;
+    }
+    for (int m = 0; m < K; ++m) {
+      const float Gwm = (m == e.k ? G_w : 0.f) + (m < e.k ? G_X : 0.f);
+      const float Ghm = (m == e.k ? G_h : 0.f) + (m < e.k ? G_Y : 0.f);
+      const float ga = cw * sw[m] * (Gwm - sdotw);
+      const float gb = cw * sh[m] * (Ghm - sdoth);
+      g[m] = ga;
+      g[K + m] = gb;
+      g16[m] = (__bf16)ga;
+      g16[K + m] = (__bf16)gb;
+    }
+    // interior derivatives d_m = eps + softplus(c_{m-1}), m = 1..K-1
+    for (int m = 0; m < K - 1; ++m) {
+      float gc = 0.f;
+      if (m == e.k - 1) gc += G_dk * sigmoidf(p[2 * K + m]);
+      if (m == e.k) gc += G_dk1 * sigmoidf(p[2 * K + m]);
+      g[2 * K + m] = gc;
+      g16[2 * K + m] = (__bf16)gc;
+    }
+  }
+  // zero the padding columns of the parameter gradient
+  for (int c = L.dB * np; c < ldp; ++c) {
+    gp[(size_t)b * ldp + c] = 0.f;
+    gp16[(size_t)b * ldp + c] = (__bf16)0.f;
+  }
+  if (logq_chk) logq_chk[b] += ljsum;
+}
+
+// ---------------------------------------------------------------------------
+// MLP kernels
+// ---------------------------------------------------------------------------
+// Input layer (FP32 VALU): h[b][j] = relu(sum_i u[b][A_i] W1[i][j] + b1[j])
+__global__ void k_mlp_in(const float* __restrict__ u, int D, LayerIdx L, const float* __restrict__ W1,
+                         const float* __restrict__ b1, int H, int n, __bf16* __restrict__ h16,
+                         float* __restrict__ h32) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = (int)(idx / H), j = (int)(idx % H);
+  if (b >= n) return;
+  float acc = b1[j];
+  for (int i = 0; i < L.dA; ++i) acc += u[(size_t)b * D + L.A[i]] * W1[i * H + j];
+  acc = fmaxf(acc, 0.f);
+  if (h16) h16[(size_t)b * H + j] = (__bf16)acc;
+  if (h32) h32[(size_t)b * H + j] = acc;
+}
+
+// Input layer backward: dW1[i][j] partials over batch chunks, db1, and
+// du[b][A_i] += sum_j dh[b][j] W1[i][j].
+__global__ void k_mlp_in_dx(const float* __restrict__ dh, const float* __restrict__ W1, int H, int n,
+                            int D, LayerIdx L, float* __restrict__ g) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  for (int i = 0; i < L.dA; ++i) {
+    float acc = 0.f;
+    for (int j = 0; j < H; ++j) acc += dh[(size_t)b * H + j] * W1[i * H + j];
+    g[(size_t)b * D + L.A[i]] += acc;
+  }
+}
+
+// Column sums over batch chunks (bias gradients and input-layer weight
+// gradients): part[chunk][c] = sum_{b in chunk} X[b][c] * (Y ? Y[b][ycol] : 1)
+__global__ void k_colsum_part(const float* __restrict__ X, int ldx, int ncol, int n, int chunk,
+                              const float* __restrict__ Y, int ldy, int ycol,
+                              float* __restrict__ part) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ch = blockIdx.y;
+  if (c >= ncol) return;
+  const int b0 = ch * chunk, b1 = min(n, b0 + chunk);
+  float acc = 0.f;
+  for (int b = b0; b < b1; ++b)
+    acc += X[(size_t)b * ldx + c] * (Y ? Y[(size_t)b * ldy + ycol] : 1.f);
+  part[(size_t)ch * ncol + c] = acc;
+}
+__global__ void k_part_reduce(const float* __restrict__ part, int nchunk, int ncol,
+                              float* __restrict__ out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncol) return;
+  float acc = 0.f;
+  for (int k = 0; k < nchunk; ++k) acc += part[(size_t)k * ncol + c];
+  out[c] = acc;
+}
+
+// Dense product C[M x N] = A[M x K] * Bt[N x K]^T with row-major A and Bt,
+// 64 x 64 tiles, 4 waves each computing 16 rows x 64 columns.
+// Epilogue: + bias[col], ReLU, multiply by (mask[row][col] > 0), stores bf16 and/or fp32.
+struct Epi {
+  const float* bias;
+  int relu;
+  const __bf16* mask;  // optional, ldm
+  int ldm;
+  __bf16* out16;
+  float* out32;
+  int ldc;
+  int ncols_valid;  // columns >= this are written as zero
+};
+
+template <bool BF16>
+__global__ __launch_bounds__(256) void k_gemm_nt(const void* __restrict__ Av, int lda,
+                                                 const void* __restrict__ Bv, int ldb, int M, int N,
+                                                 int K, Epi ep) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int m0 = blockIdx.x * 64 + wave * 16, n0 = blockIdx.y * 64;
+  f32x4 acc[4];
+  for (int t = 0; t < 4; ++t) acc[t] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  if (BF16) {
+    const __bf16* A = (const __bf16*)Av;
+    const __bf16* Bt = (const __bf16*)Bv;
+    const int ar = m0 + (lane & 15), kq = 8 * (lane >> 4);
+    for (int k0 = 0; k0 < K; k0 += 32) {
+      const bf16x8 a = *(const bf16x8*)(A + (size_t)ar * lda + k0 + kq);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const bf16x8 bb = *(const bf16x8*)(Bt + (size_t)(n0 + 16 * t + (lane & 15)) * ldb + k0 + kq);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bb, acc[t], 0, 0, 0);
+      }
+    }
+  } else {
+    // FP32 VALU path with the same output ownership as the MFMA layout
+    const float* A = (const float*)Av;
+    const float* Bt = (const float*)Bv;
+    for (int t = 0; t < 4; ++t) {
+      const int col = n0 + 16 * t + (lane & 15);
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + 4 * (lane >> 4) + r;
+        float s = 0.f;
+        for (int k = 0; k < K; ++k) s += A[(size_t)row * lda + k] * Bt[(size_t)col * ldb + k];
+        acc[t][r] = s;
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int col = n0 + 16 * t + (lane & 15);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = m0 + 4 * (lane >> 4) + r;
+      float v = acc[t][r];
+      if (ep.bias) v += ep.bias[col];
+      if (ep.relu) v = fmaxf(v, 0.f);
+      if (ep.mask && !((float)ep.mask[(size_t)row * ep.ldm + col] > 0.f)) v = 0.f;
+      if (col >= ep.ncols_valid) v = 0.f;
+      if (ep.out16) ep.out16[(size_t)row * ep.ldc + col] = (__bf16)v;
+      if (ep.out32) ep.out32[(size_t)row * ep.ldc + col] = v;
+    }
+  }
+}
+
+// Weight gradient partials dW[Kin x N] over batch chunks of 256 rows:
+// part[chunk][m][n] = sum_{b in chunk} X[b][m] dY[b][n]; tiles 64 x 64.
+template <bool BF16>
+__global__ __launch_bounds__(256) void k_gemm_tn_part(const void* __restrict__ Xv, int ldx,
+                                                      const void* __restrict__ Yv, int ldy, int Kin,
+                                                      int N, int nrows, int chunk,
+                                                      float* __restrict__ part) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64, ch = blockIdx.z;
+  const int b0 = ch * chunk, b1 = min(nrows, b0 + chunk);
+  f32x4 acc[4];
+  for (int t = 0; t < 4; ++t) acc[t] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  if (BF16) {
+    // stage 32 batch rows of X (64 m) and dY (64 n) in LDS, rows padded by 8
+    __shared__ __bf16 sx[32][72];
+    __shared__ __bf16 sy[32][72];
+    const __bf16* X = (const __bf16*)Xv;
+    const __bf16* Y = (const __bf16*)Yv;
+    for (int bb = b0; bb < b1; bb += 32) {
+      for (int e = threadIdx.x; e < 32 * 64; e += 256) {
+        const int r = e >> 6, c = e & 63;
+        const int brow = bb + r;
+        sx[r][c] = (brow < b1) ? X[(size_t)brow * ldx + m0 + c] : (__bf16)0.f;
+        sy[r][c] = (brow < b1) ? Y[(size_t)brow * ldy + n0 + c] : (__bf16)0.f;
+      }
+      __syncthreads();
+      bf16x8 a;
+      const int kq = 8 * (lane >> 4);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) a[e] = sx[kq + e][wave * 16 + (lane & 15)];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        bf16x8 bv;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) bv[e] = sy[kq + e][16 * t + (lane & 15)];
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc[t], 0, 0, 0);
+      }
+      __syncthreads();
+    }
+  } else {
+    const float* X = (const float*)Xv;
+    const float* Y = (const float*)Yv;
+    for (int t = 0; t < 4; ++t) {
+      const int col = n0 + 16 * t + (lane & 15);
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wave * 16 + 4 * (lane >> 4) + r;
+        float s = 0.f;
+        for (int b = b0; b < b1; ++b) s += X[(size_t)b * ldx + row] * Y[(size_t)b * ldy + col];
+        acc[t][r] = s;
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int col = n0 + 16 * t + (lane & 15);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = m0 + wave * 16 + 4 * (lane >> 4) + r;
+      part[((size_t)ch * Kin + row) * N + col] = acc[t][r];
+    }
+  }
+}
+
+// Adam (Kingma & Ba 2015) on the FP32 master parameters; refreshes the bf16 copies
+__global__ void k_adam(float* __restrict__ th, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, size_t n, float lr, float b1, float b2, float eps,
+                       float c1, float c2, const unsigned char* __restrict__ frozen) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || frozen[i]) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  th[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+}
+
+// W[Kin x N] (row-major, fp32) -> Wt16[N x Kin] and W16[Kin x N] (bf16)
+__global__ void k_weights_bf16(const float* __restrict__ W, int Kin, int N, __bf16* __restrict__ Wt16,
+                               __bf16* __restrict__ W16, float* __restrict__ Wt32) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= Kin * N) return;
+  const int k = idx / N, c = idx % N;
+  const float w = W[idx];
+  Wt16[(size_t)c * Kin + k] = (__bf16)w;
+  W16[idx] = (__bf16)w;
+  Wt32[(size_t)c * Kin + k] = w;
+}
+
+// z ~ U(0,1)^D from Philox (counter = (block, idx, 0, step) under key = seed)
+__global__ void k_flow_z(uint64_t seed, uint32_t step, uint64_t first, int n, int D,
+                         float* __restrict__ z) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  double u[GRA_MAX_DIM];
+  point_uniforms(seed, 0x80000000u | step, first + b, D, u);
+  for (int j = 0; j < D; ++j) z[(size_t)b * D + j] = fminf((float)u[j], 0.99999994f);
+}
+
+// u (flow output, FP32) -> x through the VEGAS grid (FP64), vegas weight
+__global__ void k_u_to_x(const float* __restrict__ u, const double* __restrict__ grid, int D,
+                         unsigned int bins, int n, double* __restrict__ x, double* __restrict__ vw) {
+  __shared__ double xm[GRA_MAX_DIM * GRA_VEGAS_BINS];
+  for (int k = threadIdx.x; k < D * GRA_VEGAS_BINS; k += blockDim.x) xm[k] = grid[k];
+  __syncthreads();
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  double uu[GRA_MAX_DIM], xx[GRA_MAX_DIM];
+  uint8_t bb[GRA_MAX_DIM];
+  for (int j = 0; j < D; ++j) uu[j] = (double)u[(size_t)b * D + j];
+  vw[b] = vegas_map(xm, D, bins, uu, xx, bb);
+  for (int j = 0; j < D; ++j) x[(size_t)b * D + j] = xx[j];
+}
+
+// w_b = f_b vw_b / q_b in FP64 and per-block partial sums (sum w, sum w^2, max w)
+__global__ void k_weights(const double* __restrict__ W, const double* __restrict__ vw,
+                          const float* __restrict__ logq, int n, double* __restrict__ w,
+                          double* __restrict__ part) {
+  __shared__ double s1[256], s2[256], smx[256];
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  double wv = 0.0;
+  if (b < n) {
+    wv = W[b] * vw[b] * exp(-(double)logq[b]);
+    if (!(wv >= 0.0) || isinf(wv)) wv = 0.0;
+    w[b] = wv;
+  }
+  s1[threadIdx.x] = wv;
+  s2[threadIdx.x] = wv * wv;
+  smx[threadIdx.x] = wv;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      s1[threadIdx.x] += s1[threadIdx.x + o];
+      s2[threadIdx.x] += s2[threadIdx.x + o];
+      smx[threadIdx.x] = fmax(smx[threadIdx.x], smx[threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x] = s1[0];
+    part[3 * blockIdx.x + 1] = s2[0];
+    part[3 * blockIdx.x + 2] = smx[0];
+  }
+}
+
+__global__ void k_coef(const double* __restrict__ w, int n, double inv_s2, float* __restrict__ c) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < n) c[b] = (float)(w[b] * w[b] * inv_s2);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
 namespace gra_hip {
-struct FlowState {};
-void flow_release(FlowState* f) { delete f; }
+struct FlowState;
+static FlowState* flow_of(gra_handle* h);
+
+struct FlowLayerHost {
+  LayerIdx idx;
+  size_t oW1, ob1, oWo, obo;
+  size_t oWh[4], obh[4];
+};
+
+struct FlowState {
+  gra_flow_cfg fc;
+  int D = 0, L = 0, K = 0, H = 0, depth = 0, np = 0, ldp = 0;
+  int mlp_fp32 = 0;
+  std::vector<FlowLayerHost> layers;
+  size_t nparam = 0;
+  DevBuf theta, grad, adam_m, adam_v, frozen;
+  DevBuf wt16, w16, wt32;  // per layer, per GEMM weight (hidden and output)
+  std::vector<size_t> wofs;  // element offsets of each GEMM weight copy
+  // batch buffers
+  int cap = 0;
+  DevBuf z, us, ld, acts16, acts32, prm, gprm, gprm16, dact16, dact32, gu0, gu1, coef, xx, vw, W,
+      wts, part, tmp, dh32, grid;
+  int step = 0;
+  std::vector<double> hpart;
+};
+
+void flow_release(FlowState* f) {
+  if (!f) return;
+  DevBuf* b[] = {&f->theta, &f->grad,   &f->adam_m, &f->adam_v, &f->frozen, &f->wt16, &f->w16,
+                 &f->wt32,  &f->z,      &f->us,     &f->ld,     &f->acts16, &f->acts32, &f->prm,
+                 &f->gprm,  &f->gprm16, &f->dact16, &f->dact32, &f->gu0,    &f->gu1,   &f->coef,
+                 &f->xx,    &f->vw,     &f->W,      &f->wts,    &f->part,   &f->tmp,   &f->dh32,
+                 &f->grid};
+  for (DevBuf* x : b) x->release();
+  delete f;
+}
+
+static FlowState* flow_of(gra_handle* h) { return h ? h->flow : nullptr; }
+
+static LayerIdx make_mask(int l, int D) {
+  LayerIdx m;
+  std::memset(&m, 0, sizeof(m));
+  const int pat = l % 4;
+  for (int i = 0; i < D; ++i) {
+    bool inA;
+    if (pat == 0) inA = (i % 2) == 0;
+    else if (pat == 1) inA = (i % 2) == 1;
+    else if (pat == 2) inA = i < D / 2;
+    else inA = i >= D / 2;
+    if (inA) m.A[m.dA++] = i;
+    else m.B[m.dB++] = i;
+  }
+  return m;
+}
+
+static uint64_t splitmix(uint64_t& s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+static int flow_alloc_batch(gra_handle* h, FlowState* F, int n) {
+  if (n <= F->cap) return GRA_OK;
+  const int D = F->D, L = F->L, H = F->H, dep = F->depth, ldp = F->ldp;
+  const size_t N = (size_t)n;
+  bool bad = false;
+  bad |= F->z.ensure(sizeof(float) * N * D) != 0;
+  bad |= F->us.ensure(sizeof(float) * N * D * (L + 1)) != 0;
+  bad |= F->ld.ensure(sizeof(float) * N) != 0;
+  bad |= F->acts16.ensure(sizeof(__bf16) * N * H * dep * L) != 0;
+  bad |= F->acts32.ensure(F->mlp_fp32 ? sizeof(float) * N * H * dep * L : 16) != 0;
+  bad |= F->prm.ensure(sizeof(float) * N * ldp * L) != 0;
+  bad |= F->gprm.ensure(sizeof(float) * N * ldp) != 0;
+  bad |= F->gprm16.ensure(sizeof(__bf16) * N * ldp) != 0;
+  bad |= F->dact16.ensure(sizeof(__bf16) * N * H * 2) != 0;
+  bad |= F->dact32.ensure(sizeof(float) * N * H * 2) != 0;
+  bad |= F->gu0.ensure(sizeof(float) * N * D) != 0;
+  bad |= F->gu1.ensure(sizeof(float) * N * D) != 0;
+  bad |= F->coef.ensure(sizeof(float) * N) != 0;
+  bad |= F->xx.ensure(sizeof(double) * N * D) != 0;
+  bad |= F->vw.ensure(sizeof(double) * N) != 0;
+  bad |= F->W.ensure(sizeof(double) * N) != 0;
+  bad |= F->wts.ensure(sizeof(double) * N) != 0;
+  const size_t nch = (N + 2047) / 2048;
+  const size_t big = std::max((size_t)std::max(H, ldp) * std::max(H, ldp), (size_t)ldp * 16);
+  bad |= F->part.ensure(sizeof(float) * std::max(nch * big, (size_t)3 * ((N + 255) / 256) * 2)) != 0;
+  bad |= F->tmp.ensure(sizeof(double) * 3 * ((N + 255) / 256) + 64) != 0;
+  if (bad) return fail(h, GRA_E_NOMEM, "flow batch buffers for %d points", n);
+  F->cap = n;
+  return GRA_OK;
+}
+
+static float* prm_of(FlowState* F, int l) { return F->prm.as<float>() + (size_t)l * F->cap * F->ldp; }
+static float* u_of(FlowState* F, int l) { return F->us.as<float>() + (size_t)l * F->cap * F->D; }
+static __bf16* act16_of(FlowState* F, int l, int i) {
+  return F->acts16.as<__bf16>() + ((size_t)l * F->depth + i) * F->cap * F->H;
+}
+static float* act32_of(FlowState* F, int l, int i) {
+  return F->acts32.as<float>() + ((size_t)l * F->depth + i) * F->cap * F->H;
+}
+
+// bf16 / fp32 copies of the GEMM weights (hidden layers and output) of every layer
+static int refresh_weights(gra_handle* h, FlowState* F) {
+  int slot = 0;
+  for (int l = 0; l < F->L; ++l) {
+    const FlowLayerHost& Ly = F->layers[l];
+    for (int i = 1; i <= F->depth; ++i) {
+      const bool out = (i == F->depth);
+      const size_t off = out ? Ly.oWo : Ly.oWh[i];
+      const int N = out ? F->ldp : F->H;
+      const int n = F->H * N;
+      hipLaunchKernelGGL(k_weights_bf16, dim3((n + 255) / 256), dim3(256), 0, h->stream,
+                         F->theta.as<float>() + off, F->H, N, F->wt16.as<__bf16>() + F->wofs[slot],
+                         F->w16.as<__bf16>() + F->wofs[slot], F->wt32.as<float>() + F->wofs[slot]);
+      ++slot;
+    }
+  }
+  GRA_HIP_CHECK(h, hipGetLastError());
+  return GRA_OK;
+}
+
+static void gemm_nt(gra_handle* h, FlowState* F, const void* A, int lda, const void* Bt, int ldb,
+                    int M, int N, int K, const Epi& ep) {
+  const dim3 grid(M / 64, N / 64);
+  if (F->mlp_fp32)
+    hipLaunchKernelGGL(k_gemm_nt<false>, grid, dim3(256), 0, h->stream, A, lda, Bt, ldb, M, N, K, ep);
+  else
+    hipLaunchKernelGGL(k_gemm_nt<true>, grid, dim3(256), 0, h->stream, A, lda, Bt, ldb, M, N, K, ep);
+}
+
+// dW[Kin x N] = X^T dY accumulated deterministically into grad + off
+static void gemm_tn_grad(gra_handle* h, FlowState* F, const void* X, int ldx, const void* Y, int ldy,
+                         int Kin, int N, int n, float* gout) {
+  const int chunk = 2048;
+  const int nch = (n + chunk - 1) / chunk;
+  const dim3 grid(Kin / 64, N / 64, nch);
+  if (F->mlp_fp32)
+    hipLaunchKernelGGL(k_gemm_tn_part<false>, grid, dim3(256), 0, h->stream, X, ldx, Y, ldy, Kin, N,
+                       n, chunk, F->part.as<float>());
+  else
+    hipLaunchKernelGGL(k_gemm_tn_part<true>, grid, dim3(256), 0, h->stream, X, ldx, Y, ldy, Kin, N,
+                       n, chunk, F->part.as<float>());
+  const int tot = Kin * N;
+  hipLaunchKernelGGL(k_part_reduce, dim3((tot + 255) / 256), dim3(256), 0, h->stream,
+                     F->part.as<float>(), nch, tot, gout);
+}
+
+static void colsum_grad(gra_handle* h, FlowState* F, const float* X, int ldx, int ncol, int n,
+                        const float* Y, int ldy, int ycol, float* gout) {
+  const int chunk = 2048;
+  const int nch = (n + chunk - 1) / chunk;
+  hipLaunchKernelGGL(k_colsum_part, dim3((ncol + 255) / 256, nch), dim3(256), 0, h->stream, X, ldx,
+                     ncol, n, chunk, Y, ldy, ycol, F->part.as<float>());
+  hipLaunchKernelGGL(k_part_reduce, dim3((ncol + 255) / 256), dim3(256), 0, h->stream,
+                     F->part.as<float>(), nch, ncol, gout);
+}
+
+// MLP of layer l on u[A] -> raw spline parameters (prm_of(l)); activations stored
+static void mlp_forward(gra_handle* h, FlowState* F, int l, const float* u, int n) {
+  const FlowLayerHost& Ly = F->layers[l];
+  const int H = F->H;
+  const float* th = F->theta.as<float>();
+  hipLaunchKernelGGL(k_mlp_in, dim3((unsigned)(((size_t)n * H + 255) / 256)), dim3(256), 0, h->stream, u, F->D, Ly.idx,
+                     th + Ly.oW1, th + Ly.ob1, H, n, act16_of(F, l, 0),
+                     F->mlp_fp32 ? act32_of(F, l, 0) : nullptr);
+  int slot = l * F->depth;
+  for (int i = 1; i <= F->depth; ++i, ++slot) {
+    const bool out = i == F->depth;
+    Epi ep;
+    std::memset(&ep, 0, sizeof(ep));
+    ep.bias = th + (out ? Ly.obo : Ly.obh[i]);
+    ep.relu = out ? 0 : 1;
+    ep.ldc = out ? F->ldp : H;
+    ep.ncols_valid = out ? Ly.idx.dB * F->np : H;
+    if (out) ep.out32 = prm_of(F, l);
+    else {
+      ep.out16 = act16_of(F, l, i);
+      if (F->mlp_fp32) ep.out32 = act32_of(F, l, i);
+    }
+    const int N = out ? F->ldp : H;
+    if (F->mlp_fp32)
+      gemm_nt(h, F, act32_of(F, l, i - 1), H, F->wt32.as<float>() + F->wofs[slot], H, n, N, H, ep);
+    else
+      gemm_nt(h, F, act16_of(F, l, i - 1), H, F->wt16.as<__bf16>() + F->wofs[slot], H, n, N, H, ep);
+  }
+}
+
+// Sampling pass: z -> u_0 (stored per layer), ld = log q(u_0)
+static int flow_sample_pass(gra_handle* h, FlowState* F, int n, uint64_t seed, uint32_t step,
+                            uint64_t first, const float* zin) {
+  const int D = F->D, L = F->L;
+  if (zin)
+    GRA_HIP_CHECK(h, hipMemcpyAsync(u_of(F, L), zin, sizeof(float) * n * D, hipMemcpyHostToDevice,
+                                    h->stream));
+  else
+    hipLaunchKernelGGL(k_flow_z, dim3((n + 255) / 256), dim3(256), 0, h->stream, seed, step, first,
+                       n, D, u_of(F, L));
+  GRA_HIP_CHECK(h, hipMemsetAsync(F->ld.p, 0, sizeof(float) * n, h->stream));
+  for (int l = L - 1; l >= 0; --l) {
+    // layer l maps u_l (stored at slot l) to u_{l+1}; inverting reads slot l+1
+    mlp_forward(h, F, l, u_of(F, l + 1), n);
+    hipLaunchKernelGGL(k_rqs_inverse, dim3((n + 255) / 256), dim3(256), 0, h->stream, prm_of(F, l),
+                       F->ldp, u_of(F, l + 1), u_of(F, l), F->ld.as<float>(), n, D,
+                       F->layers[l].idx, F->K);
+  }
+  GRA_HIP_CHECK(h, hipGetLastError());
+  return GRA_OK;
+}
+
+// Backward of L = -sum_b coef_b log q(u_b) into F->grad (overwritten);
+// requires the stored sampling-pass state of the same points. If logq_chk is
+// given it receives the forward-direction log q recomputed from the splines.
+static int flow_backward(gra_handle* h, FlowState* F, int n, float* logq_chk) {
+  const int D = F->D, L = F->L, H = F->H, ldp = F->ldp;
+  float* g = F->gu0.as<float>();
+  float* g2 = F->gu1.as<float>();
+  float* grad = F->grad.as<float>();
+  GRA_HIP_CHECK(h, hipMemsetAsync(g, 0, sizeof(float) * n * D, h->stream));
+  GRA_HIP_CHECK(h, hipMemsetAsync(grad, 0, sizeof(float) * F->nparam, h->stream));
+  if (logq_chk) GRA_HIP_CHECK(h, hipMemsetAsync(logq_chk, 0, sizeof(float) * n, h->stream));
+  const float* th = F->theta.as<float>();
+  for (int l = L - 1; l >= 0; --l) {
+    const FlowLayerHost& Ly = F->layers[l];
+    // g = dL/du_{l+1}; spline backward gives dL/dprm and dL/du_l (B dims; A passes through)
+    hipLaunchKernelGGL(k_rqs_backward, dim3((n + 255) / 256), dim3(256), 0, h->stream, prm_of(F, l),
+                       ldp, u_of(F, l), g, F->coef.as<float>(), g2, F->gprm.as<float>(),
+                       F->gprm16.as<__bf16>(), n, D, Ly.idx, F->K, logq_chk);
+    // MLP backward from the output layer down
+    const float* dy32 = F->gprm.as<float>();
+    const __bf16* dy16 = F->gprm16.as<__bf16>();
+    int ldy = ldp, ncol = ldp;
+    int slot = l * F->depth + F->depth - 1;
+    int cur = 0;
+    for (int i = F->depth; i >= 1; --i, --slot) {
+      const bool out = i == F->depth;
+      // weight and bias gradients of product i: X = act_{i-1} [n x H], dY [n x ncol]
+      float* gW = grad + (out ? Ly.oWo : Ly.oWh[i]);
+      float* gb = grad + (out ? Ly.obo : Ly.obh[i]);
+      if (F->mlp_fp32)
+        gemm_tn_grad(h, F, act32_of(F, l, i - 1), H, dy32, ldy, H, ncol, n, gW);
+      else
+        gemm_tn_grad(h, F, act16_of(F, l, i - 1), H, dy16, ldy, H, ncol, n, gW);
+      colsum_grad(h, F, dy32, ldy, ncol, n, nullptr, 0, 0, gb);
+      // dX = dY W^T, masked by ReLU'(act_{i-1})
+      Epi ep;
+      std::memset(&ep, 0, sizeof(ep));
+      ep.mask = act16_of(F, l, i - 1);
+      ep.ldm = H;
+      ep.out16 = F->dact16.as<__bf16>() + (size_t)cur * F->cap * H;
+      ep.out32 = F->dact32.as<float>() + (size_t)cur * F->cap * H;
+      ep.ldc = H;
+      ep.ncols_valid = H;
+      if (F->mlp_fp32)
+        gemm_nt(h, F, dy32, ldy, F->theta.as<float>() + (out ? Ly.oWo : Ly.oWh[i]), ncol, n, H,
+                ncol, ep);
+      else
+        gemm_nt(h, F, dy16, ldy, F->w16.as<__bf16>() + F->wofs[slot], ncol, n, H, ncol, ep);
+      dy32 = ep.out32;
+      dy16 = ep.out16;
+      ldy = H;
+      ncol = H;
+      cur ^= 1;
+    }
+    // input layer: dW1[i][j] = sum_b u[b][A_i] dh[b][j], db1, and dL/du[A] += dh W1^T
+    for (int i = 0; i < Ly.idx.dA; ++i)
+      colsum_grad(h, F, dy32, H, H, n, u_of(F, l), D, Ly.idx.A[i], grad + Ly.oW1 + (size_t)i * H);
+    colsum_grad(h, F, dy32, H, H, n, nullptr, 0, 0, grad + Ly.ob1);
+    hipLaunchKernelGGL(k_mlp_in_dx, dim3((n + 255) / 256), dim3(256), 0, h->stream, dy32,
+                       th + Ly.oW1, H, n, D, Ly.idx, g2);
+    std::swap(g, g2);
+  }
+  GRA_HIP_CHECK(h, hipGetLastError());
+  return GRA_OK;
+}
+
+// Integrand weights of the current sample: x = grid(u_0), w = f vw / q (FP64)
+static int flow_weights(gra_handle* h, FlowState* F, int n, double stats[3]) {
+  const int D = F->D;
+  hipLaunchKernelGGL(k_u_to_x, dim3((n + 255) / 256), dim3(256), 0, h->stream, u_of(F, 0),
+                     F->grid.as<double>(), D, (unsigned int)GRA_VEGAS_BINS, n,
+                     F->xx.as<double>(), F->vw.as<double>());
+  int r = ensure_point_buffers(h, n);
+  if (r) return r;
+  r = run_integrand(h, nullptr, 0, 0, 0, 0, F->xx.as<double>(), n, F->W.as<double>(), nullptr,
+                    h->d_flags.as<uint8_t>(), h->d_vw.as<double>(), h->d_bins.as<uint8_t>());
+  if (r) return r;
+  const int nb = (n + 255) / 256;
+  hipLaunchKernelGGL(k_weights, dim3(nb), dim3(256), 0, h->stream, F->W.as<double>(),
+                     F->vw.as<double>(), F->ld.as<float>(), n, F->wts.as<double>(),
+                     F->tmp.as<double>());
+  F->hpart.resize(3 * (size_t)nb);
+  GRA_HIP_CHECK(h, hipMemcpyAsync(F->hpart.data(), F->tmp.p, sizeof(double) * 3 * nb,
+                                  hipMemcpyDeviceToHost, h->stream));
+  GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  double s1 = 0, s2 = 0, mx = 0;
+  for (int k = 0; k < nb; ++k) {
+    s1 += F->hpart[3 * k];
+    s2 += F->hpart[3 * k + 1];
+    mx = std::max(mx, F->hpart[3 * k + 2]);
+  }
+  stats[0] = s1;
+  stats[1] = s2;
+  stats[2] = mx;
+  return GRA_OK;
+}
+
+static int flow_adam(gra_handle* h, FlowState* F) {
+  F->step += 1;
+  const gra_flow_cfg& c = F->fc;
+  const float c1 = (float)(1.0 / (1.0 - std::pow(c.beta1, F->step)));
+  const float c2 = (float)(1.0 / (1.0 - std::pow(c.beta2, F->step)));
+  hipLaunchKernelGGL(k_adam, dim3((unsigned)((F->nparam + 255) / 256)), dim3(256), 0, h->stream,
+                     F->theta.as<float>(), F->grad.as<float>(), F->adam_m.as<float>(),
+                     F->adam_v.as<float>(), F->nparam, (float)c.lr, (float)c.beta1,
+                     (float)c.beta2, (float)c.eps, c1, c2, F->frozen.as<unsigned char>());
+  return refresh_weights(h, F);
+}
+
+static void fill_stats(gra_flow_stats* st, const double s[3], uint64_t n, double seconds) {
+  const double N = (double)n;
+  const double mean = s[0] / N;
+  const double var = std::max(s[1] / N - mean * mean, 0.0);
+  st->sigma = mean;
+  st->sigma_err = std::sqrt(var / N);
+  st->cv2 = mean > 0 ? var / (mean * mean) : 0.0;
+  st->ess = s[1] > 0 ? (s[0] * s[0]) / (N * s[1]) : 0.0;
+  st->maxw = s[2];
+  st->n = n;
+  st->seconds = seconds;
+}
+
 }  // namespace gra_hip
 
 using namespace gra_hip;
 
-#define FLOW_UNAVAILABLE(h)                                               \
-  do {                                                                    \
-    if (!(h)) return set_global_error(GRA_E_ARG, "handle is NULL");       \
-    return fail((h), GRA_E_STATE, "flow sampler is not available");       \
-  } while (0)
+extern "C" int gra_hip_flow_init(gra_handle* h, const gra_flow_cfg* fc_in) {
+  if (!h) return set_global_error(GRA_E_ARG, "handle is NULL");
+  if (!fc_in) return fail(h, GRA_E_ARG, "flow_init: cfg is NULL");
+  gra_flow_cfg fc = *fc_in;
+  const int D = h->P.dim;
+  const bool mlp32 = fc.depth < 0;  // negative depth selects the FP32 MLP path
+  if (mlp32) fc.depth = -fc.depth;
+  if (fc.dim != D) return fail(h, GRA_E_ARG, "flow dim %d != process dim %d", fc.dim, D);
+  if (fc.layers < 1 || fc.layers > 32) return fail(h, GRA_E_ARG, "flow layers %d", fc.layers);
+  if (fc.bins < 2 || fc.bins > kMaxK) return fail(h, GRA_E_ARG, "flow bins %d (2..%d)", fc.bins, kMaxK);
+  if (fc.hidden < 64 || fc.hidden % 64 || fc.hidden > 1024)
+    return fail(h, GRA_E_ARG, "flow hidden %d: multiple of 64 up to 1024", fc.hidden);
+  if (fc.depth < 1 || fc.depth > 4) return fail(h, GRA_E_ARG, "flow depth %d (1..4)", fc.depth);
+  if (D < 2 || D / 2 > kMaxDB + 0 || (D - D / 2) > kMaxDB)
+    return fail(h, GRA_E_ARG, "flow needs 2 <= dim <= %d", 2 * kMaxDB);
+  if (h->flow) {
+    flow_release(h->flow);
+    h->flow = nullptr;
+  }
+  FlowState* F = new (std::nothrow) FlowState();
+  if (!F) return fail(h, GRA_E_NOMEM, "flow state");
+  F->fc = fc;
+  F->mlp_fp32 = mlp32 ? 1 : 0;
+  F->D = D;
+  F->L = fc.layers;
+  F->K = fc.bins;
+  F->H = fc.hidden;
+  F->depth = fc.depth;
+  F->np = 3 * fc.bins - 1;
+  int maxdB = 0;
+  for (int l = 0; l < F->L; ++l) {
+    FlowLayerHost Ly;
+    std::memset(&Ly, 0, sizeof(Ly));
+    Ly.idx = make_mask(l, D);
+    maxdB = std::max(maxdB, Ly.idx.dB);
+    F->layers.push_back(Ly);
+  }
+  F->ldp = ((maxdB * F->np + 63) / 64) * 64;
+  // parameter layout
+  size_t off = 0;
+  for (int l = 0; l < F->L; ++l) {
+    FlowLayerHost& Ly = F->layers[l];
+    Ly.oW1 = off;
+    off += (size_t)Ly.idx.dA * F->H;
+    Ly.ob1 = off;
+    off += F->H;
+    for (int i = 1; i < F->depth; ++i) {
+      Ly.oWh[i] = off;
+      off += (size_t)F->H * F->H;
+      Ly.obh[i] = off;
+      off += F->H;
+    }
+    Ly.oWo = off;
+    off += (size_t)F->H * F->ldp;
+    Ly.obo = off;
+    off += F->ldp;
+  }
+  F->nparam = off;
+  // GEMM weight copies
+  size_t wo = 0;
+  for (int l = 0; l < F->L; ++l)
+    for (int i = 1; i <= F->depth; ++i) {
+      F->wofs.push_back(wo);
+      wo += (size_t)F->H * ((i == F->depth) ? F->ldp : F->H);
+    }
+  if (F->theta.ensure(4 * off) || F->grad.ensure(4 * off) || F->adam_m.ensure(4 * off) ||
+      F->adam_v.ensure(4 * off) || F->frozen.ensure(off) || F->wt16.ensure(2 * wo) ||
+      F->w16.ensure(2 * wo) || F->wt32.ensure(4 * wo)) {
+    flow_release(F);
+    return fail(h, GRA_E_NOMEM, "flow parameters");
+  }
+  // initialisation: He-uniform hidden weights, zero output weights, output biases
+  // giving the identity spline (uniform bins, unit derivatives)
+  std::vector<float> th(off, 0.f);
+  std::vector<unsigned char> fr(off, 0);
+  uint64_t s = fc.seed ^ 0x5DEECE66Dull;
+  auto unif = [&s]() { return (double)(splitmix(s) >> 11) * (1.0 / 9007199254740992.0); };
+  const float dinv = std::log(std::exp(1.0 - kEpsD) - 1.0);
+  for (int l = 0; l < F->L; ++l) {
+    FlowLayerHost& Ly = F->layers[l];
+    const double a1 = std::sqrt(6.0 / std::max(1, Ly.idx.dA));
+    for (int k = 0; k < Ly.idx.dA * F->H; ++k) th[Ly.oW1 + k] = (float)((2 * unif() - 1) * a1);
+    const double ah = std::sqrt(6.0 / F->H);
+    for (int i = 1; i < F->depth; ++i)
+      for (int k = 0; k < F->H * F->H; ++k) th[Ly.oWh[i] + k] = (float)((2 * unif() - 1) * ah);
+    for (int j = 0; j < Ly.idx.dB; ++j)
+      for (int m = 0; m < F->K - 1; ++m) th[Ly.obo + j * F->np + 2 * F->K + m] = dinv;
+    // padding columns of the output layer stay zero and frozen
+    for (int hrow = 0; hrow < F->H; ++hrow)
+      for (int c = Ly.idx.dB * F->np; c < F->ldp; ++c) fr[Ly.oWo + (size_t)hrow * F->ldp + c] = 1;
+    for (int c = Ly.idx.dB * F->np; c < F->ldp; ++c) fr[Ly.obo + c] = 1;
+  }
+  hipError_t e1 = hipMemcpy(F->theta.p, th.data(), 4 * off, hipMemcpyHostToDevice);
+  hipError_t e2 = hipMemcpy(F->frozen.p, fr.data(), off, hipMemcpyHostToDevice);
+  hipError_t e3 = hipMemset(F->adam_m.p, 0, 4 * off);
+  hipError_t e4 = hipMemset(F->adam_v.p, 0, 4 * off);
+  if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess) {
+    flow_release(F);
+    return fail(h, GRA_E_HIP, "flow parameter upload");
+  }
+  h->flow = F;
+  int r = refresh_weights(h, F);
+  if (r) return r;
+  // the VEGAS grid the flow composes with: uniform unless a grid was set
+  if (F->grid.ensure(sizeof(double) * GRA_MAX_DIM * GRA_VEGAS_BINS))
+    return fail(h, GRA_E_NOMEM, "grid buffer");
+  gra_grid g;
+  grid_init(&g, D, GRA_VEGAS_BINS);
+  GRA_HIP_CHECK(h, hipMemcpy(F->grid.p, g.xmat, sizeof(double) * D * GRA_VEGAS_BINS,
+                             hipMemcpyHostToDevice));
+  GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  return GRA_OK;
+}
 
-extern "C" int gra_hip_flow_init(gra_handle* h, const gra_flow_cfg*) { FLOW_UNAVAILABLE(h); }
-extern "C" int gra_hip_flow_train(gra_handle* h, int32_t, gra_flow_stats*) { FLOW_UNAVAILABLE(h); }
-extern "C" int gra_hip_flow_sample(gra_handle* h, uint64_t, uint64_t, gra_flow_stats*) {
-  FLOW_UNAVAILABLE(h);
+extern "C" int gra_hip_flow_set_grid(gra_handle* h, const gra_grid* g) {
+  FlowState* F = flow_of(h);
+  if (!F) return h ? fail(h, GRA_E_STATE, "flow_set_grid: flow not initialised")
+                   : set_global_error(GRA_E_ARG, "handle is NULL");
+  if (!g || g->dim != h->P.dim || g->bins != GRA_VEGAS_BINS)
+    return fail(h, GRA_E_ARG, "flow_set_grid: grid must have the process dim and %d bins",
+                GRA_VEGAS_BINS);
+  GRA_HIP_CHECK(h, hipMemcpy(F->grid.p, g->xmat, sizeof(double) * g->dim * GRA_VEGAS_BINS,
+                             hipMemcpyHostToDevice));
+  return GRA_OK;
 }
-extern "C" int gra_hip_flow_param_count(gra_handle* h, uint64_t*) { FLOW_UNAVAILABLE(h); }
-extern "C" int gra_hip_flow_get_params(gra_handle* h, float*) { FLOW_UNAVAILABLE(h); }
-extern "C" int gra_hip_flow_set_params(gra_handle* h, const float*) { FLOW_UNAVAILABLE(h); }
-extern "C" int gra_hip_flow_forward(gra_handle* h, const float*, size_t, float*, float*) {
-  FLOW_UNAVAILABLE(h);
+
+extern "C" int gra_hip_flow_train(gra_handle* h, int32_t steps, gra_flow_stats* last) {
+  FlowState* F = flow_of(h);
+  if (!F) return h ? fail(h, GRA_E_STATE, "flow_train: flow not initialised")
+                   : set_global_error(GRA_E_ARG, "handle is NULL");
+  if (steps < 0) return fail(h, GRA_E_ARG, "flow_train: steps < 0");
+  const uint64_t B64 = F->fc.batch;
+  if (B64 < 256 || B64 % 256 || B64 > (1u << 24))
+    return fail(h, GRA_E_ARG, "flow batch %llu: multiple of 256 up to 2^24",
+                (unsigned long long)B64);
+  const int n = (int)B64;
+  int r = flow_alloc_batch(h, F, n);
+  if (r) return r;
+  const auto t0 = std::chrono::steady_clock::now();
+  double st[3] = {0, 0, 0};
+  for (int it = 0; it < steps; ++it) {
+    r = flow_sample_pass(h, F, n, F->fc.seed, (uint32_t)F->step, 0, nullptr);
+    if (r) return r;
+    r = flow_weights(h, F, n, st);
+    if (r) return r;
+    if (!(st[1] > 0)) return fail(h, GRA_E_NUMERIC, "flow_train: all weights are zero");
+    hipLaunchKernelGGL(k_coef, dim3((n + 255) / 256), dim3(256), 0, h->stream, F->wts.as<double>(),
+                       n, 1.0 / st[1], F->coef.as<float>());
+    r = flow_backward(h, F, n, nullptr);
+    if (r) return r;
+    r = flow_adam(h, F);
+    if (r) return r;
+  }
+  GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  if (last) {
+    std::memset(last, 0, sizeof(*last));
+    fill_stats(last, st, (uint64_t)n,
+               std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+    last->loss = st[1] > 0 ? st[1] / n : 0.0;
+  }
+  return GRA_OK;
 }
-extern "C" int gra_hip_flow_grad(gra_handle* h, const float*, const float*, size_t, float*, float*) {
-  FLOW_UNAVAILABLE(h);
+
+extern "C" int gra_hip_flow_sample(gra_handle* h, uint64_t n_total, uint64_t seed,
+                                   gra_flow_stats* out) {
+  FlowState* F = flow_of(h);
+  if (!F) return h ? fail(h, GRA_E_STATE, "flow_sample: flow not initialised")
+                   : set_global_error(GRA_E_ARG, "handle is NULL");
+  if (!out) return fail(h, GRA_E_ARG, "flow_sample: out is NULL");
+  if (n_total == 0) return fail(h, GRA_E_ARG, "flow_sample: n = 0");
+  // chunks are multiples of 256 points (the MLP products tile rows by 64); the
+  // last chunk is padded with extra flow points that do not enter the sums
+  uint64_t c0 = F->fc.batch > 0 ? F->fc.batch : (1u << 18);
+  c0 = std::min<uint64_t>(std::max<uint64_t>(256, (c0 / 256) * 256), 1u << 20);
+  const int chunk = (int)c0;
+  int r = flow_alloc_batch(h, F, chunk);
+  if (r) return r;
+  const auto t0 = std::chrono::steady_clock::now();
+  double acc[3] = {0, 0, 0};
+  uint64_t done = 0;
+  uint32_t call = 0;
+  while (done < n_total) {
+    const int n = (int)std::min<uint64_t>(chunk, n_total - done);
+    const int npad = ((n + 255) / 256) * 256;
+    r = flow_sample_pass(h, F, npad, seed, 0x40000000u | call, done, nullptr);
+    if (r) return r;
+    double st[3];
+    r = flow_weights(h, F, n, st);
+    if (r) return r;
+    acc[0] += st[0];
+    acc[1] += st[1];
+    acc[2] = std::max(acc[2], st[2]);
+    done += n;
+    ++call;
+  }
+  std::memset(out, 0, sizeof(*out));
+  fill_stats(out, acc, n_total,
+             std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+  out->fallback = (out->ess < F->fc.ess_fallback) ? 1 : 0;
+  return GRA_OK;
+}
+
+extern "C" int gra_hip_flow_param_count(gra_handle* h, uint64_t* count) {
+  FlowState* F = flow_of(h);
+  if (!F) return h ? fail(h, GRA_E_STATE, "flow not initialised")
+                   : set_global_error(GRA_E_ARG, "handle is NULL");
+  if (count) *count = F->nparam;
+  return GRA_OK;
+}
+
+extern "C" int gra_hip_flow_get_params(gra_handle* h, float* params) {
+  FlowState* F = flow_of(h);
+  if (!F) return h ? fail(h, GRA_E_STATE, "flow not initialised")
+                   : set_global_error(GRA_E_ARG, "handle is NULL");
+  if (!params) return fail(h, GRA_E_ARG, "params is NULL");
+  GRA_HIP_CHECK(h, hipMemcpy(params, F->theta.p, 4 * F->nparam, hipMemcpyDeviceToHost));
+  return GRA_OK;
+}
+
+extern "C" int gra_hip_flow_set_params(gra_handle* h, const float* params) {
+  FlowState* F = flow_of(h);
+  if (!F) return h ? fail(h, GRA_E_STATE, "flow not initialised")
+                   : set_global_error(GRA_E_ARG, "handle is NULL");
+  if (!params) return fail(h, GRA_E_ARG, "params is NULL");
+  GRA_HIP_CHECK(h, hipMemcpy(F->theta.p, params, 4 * F->nparam, hipMemcpyHostToDevice));
+  return refresh_weights(h, F);
+}
+
+extern "C" int gra_hip_flow_forward(gra_handle* h, const float* z, size_t n, float* x,
+                                    float* logq) {
+  FlowState* F = flow_of(h);
+  if (!F) return h ? fail(h, GRA_E_STATE, "flow not initialised")
+                   : set_global_error(GRA_E_ARG, "handle is NULL");
+  if (!z || !x || !logq) return fail(h, GRA_E_ARG, "flow_forward: NULL buffer");
+  if (n == 0 || n % 64 || n > (1u << 24)) return fail(h, GRA_E_ARG, "flow_forward: n %% 64 != 0");
+  int r = flow_alloc_batch(h, F, (int)n);
+  if (r) return r;
+  r = flow_sample_pass(h, F, (int)n, 0, 0, 0, z);
+  if (r) return r;
+  GRA_HIP_CHECK(h, hipMemcpyAsync(x, u_of(F, 0), sizeof(float) * n * F->D, hipMemcpyDeviceToHost,
+                                  h->stream));
+  GRA_HIP_CHECK(h, hipMemcpyAsync(logq, F->ld.p, sizeof(float) * n, hipMemcpyDeviceToHost,
+                                  h->stream));
+  GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  return GRA_OK;
+}
+
+// Gradient at caller points u (flow output space) and coefficients c. The
+// sampling pass is reconstructed by running the forward direction layer by
+// layer (MLP on the conditioning half, forward spline on the other half).
+namespace {
+__global__ void k_rqs_forward(const float* __restrict__ prm, int ldp, const float* __restrict__ u_in,
+                              float* __restrict__ u_out, int n, int D, LayerIdx L, int K) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  const float* uin = u_in + (size_t)b * D;
+  float* uo = u_out + (size_t)b * D;
+  for (int i = 0; i < L.dA; ++i) uo[L.A[i]] = uin[L.A[i]];
+  const int np = 3 * K - 1;
+  for (int j = 0; j < L.dB; ++j) {
+    const float* p = prm + (size_t)b * ldp + j * np;
+    float sw[kMaxK], sh[kMaxK];
+    softmax_k(p, K, sw);
+    softmax_k(p + K, K, sh);
+    const float x = fminf(fmaxf(uin[L.B[j]], 0.f), 1.f);
+    SplineEval e;
+    locate(sw, sh, p + 2 * K, K, x, false, e);
+    e.xi = fminf(fmaxf((x - e.X) / e.w, 0.f), 1.f);
+    float z, lj;
+    rqs_value(e, z, lj);
+    uo[L.B[j]] = z;
+  }
+}
+}  // namespace
+
+extern "C" int gra_hip_flow_grad(gra_handle* h, const float* u, const float* c, size_t n,
+                                 float* grad, float* logq) {
+  FlowState* F = flow_of(h);
+  if (!F) return h ? fail(h, GRA_E_STATE, "flow not initialised")
+                   : set_global_error(GRA_E_ARG, "handle is NULL");
+  if (!u || !c || !grad) return fail(h, GRA_E_ARG, "flow_grad: NULL buffer");
+  if (n == 0 || n % 64 || n > (1u << 24)) return fail(h, GRA_E_ARG, "flow_grad: n %% 64 != 0");
+  const int N = (int)n, D = F->D;
+  int r = flow_alloc_batch(h, F, N);
+  if (r) return r;
+  GRA_HIP_CHECK(h, hipMemcpyAsync(u_of(F, 0), u, sizeof(float) * n * D, hipMemcpyHostToDevice,
+                                  h->stream));
+  GRA_HIP_CHECK(h, hipMemcpyAsync(F->coef.p, c, sizeof(float) * n, hipMemcpyHostToDevice,
+                                  h->stream));
+  for (int l = 0; l < F->L; ++l) {
+    mlp_forward(h, F, l, u_of(F, l), N);
+    hipLaunchKernelGGL(k_rqs_forward, dim3((N + 255) / 256), dim3(256), 0, h->stream, prm_of(F, l),
+                       F->ldp, u_of(F, l), u_of(F, l + 1), N, D, F->layers[l].idx, F->K);
+  }
+  r = flow_backward(h, F, N, F->ld.as<float>());
+  if (r) return r;
+  GRA_HIP_CHECK(h, hipMemcpyAsync(grad, F->grad.p, 4 * F->nparam, hipMemcpyDeviceToHost, h->stream));
+  if (logq)
+    GRA_HIP_CHECK(h, hipMemcpyAsync(logq, F->ld.p, sizeof(float) * n, hipMemcpyDeviceToHost,
+                                    h->stream));
+  GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  return GRA_OK;
 }

@@ -175,7 +175,8 @@ typedef struct gra_flow_cfg {
   int32_t layers;    /* coupling layers */
   int32_t bins;      /* spline bins K */
   int32_t hidden;    /* MLP width (multiple of 32) */
-  int32_t depth;     /* hidden layers per MLP (>= 1) */
+  int32_t depth;     /* hidden layers per MLP (1..4); a negative value selects the FP32
+                        VALU MLP path (gradient checks) instead of BF16 MFMA */
   int32_t steps;     /* Adam steps */
   uint64_t batch;    /* points per training step */
   double lr;         /* Adam learning rate */
@@ -269,6 +270,9 @@ int gra_hip_eikonal_tables(gra_handle* h, double* node_amp, double* mbt, double*
 /* Create / train the flow sampler in the NEURO integrator slot
  * (MGraniitti::CallIntegrator, src/MGraniitti.cc:1056-1071; SampleNeuro 1527-1648). */
 int gra_hip_flow_init(gra_handle* h, const gra_flow_cfg* fc);
+/* VEGAS grid the flow composes with (x = grid(u), weight f(x) vw(u) / q(u));
+ * uniform after flow_init. */
+int gra_hip_flow_set_grid(gra_handle* h, const gra_grid* grid);
 int gra_hip_flow_train(gra_handle* h, int32_t steps, gra_flow_stats* last);
 /* Sample n points from the flow, evaluate the integrand and report sigma, CV^2,
  * ESS; W (nullable, device size n) receives w = f/q. */
@@ -279,11 +283,12 @@ int gra_hip_flow_sample(gra_handle* h, uint64_t n, uint64_t seed, gra_flow_stats
 int gra_hip_flow_param_count(gra_handle* h, uint64_t* count);
 int gra_hip_flow_get_params(gra_handle* h, float* params);
 int gra_hip_flow_set_params(gra_handle* h, const float* params);
-/* Forward of the flow in the sampling direction on caller points: z (n x dim)
- * -> x (n x dim), logq (n). Host pointers. */
+/* Sampling direction on caller points: z (n x dim, n a multiple of 64) ->
+ * u (n x dim) in the grid's unit cube, logq (n) = log q(u). Host pointers. */
 int gra_hip_flow_forward(gra_handle* h, const float* z, size_t n, float* x, float* logq);
-/* Gradient of L = -sum_i c_i log q(x_i) with respect to every parameter, at
- * caller points x and coefficients c. Host pointers. */
+/* Gradient of L = -sum_i c_i log q(u_i) with respect to every parameter, at
+ * caller points u (n a multiple of 64) and coefficients c; logq (nullable)
+ * receives log q(u). Host pointers. */
 int gra_hip_flow_grad(gra_handle* h, const float* x, const float* c, size_t n, float* grad,
                       float* logq);
 
