@@ -43,53 +43,90 @@ __device__ __forceinline__ float softplusf(float x) {
 }
 __device__ __forceinline__ float sigmoidf(float x) { return 1.f / (1.f + __expf(-x)); }
 
-// Spline knots of one transformed dimension from its raw parameters
-// p[0..K) widths, p[K..2K) heights, p[2K..3K-1) interior derivatives.
-struct Knots {
-  float sw[kMaxK], sh[kMaxK];  // softmax of widths / heights
-};
-__device__ __forceinline__ void softmax_k(const float* a, int K, float* out) {
-  float mx = a[0];
-  for (int k = 1; k < K; ++k) mx = fmaxf(mx, a[k]);
-  float sum = 0.f;
-  for (int k = 0; k < K; ++k) {
-    out[k] = __expf(a[k] - mx);
-    sum += out[k];
+// ---------------------------------------------------------------------------
+// RQS kernels. One spline (one point, one transformed dimension) is handled by
+// a group of G lanes (G = 8, 16 or 32, K <= G): lane m owns bin m. The softmax
+// of the widths and heights is a group max / sum, the knot positions are an
+// exclusive prefix sum over the group, the bin of a point is the population
+// count of a ballot, and the raw parameters and their gradients are read and
+// written as contiguous runs (coalesced), not one 3K-1 row per thread.
+// ---------------------------------------------------------------------------
+template <int G>
+__device__ __forceinline__ float grp_max(float v) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, G));
+  return v;
+}
+template <int G>
+__device__ __forceinline__ float grp_sum(float v) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+  return v;
+}
+template <int G>
+__device__ __forceinline__ float grp_excl_scan(float v, int m) {
+  float s = v;
+#pragma unroll
+  for (int d = 1; d < G; d <<= 1) {
+    const float t = __shfl_up(s, d, G);
+    if (m >= d) s += t;
   }
-  const float inv = 1.f / sum;
-  for (int k = 0; k < K; ++k) out[k] *= inv;
+  return s - v;
+}
+// number of lanes of this lane's group whose predicate holds
+template <int G>
+__device__ __forceinline__ int grp_count(bool pred) {
+  const unsigned long long bl = __ballot(pred);
+  const int base = (threadIdx.x & 63) & ~(G - 1);
+  const unsigned long long msk = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
+  return __popcll((bl >> base) & msk);
 }
 
-struct SplineEval {
-  int k;
-  float X, w, Y, h, dk, dk1, xi, s;
+// Knots of one spline: lane m (< K) holds bin m's width, height, lower knot
+// (X, Y) and normalised softmax weights; p[0..K) widths, p[K..2K) heights,
+// p[2K..3K-1) interior derivatives (raw).
+struct BinLane {
+  float w, h, X, Y, sw, sh;
 };
-
-// Bin search in the cumulative widths; returns the bin and its parameters
-__device__ __forceinline__ void locate(const float* sw, const float* sh, const float* c, int K,
-                                       float v, bool by_y, SplineEval& e) {
+template <int G>
+__device__ __forceinline__ void grp_knots(const float* __restrict__ p, int K, int m, BinLane& q) {
+  const bool act = m < K;
+  const float aw = act ? p[m] : -3.0e38f;
+  const float ah = act ? p[K + m] : -3.0e38f;
+  const float mw = grp_max<G>(aw), mh = grp_max<G>(ah);
+  const float ew = act ? __expf(aw - mw) : 0.f;
+  const float eh = act ? __expf(ah - mh) : 0.f;
+  const float iw = 1.f / grp_sum<G>(ew), ih = 1.f / grp_sum<G>(eh);
   const float cw = 1.f - K * kEpsW;
-  float X = 0.f, Y = 0.f;
-  int k = 0;
-  for (; k < K - 1; ++k) {
-    const float w = kEpsW + cw * sw[k], h = kEpsW + cw * sh[k];
-    if ((by_y ? Y + h : X + w) > v) break;
-    X += w;
-    Y += h;
-  }
+  q.sw = ew * iw;
+  q.sh = eh * ih;
+  q.w = act ? kEpsW + cw * q.sw : 0.f;
+  q.h = act ? kEpsW + cw * q.sh : 0.f;
+  q.X = grp_excl_scan<G>(q.w, m);
+  q.Y = grp_excl_scan<G>(q.h, m);
+}
+
+// The selected bin's parameters, identical in every lane of the group
+struct Bin {
+  int k;
+  float X, w, Y, h, dk, dk1, s;
+};
+template <int G>
+__device__ __forceinline__ void grp_bin(const float* __restrict__ p, int K, const BinLane& q, int k,
+                                        Bin& e) {
   e.k = k;
-  e.X = X;
-  e.Y = Y;
-  e.w = kEpsW + cw * sw[k];
-  e.h = kEpsW + cw * sh[k];
-  e.dk = (k == 0) ? 1.f : kEpsD + softplusf(c[k - 1]);
-  e.dk1 = (k == K - 1) ? 1.f : kEpsD + softplusf(c[k]);
+  e.X = __shfl(q.X, k, G);
+  e.w = __shfl(q.w, k, G);
+  e.Y = __shfl(q.Y, k, G);
+  e.h = __shfl(q.h, k, G);
+  e.dk = (k == 0) ? 1.f : kEpsD + softplusf(p[2 * K + k - 1]);
+  e.dk1 = (k == K - 1) ? 1.f : kEpsD + softplusf(p[2 * K + k]);
   e.s = e.h / e.w;
 }
 
 // Forward value and log-derivative at xi
-__device__ __forceinline__ void rqs_value(const SplineEval& e, float& z, float& logJ) {
-  const float xi = e.xi, t = xi * (1.f - xi);
+__device__ __forceinline__ void rqs_value(const Bin& e, float xi, float& z, float& logJ) {
+  const float t = xi * (1.f - xi);
   const float Dn = e.s + (e.dk1 + e.dk - 2.f * e.s) * t;
   const float N = e.h * (e.s * xi * xi + e.dk * t);
   z = e.Y + N / Dn;
@@ -97,9 +134,6 @@ __device__ __forceinline__ void rqs_value(const SplineEval& e, float& z, float& 
   logJ = 2.f * __logf(e.s) + __logf(Pn) - 2.f * __logf(Dn);
 }
 
-// ---------------------------------------------------------------------------
-// RQS kernels
-// ---------------------------------------------------------------------------
 struct LayerIdx {
   int dA, dB;
   int A[16], B[16];
@@ -107,24 +141,27 @@ struct LayerIdx {
 
 // Sampling direction: u_{l-1}[B] = RQS^{-1}(u_l[B]), u_{l-1}[A] = u_l[A];
 // ld[b] += log dF_l at the pre-image.
-__global__ void k_rqs_inverse(const float* __restrict__ prm, int ldp, const float* __restrict__ u_in,
-                              float* __restrict__ u_out, float* __restrict__ ld, int n, int D,
-                              LayerIdx L, int K) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+template <int G>
+__global__ __launch_bounds__(256) void k_rqs_inverse(const float* __restrict__ prm, int ldp,
+                                                     const float* __restrict__ u_in,
+                                                     float* __restrict__ u_out, float* __restrict__ ld,
+                                                     int n, int D, LayerIdx L, int K) {
+  const int m = threadIdx.x & (G - 1);
+  const int b = (int)(((size_t)blockIdx.x * blockDim.x + threadIdx.x) / G);
   if (b >= n) return;
   const float* uin = u_in + (size_t)b * D;
   float* uo = u_out + (size_t)b * D;
-  for (int i = 0; i < L.dA; ++i) uo[L.A[i]] = uin[L.A[i]];
+  if (m < L.dA) uo[L.A[m]] = uin[L.A[m]];
   float acc = 0.f;
   const int np = 3 * K - 1;
   for (int j = 0; j < L.dB; ++j) {
     const float* p = prm + (size_t)b * ldp + j * np;
-    float sw[kMaxK], sh[kMaxK];
-    softmax_k(p, K, sw);
-    softmax_k(p + K, K, sh);
-    float zv = fminf(fmaxf(uin[L.B[j]], 0.f), 1.f);
-    SplineEval e;
-    locate(sw, sh, p + 2 * K, K, zv, true, e);
+    BinLane q;
+    grp_knots<G>(p, K, m, q);
+    const float zv = fminf(fmaxf(uin[L.B[j]], 0.f), 1.f);
+    const int k = max(grp_count<G>(m < K && q.Y <= zv) - 1, 0);
+    Bin e;
+    grp_bin<G>(p, K, q, k, e);
     const float zeta = zv - e.Y;
     const float Dl = e.dk1 + e.dk - 2.f * e.s;
     const float qa = e.h * (e.s - e.dk) + zeta * Dl;
@@ -133,46 +170,75 @@ __global__ void k_rqs_inverse(const float* __restrict__ prm, int ldp, const floa
     const float disc = fmaxf(qb * qb - 4.f * qa * qc, 0.f);
     float xi = (2.f * qc) / (-qb - sqrtf(disc));
     xi = fminf(fmaxf(xi, 0.f), 1.f);
-    e.xi = xi;
     float zz, lj;
-    rqs_value(e, zz, lj);
-    const float x = fminf(fmaxf(e.X + xi * e.w, 0.f), 0.99999994f);
-    uo[L.B[j]] = x;
+    rqs_value(e, xi, zz, lj);
+    if (m == 0) uo[L.B[j]] = fminf(fmaxf(e.X + xi * e.w, 0.f), 0.99999994f);
     acc += lj;
   }
-  ld[b] += acc;
+  if (m == 0) ld[b] += acc;
+}
+
+// Density direction: u_l[B] = RQS(u_{l-1}[B]) (reconstructs a sampling pass's
+// layer inputs from the flow's output points)
+template <int G>
+__global__ __launch_bounds__(256) void k_rqs_forward(const float* __restrict__ prm, int ldp,
+                                                     const float* __restrict__ u_in,
+                                                     float* __restrict__ u_out, int n, int D,
+                                                     LayerIdx L, int K) {
+  const int m = threadIdx.x & (G - 1);
+  const int b = (int)(((size_t)blockIdx.x * blockDim.x + threadIdx.x) / G);
+  if (b >= n) return;
+  const float* uin = u_in + (size_t)b * D;
+  float* uo = u_out + (size_t)b * D;
+  if (m < L.dA) uo[L.A[m]] = uin[L.A[m]];
+  const int np = 3 * K - 1;
+  for (int j = 0; j < L.dB; ++j) {
+    const float* p = prm + (size_t)b * ldp + j * np;
+    BinLane q;
+    grp_knots<G>(p, K, m, q);
+    const float x = fminf(fmaxf(uin[L.B[j]], 0.f), 1.f);
+    const int k = max(grp_count<G>(m < K && q.X <= x) - 1, 0);
+    Bin e;
+    grp_bin<G>(p, K, q, k, e);
+    const float xi = fminf(fmaxf((x - e.X) / e.w, 0.f), 1.f);
+    float z, lj;
+    rqs_value(e, xi, z, lj);
+    if (m == 0) uo[L.B[j]] = z;
+  }
 }
 
 // Training direction, backward of layer l: from g = dL/du_l and the loss
 // coefficient c_b (dL/dlog q = -c_b) compute dL/d(raw spline params) (fp32 and
 // bf16 copies) and dL/du_{l-1} for the transformed dimensions; the conditioning
 // dimensions pass g through (their MLP-input gradient is added later).
-__global__ void k_rqs_backward(const float* __restrict__ prm, int ldp, const float* __restrict__ u_prev,
-                               const float* __restrict__ g_in, const float* __restrict__ coef,
-                               float* __restrict__ g_out, float* __restrict__ gp,
-                               __bf16* __restrict__ gp16, int n, int D, LayerIdx L, int K,
-                               float* __restrict__ logq_chk) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+template <int G>
+__global__ __launch_bounds__(256) void k_rqs_backward(
+    const float* __restrict__ prm, int ldp, const float* __restrict__ u_prev,
+    const float* __restrict__ g_in, const float* __restrict__ coef, float* __restrict__ g_out,
+    float* __restrict__ gp, __bf16* __restrict__ gp16, int n, int D, LayerIdx L, int K,
+    float* __restrict__ logq_chk) {
+  const int m = threadIdx.x & (G - 1);
+  const int b = (int)(((size_t)blockIdx.x * blockDim.x + threadIdx.x) / G);
   if (b >= n) return;
   const float* up = u_prev + (size_t)b * D;
   const float* gi = g_in + (size_t)b * D;
   float* go = g_out + (size_t)b * D;
-  for (int i = 0; i < L.dA; ++i) go[L.A[i]] = gi[L.A[i]];
+  if (m < L.dA) go[L.A[m]] = gi[L.A[m]];
   const float gld = -coef[b];
   const int np = 3 * K - 1;
+  const float cw = 1.f - K * kEpsW;
   float ljsum = 0.f;
   for (int j = 0; j < L.dB; ++j) {
     const float* p = prm + (size_t)b * ldp + j * np;
     float* g = gp + (size_t)b * ldp + j * np;
     __bf16* g16 = gp16 + (size_t)b * ldp + j * np;
-    float sw[kMaxK], sh[kMaxK];
-    softmax_k(p, K, sw);
-    softmax_k(p + K, K, sh);
+    BinLane q;
+    grp_knots<G>(p, K, m, q);
     const float x = fminf(fmaxf(up[L.B[j]], 0.f), 1.f);
-    SplineEval e;
-    locate(sw, sh, p + 2 * K, K, x, false, e);
+    const int k = max(grp_count<G>(m < K && q.X <= x) - 1, 0);
+    Bin e;
+    grp_bin<G>(p, K, q, k, e);
     const float xi = fminf(fmaxf((x - e.X) / e.w, 0.f), 1.f);
-    e.xi = xi;
     const float t = xi * (1.f - xi), om = 1.f - xi;
     const float Dl = e.dk1 + e.dk - 2.f * e.s;
     const float Dn = e.s + Dl * t;
@@ -196,43 +262,38 @@ __global__ void k_rqs_backward(const float* __restrict__ prm, int ldp, const flo
     const float G_Y = gu;
     // chain through xi = (x - X) / w and s = h / w
     const float iw = 1.f / e.w;
-    const float G_x = G_xi * iw;
     const float G_X = -G_xi * iw;
     const float G_w = -G_xi * xi * iw - G_s * e.s * iw;
     const float G_h = G_h0 + G_s * iw;
-    go[L.B[j]] = G_x;
-    // widths: w_m = eps + cw softmax(a)_m, X = sum_{m<k} w_m
-    const float cw = 1.f - K * kEpsW;
-    float sdotw = sw[e.k] * G_w, sdoth = sh[e.k] * G_h;
-    for (int m = 0; m < e.k; ++m) {
-      sdotw += sw[m] * G_X;
-      sdoth += sh[m] * G_Y;
-    }
-    for (int m = 0; m < K; ++m) {
-      const float Gwm = (m == e.k ? G_w : 0.f) + (m < e.k ? G_X : 0.f);
-      const float Ghm = (m == e.k ? G_h : 0.f) + (m < e.k ? G_Y : 0.f);
-      const float ga = cw * sw[m] * (Gwm - sdotw);
-      const float gb = cw * sh[m] * (Ghm - sdoth);
+    if (m == 0) go[L.B[j]] = G_xi * iw;
+    // widths: w_m = eps + cw softmax(a)_m, X_k = sum_{m<k} w_m (heights alike)
+    const bool act = m < K;
+    const float Gwm = (m == k ? G_w : 0.f) + (m < k ? G_X : 0.f);
+    const float Ghm = (m == k ? G_h : 0.f) + (m < k ? G_Y : 0.f);
+    const float sdw = grp_sum<G>(act ? q.sw * Gwm : 0.f);
+    const float sdh = grp_sum<G>(act ? q.sh * Ghm : 0.f);
+    if (act) {
+      const float ga = cw * q.sw * (Gwm - sdw);
+      const float gb = cw * q.sh * (Ghm - sdh);
       g[m] = ga;
       g[K + m] = gb;
       g16[m] = (__bf16)ga;
       g16[K + m] = (__bf16)gb;
     }
-    // interior derivatives d_m = eps + softplus(c_{m-1}), m = 1..K-1
-    for (int m = 0; m < K - 1; ++m) {
-      float gc = 0.f;
-      if (m == e.k - 1) gc += G_dk * sigmoidf(p[2 * K + m]);
-      if (m == e.k) gc += G_dk1 * sigmoidf(p[2 * K + m]);
+    // interior derivatives d_{m+1} = eps + softplus(c_m), m = 0..K-2
+    if (m < K - 1) {
+      const float sg = sigmoidf(p[2 * K + m]);
+      const float gc = (m == k - 1 ? G_dk * sg : 0.f) + (m == k ? G_dk1 * sg : 0.f);
       g[2 * K + m] = gc;
       g16[2 * K + m] = (__bf16)gc;
     }
   }
   // zero the padding columns of the parameter gradient
-  for (int c = L.dB * np; c < ldp; ++c) {
+  for (int c = L.dB * np + m; c < ldp; c += G) {
     gp[(size_t)b * ldp + c] = 0.f;
     gp16[(size_t)b * ldp + c] = (__bf16)0.f;
   }
-  if (logq_chk) logq_chk[b] += ljsum;
+  if (logq_chk && m == 0) logq_chk[b] += ljsum;
 }
 
 // ---------------------------------------------------------------------------
@@ -252,40 +313,47 @@ __global__ void k_mlp_in(const float* __restrict__ u, int D, LayerIdx L, const f
   if (h32) h32[(size_t)b * H + j] = acc;
 }
 
-// Input layer backward: dW1[i][j] partials over batch chunks, db1, and
-// du[b][A_i] += sum_j dh[b][j] W1[i][j].
-__global__ void k_mlp_in_dx(const float* __restrict__ dh, const float* __restrict__ W1, int H, int n,
-                            int D, LayerIdx L, float* __restrict__ g) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+// Input layer backward, dx part: du[b][A_i] += sum_j dh[b][j] W1[i][j]. One
+// wave per point: lanes stride the H columns (coalesced row reads), then a
+// butterfly sum over the wave per conditioning dimension.
+__global__ __launch_bounds__(256) void k_mlp_in_dx(const float* __restrict__ dh,
+                                                   const float* __restrict__ W1, int H, int n, int D,
+                                                   LayerIdx L, float* __restrict__ g) {
+  const int lane = threadIdx.x & 63;
+  const int b = (int)(((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   if (b >= n) return;
-  for (int i = 0; i < L.dA; ++i) {
-    float acc = 0.f;
-    for (int j = 0; j < H; ++j) acc += dh[(size_t)b * H + j] * W1[i * H + j];
-    g[(size_t)b * D + L.A[i]] += acc;
+  const float* row = dh + (size_t)b * H;
+  float acc[kMaxDB];
+#pragma unroll
+  for (int i = 0; i < kMaxDB; ++i) acc[i] = 0.f;
+  for (int j = lane; j < H; j += 64) {
+    const float d = row[j];
+#pragma unroll
+    for (int i = 0; i < kMaxDB; ++i)
+      if (i < L.dA) acc[i] += d * W1[i * H + j];
+  }
+#pragma unroll
+  for (int i = 0; i < kMaxDB; ++i) {
+    if (i < L.dA) {
+      float v = acc[i];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      if (lane == 0) g[(size_t)b * D + L.A[i]] += v;
+    }
   }
 }
 
-// Column sums over batch chunks (bias gradients and input-layer weight
-// gradients): part[chunk][c] = sum_{b in chunk} X[b][c] * (Y ? Y[b][ycol] : 1)
-__global__ void k_colsum_part(const float* __restrict__ X, int ldx, int ncol, int n, int chunk,
-                              const float* __restrict__ Y, int ldy, int ycol,
-                              float* __restrict__ part) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const int ch = blockIdx.y;
-  if (c >= ncol) return;
-  const int b0 = ch * chunk, b1 = min(n, b0 + chunk);
+// Sum of the per-chunk weight-gradient partials: rows < kw go to gW[row][col],
+// row kw (the all-ones input row) to the bias gradient gb[col].
+__global__ void k_part_reduce(const float* __restrict__ part, int nchunk, int kin_t, int N, int kw,
+                              float* __restrict__ gW, float* __restrict__ gb) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (kw + 1) * N) return;
+  const int row = idx / N, col = idx % N;
   float acc = 0.f;
-  for (int b = b0; b < b1; ++b)
-    acc += X[(size_t)b * ldx + c] * (Y ? Y[(size_t)b * ldy + ycol] : 1.f);
-  part[(size_t)ch * ncol + c] = acc;
-}
-__global__ void k_part_reduce(const float* __restrict__ part, int nchunk, int ncol,
-                              float* __restrict__ out) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncol) return;
-  float acc = 0.f;
-  for (int k = 0; k < nchunk; ++k) acc += part[(size_t)k * ncol + c];
-  out[c] = acc;
+  for (int k = 0; k < nchunk; ++k) acc += part[((size_t)k * kin_t + row) * N + col];
+  if (row < kw) gW[(size_t)row * N + col] = acc;
+  else gb[col] = acc;
 }
 
 // Dense product C[M x N] = A[M x K] * Bt[N x K]^T with row-major A and Bt,
@@ -353,41 +421,67 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const void* __restrict__ Av, in
   }
 }
 
-// Weight gradient partials dW[Kin x N] over batch chunks of 256 rows:
-// part[chunk][m][n] = sum_{b in chunk} X[b][m] dY[b][n]; tiles 64 x 64.
+// Weight-gradient partials over batch chunks: part[chunk][m][n] =
+// sum_{b in chunk} X[b][m] dY[b][n], 64 x 64 tiles, four 16-row waves. X is
+// either a dense activation matrix [rows x Kin] followed by one extra m-tile
+// whose first row is all ones (so row Kin of the result is the bias gradient,
+// the column sum of dY), or, when ug is given, the input layer's conditioning
+// coordinates gathered from the points: X[b][i] = ug[b][A_i] for i < dA,
+// X[b][dA] = 1, zero beyond. Tiles of 32 batch rows are staged transposed in
+// LDS so that each MFMA operand is one 16-byte read.
 template <bool BF16>
 __global__ __launch_bounds__(256) void k_gemm_tn_part(const void* __restrict__ Xv, int ldx,
                                                       const void* __restrict__ Yv, int ldy, int Kin,
                                                       int N, int nrows, int chunk,
-                                                      float* __restrict__ part) {
+                                                      float* __restrict__ part,
+                                                      const float* __restrict__ ug, int D,
+                                                      LayerIdx L) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64, ch = blockIdx.z;
+  const int kin_t = gridDim.x * 64;
   const int b0 = ch * chunk, b1 = min(nrows, b0 + chunk);
+  const bool gather = ug != nullptr;
+  const bool ones = !gather && m0 >= Kin;
   f32x4 acc[4];
   for (int t = 0; t < 4; ++t) acc[t] = (f32x4){0.f, 0.f, 0.f, 0.f};
   if (BF16) {
-    // stage 32 batch rows of X (64 m) and dY (64 n) in LDS, rows padded by 8
-    __shared__ __bf16 sx[32][72];
-    __shared__ __bf16 sy[32][72];
+    __shared__ __attribute__((aligned(16))) __bf16 sx[64][40];
+    __shared__ __attribute__((aligned(16))) __bf16 sy[64][40];
     const __bf16* X = (const __bf16*)Xv;
     const __bf16* Y = (const __bf16*)Yv;
+    const int r = threadIdx.x >> 3, c8 = (threadIdx.x & 7) * 8;
     for (int bb = b0; bb < b1; bb += 32) {
-      for (int e = threadIdx.x; e < 32 * 64; e += 256) {
-        const int r = e >> 6, c = e & 63;
-        const int brow = bb + r;
-        sx[r][c] = (brow < b1) ? X[(size_t)brow * ldx + m0 + c] : (__bf16)0.f;
-        sy[r][c] = (brow < b1) ? Y[(size_t)brow * ldy + n0 + c] : (__bf16)0.f;
+      const int brow = bb + r;
+      const bool valid = brow < b1;
+      bf16x8 yv, xv;
+      if (valid) yv = *(const bf16x8*)(Y + (size_t)brow * ldy + n0 + c8);
+      else
+        for (int e = 0; e < 8; ++e) yv[e] = (__bf16)0.f;
+      if (gather) {
+        for (int e = 0; e < 8; ++e) {
+          const int col = c8 + e;
+          float v = 0.f;
+          if (valid) v = col < L.dA ? ug[(size_t)brow * D + L.A[col]] : (col == L.dA ? 1.f : 0.f);
+          xv[e] = (__bf16)v;
+        }
+      } else if (ones) {
+        for (int e = 0; e < 8; ++e) xv[e] = (__bf16)((valid && c8 + e == 0) ? 1.f : 0.f);
+      } else if (valid) {
+        xv = *(const bf16x8*)(X + (size_t)brow * ldx + m0 + c8);
+      } else {
+        for (int e = 0; e < 8; ++e) xv[e] = (__bf16)0.f;
+      }
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        sx[c8 + e][r] = xv[e];
+        sy[c8 + e][r] = yv[e];
       }
       __syncthreads();
-      bf16x8 a;
       const int kq = 8 * (lane >> 4);
-#pragma unroll
-      for (int e = 0; e < 8; ++e) a[e] = sx[kq + e][wave * 16 + (lane & 15)];
+      const bf16x8 a = *(const bf16x8*)&sx[wave * 16 + (lane & 15)][kq];
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        bf16x8 bv;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) bv[e] = sy[kq + e][16 * t + (lane & 15)];
+        const bf16x8 bv = *(const bf16x8*)&sy[16 * t + (lane & 15)][kq];
         acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc[t], 0, 0, 0);
       }
       __syncthreads();
@@ -397,11 +491,16 @@ __global__ __launch_bounds__(256) void k_gemm_tn_part(const void* __restrict__ X
     const float* Y = (const float*)Yv;
     for (int t = 0; t < 4; ++t) {
       const int col = n0 + 16 * t + (lane & 15);
-      for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wave * 16 + 4 * (lane >> 4) + r;
+      for (int rr = 0; rr < 4; ++rr) {
+        const int row = m0 + wave * 16 + 4 * (lane >> 4) + rr;
         float s = 0.f;
-        for (int b = b0; b < b1; ++b) s += X[(size_t)b * ldx + row] * Y[(size_t)b * ldy + col];
-        acc[t][r] = s;
+        for (int b = b0; b < b1; ++b) {
+          float xb;
+          if (gather) xb = row < L.dA ? ug[(size_t)b * D + L.A[row]] : (row == L.dA ? 1.f : 0.f);
+          else xb = row < Kin ? X[(size_t)b * ldx + row] : (row == Kin ? 1.f : 0.f);
+          s += xb * Y[(size_t)b * ldy + col];
+        }
+        acc[t][rr] = s;
       }
     }
   }
@@ -409,9 +508,9 @@ __global__ __launch_bounds__(256) void k_gemm_tn_part(const void* __restrict__ X
   for (int t = 0; t < 4; ++t) {
     const int col = n0 + 16 * t + (lane & 15);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int row = m0 + wave * 16 + 4 * (lane >> 4) + r;
-      part[((size_t)ch * Kin + row) * N + col] = acc[t][r];
+    for (int rr = 0; rr < 4; ++rr) {
+      const int row = m0 + wave * 16 + 4 * (lane >> 4) + rr;
+      part[((size_t)ch * kin_t + row) * N + col] = acc[t][rr];
     }
   }
 }
@@ -505,6 +604,16 @@ __global__ void k_coef(const double* __restrict__ w, int n, double inv_s2, float
 
 }  // namespace
 
+// one spline per group of G >= K lanes (G = 8, 16, 32)
+#define LAUNCH_SPLINE(KERNEL, K, n, ...)                                                        \
+  do {                                                                                          \
+    const int g_ = (K) <= 8 ? 8 : ((K) <= 16 ? 16 : 32);                                         \
+    const dim3 grd_((unsigned)(((size_t)(n) * g_ + 255) / 256));                                \
+    if (g_ == 8) hipLaunchKernelGGL(KERNEL<8>, grd_, dim3(256), 0, h->stream, __VA_ARGS__);      \
+    else if (g_ == 16) hipLaunchKernelGGL(KERNEL<16>, grd_, dim3(256), 0, h->stream, __VA_ARGS__); \
+    else hipLaunchKernelGGL(KERNEL<32>, grd_, dim3(256), 0, h->stream, __VA_ARGS__);             \
+  } while (0)
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
@@ -594,7 +703,7 @@ static int flow_alloc_batch(gra_handle* h, FlowState* F, int n) {
   bad |= F->W.ensure(sizeof(double) * N) != 0;
   bad |= F->wts.ensure(sizeof(double) * N) != 0;
   const size_t nch = (N + 2047) / 2048;
-  const size_t big = std::max((size_t)std::max(H, ldp) * std::max(H, ldp), (size_t)ldp * 16);
+  const size_t big = (size_t)(std::max(H, ldp) + 64) * std::max(H, ldp);
   bad |= F->part.ensure(sizeof(float) * std::max(nch * big, (size_t)3 * ((N + 255) / 256) * 2)) != 0;
   bad |= F->tmp.ensure(sizeof(double) * 3 * ((N + 255) / 256) + 64) != 0;
   if (bad) return fail(h, GRA_E_NOMEM, "flow batch buffers for %d points", n);
@@ -640,31 +749,41 @@ static void gemm_nt(gra_handle* h, FlowState* F, const void* A, int lda, const v
     hipLaunchKernelGGL(k_gemm_nt<true>, grid, dim3(256), 0, h->stream, A, lda, Bt, ldb, M, N, K, ep);
 }
 
-// dW[Kin x N] = X^T dY accumulated deterministically into grad + off
-static void gemm_tn_grad(gra_handle* h, FlowState* F, const void* X, int ldx, const void* Y, int ldy,
-                         int Kin, int N, int n, float* gout) {
+// dW[Kin x N] = X^T dY and the bias gradient db[N] = sum_b dY[b], accumulated
+// deterministically (per-chunk partials, then a fixed-order sum)
+static void weight_grad(gra_handle* h, FlowState* F, const void* X, int ldx, const void* Y, int ldy,
+                        int Kin, int N, int n, float* gW, float* gb) {
   const int chunk = 2048;
   const int nch = (n + chunk - 1) / chunk;
-  const dim3 grid(Kin / 64, N / 64, nch);
+  const dim3 grid(Kin / 64 + 1, N / 64, nch);
+  LayerIdx none;
+  std::memset(&none, 0, sizeof(none));
   if (F->mlp_fp32)
     hipLaunchKernelGGL(k_gemm_tn_part<false>, grid, dim3(256), 0, h->stream, X, ldx, Y, ldy, Kin, N,
-                       n, chunk, F->part.as<float>());
+                       n, chunk, F->part.as<float>(), nullptr, 0, none);
   else
     hipLaunchKernelGGL(k_gemm_tn_part<true>, grid, dim3(256), 0, h->stream, X, ldx, Y, ldy, Kin, N,
-                       n, chunk, F->part.as<float>());
-  const int tot = Kin * N;
+                       n, chunk, F->part.as<float>(), nullptr, 0, none);
+  const int tot = (Kin + 1) * N;
   hipLaunchKernelGGL(k_part_reduce, dim3((tot + 255) / 256), dim3(256), 0, h->stream,
-                     F->part.as<float>(), nch, tot, gout);
+                     F->part.as<float>(), nch, Kin + 64, N, Kin, gW, gb);
 }
 
-static void colsum_grad(gra_handle* h, FlowState* F, const float* X, int ldx, int ncol, int n,
-                        const float* Y, int ldy, int ycol, float* gout) {
+// Input-layer weight and bias gradients: dW1[i][j] = sum_b u[b][A_i] dh[b][j], db1
+static void input_weight_grad(gra_handle* h, FlowState* F, const float* u, const LayerIdx& idx,
+                              const void* Y, int ldy, int N, int n, float* gW1, float* gb1) {
   const int chunk = 2048;
   const int nch = (n + chunk - 1) / chunk;
-  hipLaunchKernelGGL(k_colsum_part, dim3((ncol + 255) / 256, nch), dim3(256), 0, h->stream, X, ldx,
-                     ncol, n, chunk, Y, ldy, ycol, F->part.as<float>());
-  hipLaunchKernelGGL(k_part_reduce, dim3((ncol + 255) / 256), dim3(256), 0, h->stream,
-                     F->part.as<float>(), nch, ncol, gout);
+  const dim3 grid(1, N / 64, nch);
+  if (F->mlp_fp32)
+    hipLaunchKernelGGL(k_gemm_tn_part<false>, grid, dim3(256), 0, h->stream, nullptr, 0, Y, ldy, 64,
+                       N, n, chunk, F->part.as<float>(), u, F->D, idx);
+  else
+    hipLaunchKernelGGL(k_gemm_tn_part<true>, grid, dim3(256), 0, h->stream, nullptr, 0, Y, ldy, 64, N,
+                       n, chunk, F->part.as<float>(), u, F->D, idx);
+  const int tot = (idx.dA + 1) * N;
+  hipLaunchKernelGGL(k_part_reduce, dim3((tot + 255) / 256), dim3(256), 0, h->stream,
+                     F->part.as<float>(), nch, 64, N, idx.dA, gW1, gb1);
 }
 
 // MLP of layer l on u[A] -> raw spline parameters (prm_of(l)); activations stored
@@ -711,9 +830,8 @@ static int flow_sample_pass(gra_handle* h, FlowState* F, int n, uint64_t seed, u
   for (int l = L - 1; l >= 0; --l) {
     // layer l maps u_l (stored at slot l) to u_{l+1}; inverting reads slot l+1
     mlp_forward(h, F, l, u_of(F, l + 1), n);
-    hipLaunchKernelGGL(k_rqs_inverse, dim3((n + 255) / 256), dim3(256), 0, h->stream, prm_of(F, l),
-                       F->ldp, u_of(F, l + 1), u_of(F, l), F->ld.as<float>(), n, D,
-                       F->layers[l].idx, F->K);
+    LAUNCH_SPLINE(k_rqs_inverse, F->K, n, prm_of(F, l), F->ldp, u_of(F, l + 1), u_of(F, l),
+                  F->ld.as<float>(), n, D, F->layers[l].idx, F->K);
   }
   GRA_HIP_CHECK(h, hipGetLastError());
   return GRA_OK;
@@ -734,9 +852,8 @@ static int flow_backward(gra_handle* h, FlowState* F, int n, float* logq_chk) {
   for (int l = L - 1; l >= 0; --l) {
     const FlowLayerHost& Ly = F->layers[l];
     // g = dL/du_{l+1}; spline backward gives dL/dprm and dL/du_l (B dims; A passes through)
-    hipLaunchKernelGGL(k_rqs_backward, dim3((n + 255) / 256), dim3(256), 0, h->stream, prm_of(F, l),
-                       ldp, u_of(F, l), g, F->coef.as<float>(), g2, F->gprm.as<float>(),
-                       F->gprm16.as<__bf16>(), n, D, Ly.idx, F->K, logq_chk);
+    LAUNCH_SPLINE(k_rqs_backward, F->K, n, prm_of(F, l), ldp, u_of(F, l), g, F->coef.as<float>(),
+                  g2, F->gprm.as<float>(), F->gprm16.as<__bf16>(), n, D, Ly.idx, F->K, logq_chk);
     // MLP backward from the output layer down
     const float* dy32 = F->gprm.as<float>();
     const __bf16* dy16 = F->gprm16.as<__bf16>();
@@ -749,10 +866,9 @@ static int flow_backward(gra_handle* h, FlowState* F, int n, float* logq_chk) {
       float* gW = grad + (out ? Ly.oWo : Ly.oWh[i]);
       float* gb = grad + (out ? Ly.obo : Ly.obh[i]);
       if (F->mlp_fp32)
-        gemm_tn_grad(h, F, act32_of(F, l, i - 1), H, dy32, ldy, H, ncol, n, gW);
+        weight_grad(h, F, act32_of(F, l, i - 1), H, dy32, ldy, H, ncol, n, gW, gb);
       else
-        gemm_tn_grad(h, F, act16_of(F, l, i - 1), H, dy16, ldy, H, ncol, n, gW);
-      colsum_grad(h, F, dy32, ldy, ncol, n, nullptr, 0, 0, gb);
+        weight_grad(h, F, act16_of(F, l, i - 1), H, dy16, ldy, H, ncol, n, gW, gb);
       // dX = dY W^T, masked by ReLU'(act_{i-1})
       Epi ep;
       std::memset(&ep, 0, sizeof(ep));
@@ -774,11 +890,12 @@ static int flow_backward(gra_handle* h, FlowState* F, int n, float* logq_chk) {
       cur ^= 1;
     }
     // input layer: dW1[i][j] = sum_b u[b][A_i] dh[b][j], db1, and dL/du[A] += dh W1^T
-    for (int i = 0; i < Ly.idx.dA; ++i)
-      colsum_grad(h, F, dy32, H, H, n, u_of(F, l), D, Ly.idx.A[i], grad + Ly.oW1 + (size_t)i * H);
-    colsum_grad(h, F, dy32, H, H, n, nullptr, 0, 0, grad + Ly.ob1);
-    hipLaunchKernelGGL(k_mlp_in_dx, dim3((n + 255) / 256), dim3(256), 0, h->stream, dy32,
-                       th + Ly.oW1, H, n, D, Ly.idx, g2);
+    if (F->mlp_fp32)
+      input_weight_grad(h, F, u_of(F, l), Ly.idx, dy32, H, H, n, grad + Ly.oW1, grad + Ly.ob1);
+    else
+      input_weight_grad(h, F, u_of(F, l), Ly.idx, dy16, H, H, n, grad + Ly.oW1, grad + Ly.ob1);
+    hipLaunchKernelGGL(k_mlp_in_dx, dim3((unsigned)(((size_t)n * 64 + 255) / 256)), dim3(256), 0,
+                       h->stream, dy32, th + Ly.oW1, H, n, D, Ly.idx, g2);
     std::swap(g, g2);
   }
   GRA_HIP_CHECK(h, hipGetLastError());
@@ -1095,30 +1212,6 @@ extern "C" int gra_hip_flow_forward(gra_handle* h, const float* z, size_t n, flo
 // Gradient at caller points u (flow output space) and coefficients c. The
 // sampling pass is reconstructed by running the forward direction layer by
 // layer (MLP on the conditioning half, forward spline on the other half).
-namespace {
-__global__ void k_rqs_forward(const float* __restrict__ prm, int ldp, const float* __restrict__ u_in,
-                              float* __restrict__ u_out, int n, int D, LayerIdx L, int K) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= n) return;
-  const float* uin = u_in + (size_t)b * D;
-  float* uo = u_out + (size_t)b * D;
-  for (int i = 0; i < L.dA; ++i) uo[L.A[i]] = uin[L.A[i]];
-  const int np = 3 * K - 1;
-  for (int j = 0; j < L.dB; ++j) {
-    const float* p = prm + (size_t)b * ldp + j * np;
-    float sw[kMaxK], sh[kMaxK];
-    softmax_k(p, K, sw);
-    softmax_k(p + K, K, sh);
-    const float x = fminf(fmaxf(uin[L.B[j]], 0.f), 1.f);
-    SplineEval e;
-    locate(sw, sh, p + 2 * K, K, x, false, e);
-    e.xi = fminf(fmaxf((x - e.X) / e.w, 0.f), 1.f);
-    float z, lj;
-    rqs_value(e, z, lj);
-    uo[L.B[j]] = z;
-  }
-}
-}  // namespace
 
 extern "C" int gra_hip_flow_grad(gra_handle* h, const float* u, const float* c, size_t n,
                                  float* grad, float* logq) {
@@ -1136,8 +1229,8 @@ extern "C" int gra_hip_flow_grad(gra_handle* h, const float* u, const float* c, 
                                   h->stream));
   for (int l = 0; l < F->L; ++l) {
     mlp_forward(h, F, l, u_of(F, l), N);
-    hipLaunchKernelGGL(k_rqs_forward, dim3((N + 255) / 256), dim3(256), 0, h->stream, prm_of(F, l),
-                       F->ldp, u_of(F, l), u_of(F, l + 1), N, D, F->layers[l].idx, F->K);
+    LAUNCH_SPLINE(k_rqs_forward, F->K, N, prm_of(F, l), F->ldp, u_of(F, l), u_of(F, l + 1), N, D,
+                  F->layers[l].idx, F->K);
   }
   r = flow_backward(h, F, N, F->ld.as<float>());
   if (r) return r;
