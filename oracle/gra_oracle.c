@@ -10,24 +10,27 @@
  * x86-64, Makefile:196-204), so that the FP64 restatement reproduces the
  * reference's rounding wherever libm agrees.
  *
- * Pinning status of each part (what fixes it beyond this text) is listed in the
- * test files under tests/ and summarised here:
- *   philox4x32_10          Random123 known-answer vectors (tests/test_oracle_rng.py)
- *   vegas grid/rebin/opt   golden vectors from the reference's own MVEGAS.h compiled
- *                          from source (oracle/_ref, tests/golden/vegas_*.json)
- *   vegas map              exact integration of polynomials through the map
+ * Pinning status of each part (what fixes it beyond this text; the checks are
+ * in tests/test_oracle_core.py and tests/test_oracle_physics.py):
+ *   philox4x32_10          Random123 known-answer vectors
+ *   vegas grid/rebin/opt   golden vectors written by the reference's own MVEGAS.h
+ *                          compiled from source (oracle/_ref ->
+ *                          tests/golden/ref_vectors.json) and a live comparison
+ *   vegas map              uniform grid is the identity; exact integration of
+ *                          polynomials through an adapted map
  *   simpson 3/8, CS rule,  golden vectors from the reference's MMath.h/extmath.hpp
- *   CheckEMC, BESSJ0       (oracle/_ref) + exactness on polynomials + scipy j0
+ *   CheckEMC, BESSJ0       (oracle/_ref) + exactness on cubics + scipy j0
  *   SolvePz                residuals of the conservation equations + mpmath root
- *   kinematics/scalars     4-momentum conservation, on-shell, invariants vs boosts
- *   phase-space weight     FLATAMP=4 volume vs analytic 2->4 phase space (brute force)
- *   Regge ME4              ReggeEtaLinear vs ReggeEta (reference testbench3 5% check),
- *                          |P| = |eta| (s/s0)^alpha, t<->u symmetry, s-scaling
- *   eikonal Fourier-Bessel Hankel pair of a Gaussian (closed form), alpha_P(0)=1+Delta
- *   screening loop         Simpson exactness on polynomial node functions;
- *                          zero-table limit equals the bare amplitude
- *   sigma_tot/sigma_el      ballpark of TOTEM 13 TeV (sanity, not a pin)
- *   tensor-pomeron ME4     see the tensor section (Lorentz invariance, completeness)
+ *   kinematics/scalars     4-momentum conservation, on-shell masses
+ *   phase-space weight     equals the finite-difference Jacobian of the 2->4 map
+ *   Regge ME4              ReggeEtaLinear vs ReggeEta (the reference's testbench3
+ *                          5% check), |P| = |eta| (s/s0)^alpha, t<->u symmetry;
+ *                          TUNE0 amplitude values themselves: parity unpinned
+ *   pomeron trajectory     alpha(0) = 1 + Delta, small-t pion-loop slope (closed form)
+ *   eikonal Fourier-Bessel Hankel pair of a Gaussian (closed form); table
+ *                          convergence; sigma_tot/sigma_el ballpark (sanity only)
+ *   screening loop         node amplitudes equal the table's interpolation; small
+ *                          table limit; survival factor in (0.05, 0.5)
  */
 #include <math.h>
 #include <pthread.h>
