@@ -159,8 +159,6 @@ int validate(const gra_cfg* c) {
                             (int)sizeof(gra_cfg));
   if (c->process != GRA_PROC_REGGE_CON && c->process != GRA_PROC_TENSOR_CON)
     return set_global_error(GRA_E_ARG, "unknown process %d", c->process);
-  if (c->process == GRA_PROC_TENSOR_CON)
-    return set_global_error(GRA_E_ARG, "process PP[CONTENSOR] is not available in this build");
   const int dmin = (c->m_daughter > 0.0) ? 14 : (c->w_central > 0.0 ? 10 : 8);
   if (c->dim < dmin || c->dim > GRA_MAX_DIM)
     return set_global_error(GRA_E_ARG, "dim %d: this process needs %d..%d", c->dim, dmin,
@@ -189,7 +187,7 @@ int validate(const gra_cfg* c) {
   }
   int on = 0;
   for (int k = 0; k < 3; ++k) on += c->regge_on[k] ? 1 : 0;
-  if (c->amp_precision == GRA_AMP_FP32 && on != 1)
+  if (c->process == GRA_PROC_REGGE_CON && c->amp_precision == GRA_AMP_FP32 && on != 1)
     return set_global_error(GRA_E_ARG, "FP32 amplitude needs exactly one active Regge trajectory");
   return GRA_OK;
 }
@@ -403,10 +401,16 @@ int build_eikonal(gra_handle* h) {
 
 }  // namespace
 
+// The reduced FP32-amplitude kernels implement the Regge continuum; the
+// tensor-pomeron process runs the FP64 kernels in either precision mode.
+static bool f32_kernels(const RunParams& P) {
+  return P.amp_mode == GRA_AMP_FP32 && P.process == GRA_PROC_REGGE_CON;
+}
+
 int ensure_point_buffers(gra_handle* h, uint64_t n) {
   const int D = h->P.dim;
-  const size_t recb = (h->P.amp_mode == GRA_AMP_FP32) ? k32::loop_record_bytes()
-                                                      : k64::loop_record_bytes();
+  const size_t recb = f32_kernels(h->P) ? k32::loop_record_bytes(h->P.process)
+                                        : k64::loop_record_bytes(h->P.process);
   if (h->d_W.ensure(sizeof(double) * n) || h->d_amp2.ensure(sizeof(double) * n) ||
       h->d_flags.ensure(n) || h->d_vw.ensure(sizeof(double) * n) ||
       h->d_bins.ensure((size_t)D * n) || h->d_ncand.ensure(16) ||
@@ -438,7 +442,7 @@ int run_integrand(gra_handle* h, const double* d_grid, unsigned int bins, uint64
                   uint32_t iter, uint64_t first, const double* d_x, uint64_t n, double* d_W,
                   double* d_amp2, uint8_t* d_flags, double* d_vw, uint8_t* d_bins) {
   const RunParams& P = h->P;
-  const bool f32 = P.amp_mode == GRA_AMP_FP32;
+  const bool f32 = f32_kernels(P);
   uint32_t* ncand = h->d_ncand.as<uint32_t>();
   GRA_HIP_CHECK(h, hipMemsetAsync(ncand, 0, sizeof(uint32_t), h->stream));
   time_begin(h, TS_TREE);

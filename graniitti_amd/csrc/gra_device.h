@@ -537,6 +537,179 @@ __device__ __forceinline__ float regge_e0(const RunParams& P, const Kin& K) {
   return a1 * __logf((float)K.ss13 * inv_s0) + a2 * __logf((float)K.ss24 * inv_s0);
 }
 
+// ---------------------------------------------------------------------------
+// Tensor-pomeron continuum, MTensorPomeron::ME4 for a vector-meson pair (2xV),
+// src/MTensorPomeron.cc:572-898 (2xV 808-888).
+//
+// The reference contracts rank-4 complex tensors (Pomeron propagators iD_P,
+// vertices iG_PppHE and iG_Pvv = i FM (2 g1 Gamma0 - g2 Gamma2)) component by
+// component. Contracting the proton vertex c P_mu P_nu with the propagator
+// F (g g + g g - g g / 2) leaves V^{ab} = 2 c F (P^a P^b - P^2 g^{ab} / 4), and
+// Gamma0, Gamma2 are traceless in their Pomeron index pair, so every
+// (proton vertex) x (propagator) x (meson vertex) block is the 4 x 4 matrix
+//   X_{mu nu} = i FM 2 c F [Cg g_{mu nu} + C21 k2_mu k1_nu + CPP P_mu P_nu
+//                           + CP1 P_mu k1_nu + C2P k2_mu P_nu]
+// with (k1, k2) the meson-vertex momenta, and the helicity amplitudes only need
+// these matrices contracted with one polarisation vector each: a handful of
+// Minkowski products instead of ~2e4 complex multiply-adds per point.
+// ---------------------------------------------------------------------------
+struct cv4 {
+  cxd t, x, y, z;
+};
+__device__ __forceinline__ double mdot(dv4 a, dv4 b) {
+  return a.e * b.e - a.x * b.x - a.y * b.y - a.z * b.z;
+}
+__device__ __forceinline__ cxd mdotc(dv4 a, const cv4& e) {
+  return cd(a.e * e.t.re - a.x * e.x.re - a.y * e.y.re - a.z * e.z.re,
+            a.e * e.t.im - a.x * e.x.im - a.y * e.y.im - a.z * e.z.im);
+}
+__device__ __forceinline__ cxd mdotcc(const cv4& a, const cv4& b) {
+  cxd s = cmul(a.t, b.t);
+  const cxd sx = cmul(a.x, b.x), sy = cmul(a.y, b.y), sz = cmul(a.z, b.z);
+  return cd(s.re - sx.re - sy.re - sz.re, s.im - sx.im - sy.im - sz.im);
+}
+
+// form::F1 with the Kelly G_E, G_M fits (src/MForm.cc:542-548, 611-645)
+__device__ __forceinline__ double kelly_f1(double Q2, double mp) {
+  Q2 = fabs(Q2);
+  const double tau = Q2 / ((2 * mp) * (2 * mp));
+  const double GE = (1.0 - 0.24 * tau) / (1.0 + tau * (10.98 + tau * (12.82 + tau * 21.97)));
+  const double GM = 2.792847337 * (1.0 + 0.12 * tau) /
+                    (1.0 + tau * (10.97 + tau * (18.86 + tau * 6.55)));
+  return (GE + tau * GM) / (tau + 1.0);
+}
+
+struct TBlk {
+  cxd pre;
+  double Cg, C21, CPP, CP1, C2P;
+  dv4 k1, k2, P;
+};
+// iG_PppHE(prime, pa) . iD_P(s, t) . iG_Pvv(k1, -k2) reduced to the dyadic form;
+// c = -3 i gPNN F1(t), F = (-i s a')^{alpha(t) - 1} / (4 s), FM = 1 / (1 - (k1 + k2)^2 / 0.5)
+__device__ __forceinline__ TBlk tensor_block(const RunParams& P, dv4 Pv, double F1, double s,
+                                             double t, dv4 k1, dv4 k2) {
+  TBlk b;
+  const double g1 = P.tp_gPvv0, g2 = P.tp_gPvv1;
+  const double k1k2 = mdot(k1, k2), k1P = mdot(k1, Pv), k2P = mdot(k2, Pv), P2 = mdot(Pv, Pv);
+  const double S0 = 2.0 * k1P * k2P - 0.5 * k1k2 * P2;
+  b.Cg = 2.0 * g1 * S0 * k1k2 - 2.0 * g2 * k1P * k2P + g2 * P2 * k1k2;
+  b.C21 = -2.0 * g1 * S0 - g2 * P2;
+  b.CPP = -2.0 * g2 * k1k2;
+  b.CP1 = 2.0 * g2 * k2P;
+  b.C2P = 2.0 * g2 * k1P;
+  const dv4 q = v4add(k1, k2);
+  const double FM = 1.0 / (1.0 - mdot(q, q) / 0.5);
+  // 2 c F with c = -3 i gPNN F1: (-i s a')^{a-1} = exp((a - 1)(log(s a') - i pi / 2))
+  const double am1 = P.tp_delta_P + P.tp_ap_P * t;
+  const double mag = exp(am1 * log(s * P.tp_ap_P)) / (4.0 * s);
+  const double ph = -0.5 * GRA_PI * am1;
+  double sn, cs;
+  sincos(ph, &sn, &cs);
+  // i FM 2 (-3 i gPNN F1) F = 6 gPNN F1 FM F
+  const double r = 6.0 * P.tp_gPNN * F1 * FM * mag;
+  b.pre = cd(r * cs, r * sn);
+  b.k1 = k1;
+  b.k2 = k2;
+  b.P = Pv;
+  return b;
+}
+__device__ __forceinline__ cv4 cv4_comb(const TBlk& b, const cv4& e, cxd ak1, cxd ak2, cxd aP) {
+  cv4 r;
+  r.t = cadd(cadd(cscale(e.t, b.Cg), cscale(ak1, b.k1.e)), cadd(cscale(ak2, b.k2.e), cscale(aP, b.P.e)));
+  r.x = cadd(cadd(cscale(e.x, b.Cg), cscale(ak1, b.k1.x)), cadd(cscale(ak2, b.k2.x), cscale(aP, b.P.x)));
+  r.y = cadd(cadd(cscale(e.y, b.Cg), cscale(ak1, b.k1.y)), cadd(cscale(ak2, b.k2.y), cscale(aP, b.P.y)));
+  r.z = cadd(cadd(cscale(e.z, b.Cg), cscale(ak1, b.k1.z)), cadd(cscale(ak2, b.k2.z), cscale(aP, b.P.z)));
+  return r;
+}
+// sum_nu X_{mu nu} e^nu (without the block prefactor), as a contravariant vector
+__device__ __forceinline__ cv4 tensor_right(const TBlk& b, const cv4& e) {
+  const cxd k1e = mdotc(b.k1, e), Pe = mdotc(b.P, e);
+  const cxd ak2 = cadd(cscale(k1e, b.C21), cscale(Pe, b.C2P));
+  const cxd aP = cadd(cscale(Pe, b.CPP), cscale(k1e, b.CP1));
+  return cv4_comb(b, e, cd(0, 0), ak2, aP);
+}
+// sum_mu e^mu X_{mu nu}
+__device__ __forceinline__ cv4 tensor_left(const TBlk& b, const cv4& e) {
+  const cxd k2e = mdotc(b.k2, e), Pe = mdotc(b.P, e);
+  const cxd ak1 = cadd(cscale(k2e, b.C21), cscale(Pe, b.CP1));
+  const cxd aP = cadd(cscale(Pe, b.CPP), cscale(k2e, b.C2P));
+  return cv4_comb(b, e, ak1, cd(0, 0), aP);
+}
+// Conjugated massive spin-1 polarisation vectors for helicities -1, 0, +1
+// (MDirac::MassiveSpin1States "conj", index up; src/MDirac.cc:393-456, 661-679),
+// with the angles' cosines and sines formed from the momentum components
+__device__ __forceinline__ void spin1_conj(dv4 k, cv4 e[3]) {
+  const double pt = sqrt(k.x * k.x + k.y * k.y), pm = sqrt(pt * pt + k.z * k.z);
+  const double ct = pm > 0.0 ? k.z / pm : 1.0, st = pm > 0.0 ? pt / pm : 0.0;
+  const double cp = pt > 0.0 ? k.x / pt : 1.0, sp = pt > 0.0 ? k.y / pt : 0.0;
+  const double N = 0.70710678118654752440;
+#pragma unroll
+  for (int h = 0; h < 3; h += 2) {
+    const double m = (double)(h - 1);
+    e[h].t = cd(0.0, 0.0);
+    e[h].x = cd(-m * ct * cp * N, -sp * N);
+    e[h].y = cd(-m * ct * sp * N, cp * N);
+    e[h].z = cd(m * st * N, 0.0);
+  }
+  const double M = sqrt(mdot(k, k));
+  e[1].t = cd(pm / M, 0.0);
+  e[1].x = cd(k.e * st * cp / M, 0.0);
+  e[1].y = cd(k.e * st * sp / M, 0.0);
+  e[1].z = cd(k.e * ct / M, 0.0);
+}
+// Vector-meson propagator factor of iD_V (src/MTensorPomeron.cc:2353-2369) times
+// the two off-shell form factors: d = -R ff^2 / (m2 - M^2), R = (e^{i phi} s34 / s_th)^{alpha - 1}
+__device__ __forceinline__ cxd tensor_vprop(const RunParams& P, double m2, double ls, double phi) {
+  const double M2 = P.m_prop2;
+  const double am1 = 0.5 + 0.9 * m2 - 1.0;
+  const double mag = exp(am1 * ls);
+  double sn, cs;
+  sincos(am1 * phi, &sn, &cs);
+  const double ff = meson_ff(P, m2);
+  const double r = -mag * ff * ff / (m2 - M2);
+  return cd(r * cs, r * sn);
+}
+
+// The 9 helicity amplitudes h[3 h3 + h4] (h3, h4 = -1, 0, +1) at kinematics K.
+// The meson momenta are decaytree[].p4 (tree level even inside the screening
+// loop, as in the reference), the protons and s_ij those of K.
+__device__ __forceinline__ void tensor_me4_fp64(const RunParams& P, const Kin& K, cxd h[9]) {
+  const dv4 pa = P.pbeam1, pb = P.pbeam2;
+  const dv4 p1 = K.pf[1], p2 = K.pf[2], p3 = K.dt[0], p4 = K.dt[1];
+  const dv4 P1 = v4add(p1, pa), P2 = v4add(p2, pb);
+  const dv4 pt = v4sub(v4sub(pa, p1), p3);
+  const dv4 pu = v4add(v4sub(p4, pa), p1);
+  const double F1a = kelly_f1(K.t1, P.tp_mp_ff), F1b = kelly_f1(K.t2, P.tp_mp_ff);
+  const TBlk tA = tensor_block(P, P1, F1a, K.ss13, K.t1, pt, p3);
+  const TBlk tB = tensor_block(P, P2, F1b, K.ss24, K.t2, p4, v4scale(pt, -1.0));
+  const TBlk uA = tensor_block(P, P1, F1a, K.ss14, K.t1, p4, v4scale(pu, -1.0));
+  const TBlk uB = tensor_block(P, P2, F1b, K.ss23, K.t2, pu, p3);
+  const double s34 = mdot(K.pf[0], K.pf[0]);
+  const double sth = 4.0 * P.m_prop2;
+  const double phi = 0.5 * GRA_PI * exp((sth - s34) / sth) - 0.5 * GRA_PI;
+  const double ls = log(s34 / sth);
+  const cxd dt = cmul(tensor_vprop(P, mdot(pt, pt), ls, phi), cmul(tA.pre, tB.pre));
+  const cxd du = cmul(tensor_vprop(P, mdot(pu, pu), ls, phi), cmul(uA.pre, uB.pre));
+  cv4 e3[3], e4[3];
+  spin1_conj(p3, e3);
+  spin1_conj(p4, e4);
+  cv4 a3[3], b3[3], a4[3], b4[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    a3[i] = tensor_right(tA, e3[i]);  // t-channel upper block . eps3
+    b3[i] = tensor_right(uB, e3[i]);  // u-channel lower block . eps3
+    a4[i] = tensor_left(tB, e4[i]);   // eps4 . t-channel lower block
+    b4[i] = tensor_left(uA, e4[i]);   // eps4 . u-channel upper block
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const cxd m = cadd(cmul(dt, mdotcc(a3[i], a4[j])), cmul(du, mdotcc(b4[j], b3[i])));
+      h[3 * i + j] = cd(m.im, -m.re);  // -i (M_t + M_u)
+    }
+}
+
 // |z|^2 as gra::math::abs2 (hypot then square)
 __device__ __forceinline__ double abs2d(cxd z) {
   const double h = hypot(z.re, z.im);

@@ -11,7 +11,7 @@ namespace gra_hip {
 
 #define GRA_DECLARE_INTEGRAND_LAUNCHERS(NS)                                                        \
   namespace NS {                                                                                   \
-  size_t loop_record_bytes();                                                                      \
+  size_t loop_record_bytes(int process);                                                           \
   hipError_t launch_tree(const RunParams& P, const double* grid, unsigned int bins, uint64_t seed, \
                          uint32_t iter, uint64_t first, const double* xin, uint64_t n, double* W,  \
                          double* amp2, uint8_t* flags, double* vw, uint8_t* binsout, void* rec,    \

@@ -1264,13 +1264,350 @@ ORA_EXPORT void ora_vegas_combine(double fsum, double f2sum, double calls, int i
 }
 
 /* ------------------------------------------------------------------------- */
-/* Tensor pomeron (placeholder until the tensor section is restated)         */
+/* Tensor pomeron, MTensorPomeron::ME4 for a vector-meson pair ("2xV"),       */
+/* src/MTensorPomeron.cc:572-898 (2xV block 808-888). Lorentz tensors are     */
+/* stored as plain arrays with the component order (t, x, y, z) and          */
+/* contracted with plain index sums, as the reference's FTensor expressions   */
+/* do; the metric g = diag(1, -1, -1, -1) appears explicitly where the        */
+/* reference writes g[u][v], and `%` (covariant component) is v_mu = g_mumu v^mu. */
 /* ------------------------------------------------------------------------- */
+/* The tensor network is accumulated in long double: evaluated in double, as the
+ * reference does, its sums of 256 large terms lose up to ~1e-8 relative to
+ * cancellation at some points, which would dominate any FP64 comparison. */
+typedef struct {
+  long double re, im;
+} lcx;
+static inline lcx lc_make(long double re, long double im) {
+  lcx z;
+  z.re = re;
+  z.im = im;
+  return z;
+}
+static inline lcx lc_add(lcx a, lcx b) { return lc_make(a.re + b.re, a.im + b.im); }
+static inline lcx lc_mul(lcx a, lcx b) {
+  return lc_make(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+static inline lcx lc_scale(lcx a, long double d) { return lc_make(a.re * d, a.im * d); }
+typedef lcx t2_t[4][4];
+typedef lcx t4_t[4][4][4][4];
+
+static const long double GM[4][4] = {{1, 0, 0, 0}, {0, -1, 0, 0}, {0, 0, -1, 0}, {0, 0, 0, -1}};
+
+static inline long double v4_c(v4 a, int mu) { return mu == 0 ? a.e : (mu == 1 ? a.x : (mu == 2 ? a.y : a.z)); }
+/* M4Vec::operator% (lowered component), include/Graniitti/M4Vec.h:216-223 */
+static inline long double v4_lo(v4 a, int mu) { return GM[mu][mu] * v4_c(a, mu); }
+
+/* std::pow(std::complex<double>, double) of libstdc++: real pow for a positive
+ * real base, else polar(exp(y log|z|), y arg z) */
+static lcx lc_pow_real(lcx z, long double y) {
+  if (z.im == 0.0L && z.re > 0.0L) return lc_make(powl(z.re, y), 0.0L);
+  const long double r = expl(y * logl(hypotl(z.re, z.im)));
+  const long double th = y * atan2l(z.im, z.re);
+  return lc_make(r * cosl(th), r * sinl(th));
+}
+static inline long double ld_dot(v4 a, v4 b) {
+  return (long double)a.e * b.e -
+         ((long double)a.x * b.x + (long double)a.y * b.y + (long double)a.z * b.z);
+}
+static inline long double ld_m2(v4 a) { return ld_dot(a, a); }
+
+/* form::F1 with the Kelly G_E, G_M parametrisation (PARAM_STRUCTURE EM = KELLY),
+ * src/MForm.cc:542-548, 611-645; mu_p = 2.792847337 (src/MForm.cc:521) */
+static long double kelly_f1(long double Q2, long double mp) {
+  Q2 = fabsl(Q2);
+  const long double tau = Q2 / ((2 * mp) * (2 * mp));
+  const long double mu = 2.792847337L;
+  long double num = 0.0L, den = 1.0L;
+  num += 1.0;
+  num += -0.24L * tau;
+  den += 10.98L * tau;
+  den += 12.82L * (tau * tau);
+  den += 21.97L * (tau * tau * tau);
+  const long double GE = num / den;
+  num = 0.0;
+  den = 1.0;
+  num += 1.0;
+  num += 0.12L * tau;
+  den += 10.97L * tau;
+  den += 18.86L * (tau * tau);
+  den += 6.55L * (tau * tau * tau);
+  const long double GM_ = mu * num / den;
+  return 1.0 / (tau + 1) * GE + tau / (tau + 1) * GM_;
+}
+
+/* iD_P (src/MTensorPomeron.cc:2288-2298): Pomeron propagator */
+static void tp_iD_P(const gra_cfg* c, double s, double t, t4_t T) {
+  const long double alpha = 1.0L + c->tp_delta_P + (long double)c->tp_ap_P * t;
+  const lcx base = lc_make(-0.0L, -1.0L * s * c->tp_ap_P); /* -i s ap */
+  const lcx F = lc_scale(lc_pow_real(base, alpha - 1.0L), 1.0L / (4.0L * s));
+  for (int u = 0; u < 4; ++u)
+    for (int v = 0; v < 4; ++v)
+      for (int k = 0; k < 4; ++k)
+        for (int l = 0; l < 4; ++l)
+          T[u][v][k][l] =
+              lc_scale(F, GM[u][k] * GM[v][l] + GM[u][l] * GM[v][k] - 0.5L * GM[u][v] * GM[k][l]);
+}
+
+/* iG_PppHE (src/MTensorPomeron.cc:1388-1399): high-energy proton-Pomeron-proton vertex */
+static void tp_iG_PppHE(const gra_cfg* c, v4 prime, v4 p, t2_t T) {
+  const v4 ps = v4_add(prime, p);
+  const long double F = -3.0L * c->tp_gPNN * kelly_f1(ld_m2(v4_sub(prime, p)), c->tp_mp_ff);
+  for (int u = 0; u < 4; ++u)
+    for (int v = 0; v < 4; ++v) T[u][v] = lc_make(0.0L, F * (v4_lo(ps, u) * v4_lo(ps, v)));
+}
+
+/* Gamma0 and Gamma2 (src/MTensorPomeron.cc:2500-2529) */
+static long double tp_gamma0(v4 k1, v4 k2, int u, int v, int k, int l) {
+  const long double k1k2 = ld_dot(k1, k2);
+  return (k1k2 * GM[u][v] - v4_lo(k2, u) * v4_lo(k1, v)) *
+         (v4_lo(k1, k) * v4_lo(k2, l) + v4_lo(k2, k) * v4_lo(k1, l) - 0.5L * k1k2 * GM[k][l]);
+}
+static long double tp_gamma2(v4 k1, v4 k2, int u, int v, int k, int l) {
+  const long double k1k2 = ld_dot(k1, k2);
+  return k1k2 * (GM[u][k] * GM[v][l] + GM[u][l] * GM[v][k]) +
+         GM[u][v] * (v4_lo(k1, k) * v4_lo(k2, l) + v4_lo(k2, k) * v4_lo(k1, l)) -
+         v4_lo(k1, v) * v4_lo(k2, l) * GM[u][k] - v4_lo(k1, v) * v4_lo(k2, k) * GM[u][l] -
+         v4_lo(k2, u) * v4_lo(k1, l) * GM[v][k] - v4_lo(k2, u) * v4_lo(k1, k) * GM[v][l] -
+         (k1k2 * GM[u][v] - v4_lo(k2, u) * v4_lo(k1, v)) * GM[k][l];
+}
+
+/* iG_Pvv (src/MTensorPomeron.cc:2224-2237) with FM (2486), LAMBDA^2 = 0.5 */
+static void tp_iG_Pvv(v4 prime, v4 p, long double g1, long double g2, t4_t T) {
+  const v4 mp = v4_scale(p, -1.0);
+  const long double FM = 1.0L / (1.0L - ld_m2(v4_sub(prime, p)) / 0.5L);
+  for (int u = 0; u < 4; ++u)
+    for (int v = 0; v < 4; ++v)
+      for (int k = 0; k < 4; ++k)
+        for (int l = 0; l < 4; ++l)
+          T[u][v][k][l] = lc_make(0.0L, FM * (2.0L * g1 * tp_gamma0(prime, mp, u, v, k, l) -
+                                               g2 * tp_gamma2(prime, mp, u, v, k, l)));
+}
+
+/* iD_V (src/MTensorPomeron.cc:2353-2369): vector propagator with naive Reggeization */
+static void tp_iD_V(v4 p, long double M0, long double s34, t2_t T) {
+  const long double PIL = 3.141592653589793238462643383279502884L;
+  const long double m2 = ld_m2(p);
+  const long double alpha = 0.5L + 0.9L * m2;
+  const long double s_th = 4 * (M0 * M0);
+  const long double phi = PIL / 2.0L * expl((s_th - s34) / s_th) - PIL / 2.0L;
+  const lcx base = lc_scale(lc_make(cosl(phi), sinl(phi)), s34 / s_th);
+  const lcx R = lc_pow_real(base, alpha - 1.0L);
+  const long double den = m2 - M0 * M0;
+  for (int u = 0; u < 4; ++u)
+    for (int v = 0; v < 4; ++v) T[u][v] = lc_scale(R, -GM[u][v] / den);
+}
+
+/* MDirac::EpsMassiveSpin1 / EpsSpin1 (src/MDirac.cc:393-456), conjugated, index up,
+ * helicities -1, 0, +1 (MassiveSpin1States, src/MDirac.cc:661-679) */
+static void tp_eps_conj(v4 k, lcx e[3][4]) {
+  const long double pt = sqrtl((long double)k.x * k.x + (long double)k.y * k.y);
+  const long double theta = (k.x == 0.0 && k.y == 0.0 && k.z == 0.0) ? 0.0L : atan2l(pt, k.z);
+  const long double phi = (k.x == 0.0 && k.y == 0.0) ? 0.0L : atan2l(k.y, k.x);
+  const long double N = 1.0L / sqrtl(2.0L);
+  for (int h = 0; h < 3; ++h) {
+    const int m = h - 1;
+    if (m != 0) {
+      const long double neg = -(long double)m, pos = (long double)m;
+      e[h][0] = lc_make(0.0L, 0.0L);
+      e[h][1] = lc_make((neg * cosl(theta) * cosl(phi)) * N, -(sinl(phi) * N));
+      e[h][2] = lc_make((neg * cosl(theta) * sinl(phi)) * N, (cosl(phi) * N));
+      e[h][3] = lc_make(pos * sinl(theta) * N, 0.0L);
+    } else {
+      const long double E = k.e, M = sqrtl(ld_m2(k));
+      const long double p3 = sqrtl((long double)k.x * k.x + (long double)k.y * k.y +
+                                   (long double)k.z * k.z);
+      e[h][0] = lc_make(p3 / M, 0.0L);
+      e[h][1] = lc_make(E * sinl(theta) * cosl(phi) / M, 0.0L);
+      e[h][2] = lc_make(E * sinl(theta) * sinl(phi) / M, 0.0L);
+      e[h][3] = lc_make(E * cosl(theta) / M, 0.0L);
+    }
+  }
+}
+
+/* sum_{a,b} G(a,b) D(a,b,k,l) -> X(k,l) (first pair contracted) */
+static void tp_vd_first(t2_t G, t4_t Dp, t2_t X) {
+  for (int k = 0; k < 4; ++k)
+    for (int l = 0; l < 4; ++l) {
+      lcx s = lc_make(0, 0);
+      for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b) s = lc_add(s, lc_mul(G[a][b], Dp[a][b][k][l]));
+      X[k][l] = s;
+    }
+}
+/* sum_{a,b} G(a,b) D(k,l,a,b) -> X(k,l) (second pair contracted) */
+static void tp_vd_second(t2_t G, t4_t Dp, t2_t X) {
+  for (int k = 0; k < 4; ++k)
+    for (int l = 0; l < 4; ++l) {
+      lcx s = lc_make(0, 0);
+      for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b) s = lc_add(s, lc_mul(G[a][b], Dp[k][l][a][b]));
+      X[k][l] = s;
+    }
+}
+/* sum_{a,b} X(a,b) V(r,q,a,b) -> Y(r,q) */
+static void tp_xv(t2_t X, t4_t V, t2_t Y) {
+  for (int r = 0; r < 4; ++r)
+    for (int q = 0; q < 4; ++q) {
+      lcx s = lc_make(0, 0);
+      for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b) s = lc_add(s, lc_mul(X[a][b], V[r][q][a][b]));
+      Y[r][q] = s;
+    }
+}
+
+/* Helicity amplitudes of pp -> p V V p (ME4, 2xV), written to hamp in the
+ * reference's order: 4 proton-helicity replicas (ha = h1, hb = h2, identical in
+ * the high-energy vertex) x 9 (h3, h4) pairs. Returns 36. */
+/* M(r3, r4) = -i (M_t + M_u)(r3, r4) of the 2xV block (indices down) */
+static void tensor_mtensor(const ora_run* R, v4 pa, v4 pb, const kin_t* K, t2_t Mout) {
+  const gra_cfg* c = &R->c;
+  const v4 p1 = K->pf[1], p2 = K->pf[2];
+  const v4 p3 = K->dt[0], p4 = K->dt[1];
+  const long double M_ = c->m_prop;
+  const long double g1 = c->tp_gPvv[0], g2 = c->tp_gPvv[1];
+  const v4 pt = v4_sub(v4_sub(pa, p1), p3);
+  const v4 pu = v4_add(v4_sub(p4, pa), p1);
+  t4_t W[8]; /* D13, D24, D14, D23, GtA, GtB, GuA, GuB */
+  tp_iD_P(c, K->ss[1][3], K->t1, W[0]);
+  tp_iD_P(c, K->ss[2][4], K->t2, W[1]);
+  tp_iD_P(c, K->ss[1][4], K->t1, W[2]);
+  tp_iD_P(c, K->ss[2][3], K->t2, W[3]);
+  t2_t G1a, G2b;
+  tp_iG_PppHE(c, p1, pa, G1a);
+  tp_iG_PppHE(c, p2, pb, G2b);
+  tp_iG_Pvv(pt, v4_scale(p3, -1.0), g1, g2, W[4]);
+  tp_iG_Pvv(p4, pt, g1, g2, W[5]);
+  tp_iG_Pvv(p4, pu, g1, g2, W[6]);
+  tp_iG_Pvv(pu, v4_scale(p3, -1.0), g1, g2, W[7]);
+  const long double s34 = ld_m2(K->pf[0]);
+  t2_t DVt, DVu;
+  tp_iD_V(pt, M_, s34, DVt);
+  tp_iD_V(pu, M_, s34, DVu);
+  /* PARAM_REGGE::Meson_FF, pow form (offshellFF = 2) or as configured */
+  const long double M2_ = M_ * M_;
+  long double fft, ffu;
+  if (c->offshellFF == 2) {
+    fft = 1.0L / (1.0L - (ld_m2(pt) - M2_) / c->b_POW);
+    ffu = 1.0L / (1.0L - (ld_m2(pu) - M2_) / c->b_POW);
+  } else {
+    fft = meson_ff(c, v4_m2(pt), (double)M2_);
+    ffu = meson_ff(c, v4_m2(pu), (double)M2_);
+  }
+  const long double fft2 = fft * fft, ffu2 = ffu * ffu;
+  t2_t X, A, B, Mt, Mu, T;
+  /* t-channel: A(r1, r3) = G1a(m1,n1) D13(m1,n1,a1,b1) GtA(r1,r3,a1,b1) */
+  tp_vd_first(G1a, W[0], X);
+  tp_xv(X, W[4], A);
+  /* B(r4, r2) = G2b(m2,n2) D24(a2,b2,m2,n2) GtB(r4,r2,a2,b2) */
+  tp_vd_second(G2b, W[1], X);
+  tp_xv(X, W[5], B);
+  /* Mt(r3, r4) = A(r1, r3) DVt(r1, r2) B(r4, r2) ff^2 */
+  for (int r3 = 0; r3 < 4; ++r3)
+    for (int r2 = 0; r2 < 4; ++r2) {
+      lcx s = lc_make(0, 0);
+      for (int r1 = 0; r1 < 4; ++r1) s = lc_add(s, lc_mul(A[r1][r3], DVt[r1][r2]));
+      T[r3][r2] = s;
+    }
+  for (int r3 = 0; r3 < 4; ++r3)
+    for (int r4 = 0; r4 < 4; ++r4) {
+      lcx s = lc_make(0, 0);
+      for (int r2 = 0; r2 < 4; ++r2) s = lc_add(s, lc_mul(T[r3][r2], B[r4][r2]));
+      Mt[r3][r4] = lc_scale(s, fft2);
+    }
+  /* u-channel: A(r4, r1) = G1a D14 GuA(r4,r1,..); B(r2, r3) = G2b D23 GuB(r2,r3,..) */
+  tp_vd_first(G1a, W[2], X);
+  tp_xv(X, W[6], A);
+  tp_vd_second(G2b, W[3], X);
+  tp_xv(X, W[7], B);
+  /* Mu(r3, r4) = A(r4, r1) DVu(r1, r2) B(r2, r3) ff^2 */
+  for (int r4 = 0; r4 < 4; ++r4)
+    for (int r2 = 0; r2 < 4; ++r2) {
+      lcx s = lc_make(0, 0);
+      for (int r1 = 0; r1 < 4; ++r1) s = lc_add(s, lc_mul(A[r4][r1], DVu[r1][r2]));
+      T[r4][r2] = s;
+    }
+  for (int r4 = 0; r4 < 4; ++r4)
+    for (int r3 = 0; r3 < 4; ++r3) {
+      lcx s = lc_make(0, 0);
+      for (int r2 = 0; r2 < 4; ++r2) s = lc_add(s, lc_mul(T[r4][r2], B[r2][r3]));
+      Mu[r3][r4] = lc_scale(s, ffu2);
+    }
+  for (int r3 = 0; r3 < 4; ++r3)
+    for (int r4 = 0; r4 < 4; ++r4) {
+      const lcx m = lc_add(Mt[r3][r4], Mu[r3][r4]);
+      Mout[r3][r4] = lc_make(m.im, -m.re); /* (-i) m */
+    }
+}
+
+static int tensor_me4_2xv(const ora_run* R, v4 pa, v4 pb, const kin_t* K, cx* hamp) {
+  t2_t M;
+  tensor_mtensor(R, pa, pb, K, M);
+  /* helicity amplitudes eps3*(r3) eps4*(r4) M(r3, r4) (MassiveSpin1PolSum) */
+  lcx e3[3][4], e4[3][4];
+  tp_eps_conj(K->dt[0], e3);
+  tp_eps_conj(K->dt[1], e4);
+  lcx h9[9];
+  for (int h3 = 0; h3 < 3; ++h3)
+    for (int h4 = 0; h4 < 3; ++h4) {
+      lcx s = lc_make(0, 0);
+      for (int r3 = 0; r3 < 4; ++r3)
+        for (int r4 = 0; r4 < 4; ++r4)
+          s = lc_add(s, lc_mul(lc_mul(e3[h3][r3], e4[h4][r4]), M[r3][r4]));
+      h9[3 * h3 + h4] = s;
+    }
+  for (int rep = 0; rep < 4; ++rep)
+    for (int q = 0; q < 9; ++q) hamp[9 * rep + q] = cx_make((double)h9[q].re, (double)h9[q].im);
+  return 36;
+}
+
 static int tensor_me4(const ora_run* R, const kin_t* K, cx* hamp) {
-  (void)R;
-  (void)K;
-  hamp[0] = cx_make(0, 0);
-  return 1;
+  return tensor_me4_2xv(R, R->pbeam1, R->pbeam2, K, hamp);
+}
+
+static void tensor_kin(const double* p, v4 q[6], kin_t* K) {
+  for (int i = 0; i < 6; ++i) q[i] = v4_make(p[4 * i + 1], p[4 * i + 2], p[4 * i + 3], p[4 * i]);
+  memset(K, 0, sizeof(*K));
+  K->pf[1] = q[2];
+  K->pf[2] = q[3];
+  K->pf[3] = q[4];
+  K->pf[4] = q[5];
+  K->pf[0] = v4_add(q[4], q[5]);
+  K->dt[0] = q[4];
+  K->dt[1] = q[5];
+  for (int i = 0; i <= 4; ++i)
+    for (int j = 0; j <= 4; ++j) K->ss[i][j] = v4_m2(v4_add(K->pf[i], K->pf[j]));
+  K->t1 = v4_m2(v4_sub(q[0], q[2]));
+  K->t2 = v4_m2(v4_sub(q[1], q[3]));
+}
+
+/* The amplitude tensor M(r3, r4) (indices down, (t, x, y, z) order, 4 x 4
+ * complex as re, im pairs) at caller momenta. Test entry point. */
+ORA_EXPORT void ora_tensor_mtensor(ora_run* R, const double* p, double* out) {
+  v4 q[6];
+  kin_t K;
+  tensor_kin(p, q, &K);
+  t2_t M;
+  tensor_mtensor(R, q[0], q[1], &K, M);
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) {
+      out[2 * (4 * a + b)] = (double)M[a][b].re;
+      out[2 * (4 * a + b) + 1] = (double)M[a][b].im;
+    }
+}
+
+/* Helicity amplitudes (9, one replica) at caller momenta pa, pb, p1, p2, p3, p4
+ * (6 x (E, x, y, z)); invariants are formed from them. Test entry point. */
+ORA_EXPORT int ora_tensor_amplitudes(ora_run* R, const double* p, double* hamp_reim) {
+  v4 q[6];
+  kin_t K;
+  tensor_kin(p, q, &K);
+  cx h[36];
+  tensor_me4_2xv(R, q[0], q[1], &K, h);
+  for (int k = 0; k < 9; ++k) {
+    hamp_reim[2 * k] = h[k].re;
+    hamp_reim[2 * k + 1] = h[k].im;
+  }
+  return 9;
 }
 
 /* Phase-space part of the weight, BNPhaseSpaceWeight x BNIntegralVolume

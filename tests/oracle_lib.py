@@ -46,6 +46,8 @@ def _load_oracle():
         "ora_eval_points": (None, [VP, P(D), C.c_int64, P(D), P(D), P(C.c_uint8)]),
         "ora_kinematics": (C.c_int, [VP, P(D), P(D), P(D), P(D)]),
         "ora_bare_amplitudes": (C.c_int, [VP, P(D), P(D)]),
+        "ora_tensor_amplitudes": (C.c_int, [VP, P(D), P(D)]),
+        "ora_tensor_mtensor": (None, [VP, P(D), P(D)]),
         "ora_vegas_iterate": (None, [VP, P(abi.GraGrid), U64, U64, U64, C.c_uint32, C.c_int,
                                      C.c_int, P(abi.GraIterStats)]),
         "ora_vegas_combine": (None, [D, D, D, C.c_int, P(D), P(D)]),
@@ -143,6 +145,31 @@ class OracleRun:
         sc = np.zeros(8)
         ok = self.L.ora_kinematics(self.h, abi.dptr(x), abi.dptr(pf), abi.dptr(legs), abi.dptr(sc))
         return ok, pf, legs, sc
+
+    def bare_amplitudes(self, x):
+        """Helicity amplitudes of the bare process at hypercube point x (None if the
+        kinematics fail)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        h = np.zeros(2 * 64)
+        nh = self.L.ora_bare_amplitudes(self.h, abi.dptr(x), abi.dptr(h))
+        if nh == 0:
+            return None
+        return h[0:2 * nh:2] + 1j * h[1:2 * nh:2]
+
+    def tensor_amplitudes(self, p):
+        """9 helicity amplitudes (h3, h4 in -1, 0, +1) of the tensor-pomeron 2xV
+        ME4 at momenta p = (pa, pb, p1, p2, p3, p4), each (E, px, py, pz)."""
+        p = np.ascontiguousarray(p, dtype=np.float64).reshape(6, 4)
+        h = np.zeros(18)
+        self.L.ora_tensor_amplitudes(self.h, abi.dptr(p), abi.dptr(h))
+        return h[0::2] + 1j * h[1::2]
+
+    def tensor_mtensor(self, p):
+        """Amplitude tensor M(r3, r4) (indices down, (t, x, y, z)) of the 2xV ME4."""
+        p = np.ascontiguousarray(p, dtype=np.float64).reshape(6, 4)
+        o = np.zeros(32)
+        self.L.ora_tensor_mtensor(self.h, abi.dptr(p), abi.dptr(o))
+        return (o[0::2] + 1j * o[1::2]).reshape(4, 4)
 
     def vegas_iterate(self, grid, n, seed, it, nthreads=8, first=0, use_mutex=0):
         st = abi.GraIterStats()

@@ -204,3 +204,40 @@ def test_unweighting_efficiency():
     assert no == (w > wmax).sum()
     expect = np.minimum(w, 0).sum() + (w[w <= wmax] / wmax).sum()
     assert abs(na - expect) < 5 * np.sqrt(expect)
+
+
+# The tensor-pomeron |M|^2 depends on t and on pt = pa - p1 - p3 formed from
+# 6.5 TeV components: independent 1-ulp changes of the final-state momenta move
+# it by 2e-10 (median) to 2.5e-9 (max) relative. GPU and oracle kinematics differ
+# only where device and host libm round a transcendental differently, so the
+# pointwise bound is 1e-9 and the median must stay at round-off level.
+TENSOR_RTOL = 1e-9
+
+
+def tensor_cfg(screening):
+    cfg = abi.cfg_default(3)
+    cfg.amp_precision = abi.GRA_AMP_FP64
+    cfg.screening = screening
+    return small_tables(cfg, 1000) if screening else cfg
+
+
+def test_config3_tensor_fp64_pointwise():
+    cfg = tensor_cfg(0)
+    say("case: preset 3 (tensor pomeron rho rho) FP64 unscreened, 20000 points, seed 5")
+    u = points(20000, cfg.dim, 5)
+    with MContinuum(cfg) as P:
+        Wg, ag, fg = P.eval(u)
+    Wo, ao, fo = ol.OracleRun(cfg).eval(u)
+    rel = compare(Wg, Wo, fg, fo, TENSOR_RTOL, "cfg3 tensor fp64")
+    assert np.median(rel) < 1e-12
+
+
+def test_config3_tensor_fp64_screened():
+    cfg = tensor_cfg(1)
+    say("case: preset 3 (tensor pomeron rho rho) FP64 screened, 400 points, seed 6")
+    u = points(400, cfg.dim, 6)
+    with MContinuum(cfg) as P:
+        Wg, ag, fg = P.eval(u)
+    Wo, ao, fo = ol.OracleRun(cfg).eval(u)
+    rel = compare(Wg, Wo, fg, fo, TENSOR_RTOL, "cfg3 tensor fp64 screened")
+    assert np.median(rel) < 1e-12
