@@ -211,7 +211,8 @@ def main():
         f"{sigma_err:.2e} barn")
 
     roofline = None
-    if cfg.screening and n_scr > 0:
+    # per-node operation counts are those of the Regge continuum node
+    if cfg.screening and n_scr > 0 and cfg.process == abi.GRA_PROC_REGGE_CON:
         nodes = (cfg.n_loop_phi + 1) * (cfg.n_loop_kt + 1)
         cand_per_launch = n_valid / n_scr
         avg_ms = ms_scr / n_scr

@@ -602,6 +602,200 @@ __global__ void k_coef(const double* __restrict__ w, int n, double inv_s2, float
   if (b < n) c[b] = (float)(w[b] * w[b] * inv_s2);
 }
 
+// ---------------------------------------------------------------------------
+// Fused coupling layer, sampling direction (BF16 MFMA MLP): one block of 256
+// threads takes kRows points through the whole layer. The conditioner MLP runs
+// out of LDS (input layer on the VALU in FP32, hidden and output products on
+// v_mfma_f32_16x16x32_bf16 with the transposed BF16 weights read through L2),
+// the spline parameters stay in LDS, and each thread then inverts one
+// (point, transformed dimension) spline with the knots accumulated in order.
+// Only u_{l-1}, log q and, when the layer's activations are needed for the
+// backward pass, the activations and spline parameters are written to HBM.
+// ---------------------------------------------------------------------------
+constexpr int kRows = 32;
+
+struct FusedLayer {
+  const float* W1;            // [dA x H]
+  const float* b1;            // [H]
+  const __bf16* wt[4];        // transposed weights of products 1..depth: [N_i x H]
+  const float* bias[4];       // biases of products 1..depth
+  __bf16* act_out[4];         // optional stores of activations 0..depth-1 ([n x H])
+  float* prm_out;             // optional store of the spline parameters ([n x ldp])
+};
+
+// One product of the MLP on the block's kRows rows: out = A[kRows x H] Wt^T,
+// with bias (+ ReLU into the next BF16 activation buffer, or into the FP32
+// parameter tile for the output product)
+template <int H>
+__device__ __forceinline__ void fused_product(const __bf16* sA, int lda, const __bf16* __restrict__ wt,
+                                              const float* __restrict__ bias, int N,
+                                              int nvalid, bool out_layer, __bf16* sOut, int ldo,
+                                              float* sP, int ldpp, __bf16* gact, float* gprm,
+                                              int row0, int n, int ldp) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int ntiles = N / 16;
+  const int kq = 8 * (lane >> 4);
+  for (int nt0 = wave; nt0 < ntiles; nt0 += 16) {
+    // each wave owns n-tiles nt0, nt0 + 4, nt0 + 8, nt0 + 12 of this chunk
+    f32x4 acc[2][4];
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc[m][t] = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k0 = 0; k0 < H; k0 += 32) {
+      const bf16x8 a0 = *(const bf16x8*)(sA + (size_t)(lane & 15) * lda + k0 + kq);
+      const bf16x8 a1 = *(const bf16x8*)(sA + (size_t)(16 + (lane & 15)) * lda + k0 + kq);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int nt = nt0 + 4 * t;
+        if (nt < ntiles) {
+          const bf16x8 b = *(const bf16x8*)(wt + (size_t)(nt * 16 + (lane & 15)) * H + k0 + kq);
+          acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b, acc[0][t], 0, 0, 0);
+          acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b, acc[1][t], 0, 0, 0);
+        }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int nt = nt0 + 4 * t;
+      if (nt >= ntiles) continue;
+      const int col = nt * 16 + (lane & 15);
+      const float bv = bias[col];
+#pragma unroll
+      for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = 16 * m + 4 * (lane >> 4) + r;
+          float v = acc[m][t][r] + bv;
+          if (out_layer) {
+            if (col >= nvalid) v = 0.f;
+            sP[row * ldpp + col] = v;
+            if (gprm && row0 + row < n) gprm[(size_t)(row0 + row) * ldp + col] = v;
+          } else {
+            v = fmaxf(v, 0.f);
+            const __bf16 hv = (__bf16)v;
+            sOut[row * ldo + col] = hv;
+            if (gact && row0 + row < n) gact[(size_t)(row0 + row) * H + col] = hv;
+          }
+        }
+    }
+  }
+}
+
+template <int H>
+__global__ __launch_bounds__(256) void k_flow_layer_inverse(const float* __restrict__ u_in,
+                                                            float* __restrict__ u_out,
+                                                            float* __restrict__ ld, int n, int D,
+                                                            LayerIdx L, int K, int depth, int ldp,
+                                                            FusedLayer W) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  constexpr int lda = H + 8;  // BF16 row stride (16-byte multiple, staggers banks)
+  const int ldpp = ldp + 1;
+  __bf16* sAct0 = (__bf16*)smem;
+  __bf16* sAct1 = sAct0 + kRows * lda;
+  float* sP = (float*)(sAct1 + kRows * lda);
+  float* sLj = sP + kRows * ldpp;
+  float* sU = sLj + kRows * 8;  // the block's input rows [kRows x D]
+  const int row0 = blockIdx.x * kRows;
+  for (int e = threadIdx.x; e < kRows * D; e += blockDim.x) {
+    const int row = row0 + e / D;
+    sU[e] = row < n ? u_in[(size_t)row0 * D + e] : 0.f;
+  }
+  __syncthreads();
+  // input layer (FP32 VALU): act0 = relu(u[A] W1 + b1); each thread owns one
+  // column, with its weights in registers, for kRows / (256 / H) rows
+  {
+    constexpr int nrg = 256 / H;
+    const int j = threadIdx.x % H, rg = threadIdx.x / H;
+    float w[kMaxDB];
+#pragma unroll
+    for (int i = 0; i < kMaxDB; ++i) w[i] = i < L.dA ? W.W1[i * H + j] : 0.f;
+    const float bj = W.b1[j];
+    for (int r = rg; r < kRows; r += nrg) {
+      float acc = bj;
+#pragma unroll
+      for (int i = 0; i < kMaxDB; ++i)
+        if (i < L.dA) acc += sU[r * D + L.A[i]] * w[i];
+      const __bf16 hv = (__bf16)fmaxf(acc, 0.f);
+      sAct0[r * lda + j] = hv;
+      if (W.act_out[0] && row0 + r < n) W.act_out[0][(size_t)(row0 + r) * H + j] = hv;
+    }
+  }
+  __syncthreads();
+  __bf16* cur = sAct0;
+  __bf16* nxt = sAct1;
+  const int np = 3 * K - 1;
+  for (int i = 1; i <= depth; ++i) {
+    const bool out = i == depth;
+    fused_product<H>(cur, lda, W.wt[i - 1], W.bias[i - 1], out ? ldp : H, out ? L.dB * np : H, out,
+                  nxt, lda, sP, ldpp, out ? nullptr : W.act_out[i], out ? W.prm_out : nullptr,
+                  row0, n, ldp);
+    __syncthreads();
+    __bf16* t = cur;
+    cur = nxt;
+    nxt = t;
+  }
+  // spline inverses: thread -> (row, transformed dimension)
+  const int t = threadIdx.x;
+  const int r = t / L.dB, j = t % L.dB;
+  const int row = row0 + r;
+  if (r < kRows && row < n) {
+    if (j == 0)
+      for (int i = 0; i < L.dA; ++i) u_out[(size_t)row * D + L.A[i]] = sU[r * D + L.A[i]];
+    const float* p = sP + r * ldpp + j * np;
+    float mw = p[0], mh = p[K];
+    for (int k = 1; k < K; ++k) {
+      mw = fmaxf(mw, p[k]);
+      mh = fmaxf(mh, p[K + k]);
+    }
+    float sw = 0.f, sh = 0.f;
+    for (int k = 0; k < K; ++k) {
+      sw += __expf(p[k] - mw);
+      sh += __expf(p[K + k] - mh);
+    }
+    const float cw = 1.f - K * kEpsW;
+    const float iw = cw / sw, ih = cw / sh;
+    const float zv = fminf(fmaxf(sU[r * D + L.B[j]], 0.f), 1.f);
+    float X = 0.f, Y = 0.f, w = 0.f, h = 0.f;
+    int k = 0;
+    for (; k < K; ++k) {
+      w = kEpsW + __expf(p[k] - mw) * iw;
+      h = kEpsW + __expf(p[K + k] - mh) * ih;
+      if (k == K - 1 || Y + h > zv) break;
+      X += w;
+      Y += h;
+    }
+    Bin e;
+    e.k = k;
+    e.X = X;
+    e.w = w;
+    e.Y = Y;
+    e.h = h;
+    e.dk = (k == 0) ? 1.f : kEpsD + softplusf(p[2 * K + k - 1]);
+    e.dk1 = (k == K - 1) ? 1.f : kEpsD + softplusf(p[2 * K + k]);
+    e.s = e.h / e.w;
+    const float zeta = zv - e.Y;
+    const float Dl = e.dk1 + e.dk - 2.f * e.s;
+    const float qa = e.h * (e.s - e.dk) + zeta * Dl;
+    const float qb = e.h * e.dk - zeta * Dl;
+    const float qc = -e.s * zeta;
+    const float disc = fmaxf(qb * qb - 4.f * qa * qc, 0.f);
+    float xi = (2.f * qc) / (-qb - sqrtf(disc));
+    xi = fminf(fmaxf(xi, 0.f), 1.f);
+    float zz, lj;
+    rqs_value(e, xi, zz, lj);
+    u_out[(size_t)row * D + L.B[j]] = fminf(fmaxf(e.X + xi * e.w, 0.f), 0.99999994f);
+    sLj[r * 8 + j] = lj;
+  }
+  __syncthreads();
+  if (t < kRows && row0 + t < n) {
+    float acc = 0.f;
+    for (int jj = 0; jj < L.dB; ++jj) acc += sLj[t * 8 + jj];
+    ld[row0 + t] += acc;
+  }
+}
+
 }  // namespace
 
 // one spline per group of G >= K lanes (G = 8, 16, 32)
@@ -816,9 +1010,16 @@ static void mlp_forward(gra_handle* h, FlowState* F, int l, const float* u, int 
   }
 }
 
-// Sampling pass: z -> u_0 (stored per layer), ld = log q(u_0)
+static size_t fused_lds_bytes(const FlowState* F) {
+  return (size_t)2 * kRows * (F->H + 8) * sizeof(__bf16) + (size_t)kRows * (F->ldp + 1) * 4 +
+         (size_t)kRows * 8 * 4 + (size_t)kRows * F->D * 4;
+}
+
+// Sampling pass: z -> u_0 (stored per layer), ld = log q(u_0). With store set,
+// the MLP activations and spline parameters of every layer are kept for the
+// backward pass.
 static int flow_sample_pass(gra_handle* h, FlowState* F, int n, uint64_t seed, uint32_t step,
-                            uint64_t first, const float* zin) {
+                            uint64_t first, const float* zin, bool store) {
   const int D = F->D, L = F->L;
   if (zin)
     GRA_HIP_CHECK(h, hipMemcpyAsync(u_of(F, L), zin, sizeof(float) * n * D, hipMemcpyHostToDevice,
@@ -827,6 +1028,40 @@ static int flow_sample_pass(gra_handle* h, FlowState* F, int n, uint64_t seed, u
     hipLaunchKernelGGL(k_flow_z, dim3((n + 255) / 256), dim3(256), 0, h->stream, seed, step, first,
                        n, D, u_of(F, L));
   GRA_HIP_CHECK(h, hipMemsetAsync(F->ld.p, 0, sizeof(float) * n, h->stream));
+  if (!F->mlp_fp32) {
+    const size_t lds = fused_lds_bytes(F);
+    for (int l = L - 1; l >= 0; --l) {
+      const FlowLayerHost& Ly = F->layers[l];
+      FusedLayer W;
+      std::memset(&W, 0, sizeof(W));
+      const float* th = F->theta.as<float>();
+      W.W1 = th + Ly.oW1;
+      W.b1 = th + Ly.ob1;
+      for (int i = 1; i <= F->depth; ++i) {
+        W.wt[i - 1] = F->wt16.as<__bf16>() + F->wofs[l * F->depth + i - 1];
+        W.bias[i - 1] = th + (i == F->depth ? Ly.obo : Ly.obh[i]);
+      }
+      if (store) {
+        for (int i = 0; i < F->depth; ++i) W.act_out[i] = act16_of(F, l, i);
+        W.prm_out = prm_of(F, l);
+      }
+      const dim3 grd((n + kRows - 1) / kRows);
+      const float* uin = u_of(F, l + 1);
+      float* uout = u_of(F, l);
+      float* ldp_ = F->ld.as<float>();
+      if (F->H == 64)
+        hipLaunchKernelGGL(k_flow_layer_inverse<64>, grd, dim3(256), lds, h->stream, uin, uout,
+                           ldp_, n, D, Ly.idx, F->K, F->depth, F->ldp, W);
+      else if (F->H == 128)
+        hipLaunchKernelGGL(k_flow_layer_inverse<128>, grd, dim3(256), lds, h->stream, uin, uout,
+                           ldp_, n, D, Ly.idx, F->K, F->depth, F->ldp, W);
+      else
+        hipLaunchKernelGGL(k_flow_layer_inverse<256>, grd, dim3(256), lds, h->stream, uin, uout,
+                           ldp_, n, D, Ly.idx, F->K, F->depth, F->ldp, W);
+    }
+    GRA_HIP_CHECK(h, hipGetLastError());
+    return GRA_OK;
+  }
   for (int l = L - 1; l >= 0; --l) {
     // layer l maps u_l (stored at slot l) to u_{l+1}; inverting reads slot l+1
     mlp_forward(h, F, l, u_of(F, l + 1), n);
@@ -1062,6 +1297,26 @@ extern "C" int gra_hip_flow_init(gra_handle* h, const gra_flow_cfg* fc_in) {
     flow_release(F);
     return fail(h, GRA_E_HIP, "flow parameter upload");
   }
+  if (!F->mlp_fp32) {
+    const size_t lds = fused_lds_bytes(F);
+    if (lds > 160 * 1024) {
+      flow_release(F);
+      return fail(h, GRA_E_ARG, "flow layer tile needs %zu B of LDS (hidden %d, %d spline columns)",
+                  lds, fc.hidden, F->ldp);
+    }
+    if (fc.hidden != 64 && fc.hidden != 128 && fc.hidden != 256) {
+      flow_release(F);
+      return fail(h, GRA_E_ARG, "flow hidden %d: the BF16 path supports 64, 128, 256", fc.hidden);
+    }
+    const void* kern = fc.hidden == 64 ? (const void*)k_flow_layer_inverse<64>
+                       : fc.hidden == 128 ? (const void*)k_flow_layer_inverse<128>
+                                          : (const void*)k_flow_layer_inverse<256>;
+    if (hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+        hipSuccess) {
+      flow_release(F);
+      return fail(h, GRA_E_HIP, "flow: cannot reserve %zu B of LDS per block", lds);
+    }
+  }
   h->flow = F;
   int r = refresh_weights(h, F);
   if (r) return r;
@@ -1103,7 +1358,7 @@ extern "C" int gra_hip_flow_train(gra_handle* h, int32_t steps, gra_flow_stats* 
   const auto t0 = std::chrono::steady_clock::now();
   double st[3] = {0, 0, 0};
   for (int it = 0; it < steps; ++it) {
-    r = flow_sample_pass(h, F, n, F->fc.seed, (uint32_t)F->step, 0, nullptr);
+    r = flow_sample_pass(h, F, n, F->fc.seed, (uint32_t)F->step, 0, nullptr, true);
     if (r) return r;
     r = flow_weights(h, F, n, st);
     if (r) return r;
@@ -1146,7 +1401,7 @@ extern "C" int gra_hip_flow_sample(gra_handle* h, uint64_t n_total, uint64_t see
   while (done < n_total) {
     const int n = (int)std::min<uint64_t>(chunk, n_total - done);
     const int npad = ((n + 255) / 256) * 256;
-    r = flow_sample_pass(h, F, npad, seed, 0x40000000u | call, done, nullptr);
+    r = flow_sample_pass(h, F, npad, seed, 0x40000000u | call, done, nullptr, false);
     if (r) return r;
     double st[3];
     r = flow_weights(h, F, n, st);
@@ -1199,7 +1454,7 @@ extern "C" int gra_hip_flow_forward(gra_handle* h, const float* z, size_t n, flo
   if (n == 0 || n % 64 || n > (1u << 24)) return fail(h, GRA_E_ARG, "flow_forward: n %% 64 != 0");
   int r = flow_alloc_batch(h, F, (int)n);
   if (r) return r;
-  r = flow_sample_pass(h, F, (int)n, 0, 0, 0, z);
+  r = flow_sample_pass(h, F, (int)n, 0, 0, 0, z, false);
   if (r) return r;
   GRA_HIP_CHECK(h, hipMemcpyAsync(x, u_of(F, 0), sizeof(float) * n * F->D, hipMemcpyDeviceToHost,
                                   h->stream));
