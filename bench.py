@@ -143,6 +143,37 @@ def flow_leg(P, grid, args, vegas_sigma, vegas_err):
     return res
 
 
+COPY_BYTES = 256 << 20
+
+
+def calibrate_copy(reps=3):
+    """Device-to-device copies of COPY_BYTES (the runtime's blit kernel): the
+    reference dispatches that scale the FETCH_SIZE / WRITE_SIZE counters of a
+    profiled run to bytes."""
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(COPY_BYTES)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(COPY_BYTES)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(COPY_BYTES)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, C.c_size_t(COPY_BYTES), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
+def screen_traffic():
+    """HBM bytes per screening launch measured with FETCH_SIZE / WRITE_SIZE
+    (profiles/k_screen_hbm_traffic_screened_pipi.md), or None."""
+    path = os.path.join(ROOT, "profiles", "k_screen_hbm_traffic_screened_pipi.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +186,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1 << 17)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-flow", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run 256 MiB device copies (counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--flow-steps", type=int, default=500)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -162,6 +195,8 @@ def main():
                           "the multi-GPU path is a design note only (bench.py --gpus 1)"}))
         sys.exit(2)
 
+    if args.calibrate_copy:
+        calibrate_copy()
     cfg = abi.cfg_default(args.preset)
     N = args.points
     P = MContinuum(cfg)
@@ -221,7 +256,7 @@ def main():
         roofline = {
             "bound": "valu", "kernel": "k_screen (screening loop)",
             "achieved": achieved, "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / VALU_PEAK_TFLOPS, "traffic": None,
+            "frac": achieved / VALU_PEAK_TFLOPS, "traffic": screen_traffic(),
             "avg_launch_ms": avg_ms, "launches": n_scr,
             "fp64_flop_per_node": FP64_FLOP_PER_NODE, "fp32_flop_per_node": FP32_FLOP_PER_NODE,
             "nodes_per_point": nodes, "screened_points_per_launch": cand_per_launch,
