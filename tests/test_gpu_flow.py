@@ -245,3 +245,31 @@ def test_flow_training_reduces_variance_and_keeps_sigma():
     assert s1.cv2 < s0.cv2
     comb = np.sqrt(res.sigma_err ** 2 + s1.sigma_err ** 2)
     assert abs(s1.sigma - res.sigma) < 3 * comb
+
+
+def test_flow_training_on_tensor_process():
+    # the 2->6 sampler's flow shape (D = 14, 10 layers, K = 32, 3 x 256, BF16)
+    # on the screened tensor-pomeron rho rho process with small eikonal tables
+    cfg = abi.cfg_default(3)
+    cfg.amp_precision = abi.GRA_AMP_FP64
+    cfg.n_bt = cfg.n_kt2 = cfg.n_fb = 1000
+    say("case: flow D=14 L=10 K=32 H=256 depth=3 on the screened rho rho process, 30 steps x 16384")
+    with MContinuum(cfg) as P:
+        G = MGraniitti(P)
+        res = G.Integrate(vegas_cfg(1 << 18, 4, burnin=4, seed=77))
+        fc = flow_cfg(14, 10, 32, 256, 3, False, batch=16384, lr=1e-3)
+        abi.check(P.lib.gra_hip_flow_init(P.h, C.byref(fc)), P.h)
+        abi.check(P.lib.gra_hip_flow_set_grid(P.h, C.byref(G.grid)), P.h)
+        s0 = abi.GraFlowStats()
+        abi.check(P.lib.gra_hip_flow_sample(P.h, 1 << 18, 5, C.byref(s0)), P.h)
+        st = abi.GraFlowStats()
+        abi.check(P.lib.gra_hip_flow_train(P.h, 30, C.byref(st)), P.h)
+        s1 = abi.GraFlowStats()
+        abi.check(P.lib.gra_hip_flow_sample(P.h, 1 << 18, 6, C.byref(s1)), P.h)
+    say(f"VEGAS sigma {res.sigma:.5e} +- {res.sigma_err:.2e}; flow before: cv2 {s0.cv2:.3f} "
+        f"ess {s0.ess:.3f}; after: sigma {s1.sigma:.5e} +- {s1.sigma_err:.2e} cv2 {s1.cv2:.3f} "
+        f"ess {s1.ess:.3f} fallback {s1.fallback}")
+    assert s1.cv2 < s0.cv2
+    assert s1.fallback == (1 if s1.ess < fc.ess_fallback else 0)
+    comb = np.sqrt(res.sigma_err ** 2 + s1.sigma_err ** 2)
+    assert abs(s1.sigma - res.sigma) < 3 * comb
