@@ -84,15 +84,16 @@ def flow_stats_sums(st):
 
 
 def flow_leg(P, grid, args, vegas_sigma, vegas_err):
-    """Learned sampler on the same process: RQS flow (6 coupling layers, 16 bins,
-    2 x 128 hidden, BF16 MFMA) composed with the adapted VEGAS grid, trained by
-    weight-variance minimisation (Adam, lr 1e-3) on batches of 2^18 of its own
-    weighted points, then sampled in chunks of 2^20 until the relative sigma
-    error reaches 0.1%."""
+    """Learned sampler on the same process: RQS flow (by default 6 coupling
+    layers, 16 bins, 2 x 128 hidden, BF16 MFMA) composed with the adapted VEGAS
+    grid, trained by weight-variance minimisation (Adam, lr 1e-3) on batches of
+    its own weighted points (2^18 by default), then sampled in chunks of 2^20
+    until the relative sigma error reaches 0.1%."""
     lib = P.lib
     fc = abi.GraFlowCfg()
-    fc.dim, fc.layers, fc.bins, fc.hidden, fc.depth = P.cfg.dim, 6, 16, 128, 2
-    fc.steps, fc.batch, fc.lr = args.flow_steps, 1 << 18, 1e-3
+    fc.dim, fc.layers, fc.bins = P.cfg.dim, args.flow_layers, args.flow_bins
+    fc.hidden, fc.depth = args.flow_hidden, args.flow_depth
+    fc.steps, fc.batch, fc.lr = args.flow_steps, args.flow_batch, 1e-3
     fc.beta1, fc.beta2, fc.eps = 0.9, 0.999, 1e-8
     fc.seed, fc.ess_fallback = 2024, 0.1
     abi.check(lib.gra_hip_flow_init(P.h, C.byref(fc)), P.h)
@@ -105,7 +106,8 @@ def flow_leg(P, grid, args, vegas_sigma, vegas_err):
     t0 = time.perf_counter()
     abi.check(lib.gra_hip_flow_train(P.h, args.flow_steps, C.byref(st)), P.h)
     t_train = time.perf_counter() - t0
-    log(f"flow: trained {args.flow_steps} steps x 2^18 in {t_train:.3f} s; last batch cv2 {st.cv2:.3f}")
+    log(f"flow: trained {args.flow_steps} steps x {args.flow_batch} in {t_train:.3f} s; "
+        f"last batch cv2 {st.cv2:.3f}")
     s1 = s2 = mx = n = 0.0
     t0 = time.perf_counter()
     calls = 0
@@ -126,8 +128,9 @@ def flow_leg(P, grid, args, vegas_sigma, vegas_err):
     pts = n / t_sample
     comb = np.hypot(err, vegas_err)
     res = {
-        "layers": 6, "bins": 16, "hidden": "2x128", "mlp": "bf16 mfma, fp32 accumulate",
-        "train_steps": args.flow_steps, "train_batch": 1 << 18, "train_s": t_train,
+        "layers": args.flow_layers, "bins": args.flow_bins,
+        "hidden": f"{args.flow_depth}x{args.flow_hidden}", "mlp": "bf16 mfma, fp32 accumulate",
+        "train_steps": args.flow_steps, "train_batch": args.flow_batch, "train_s": t_train,
         "sample_s_to_0p1pct": t_sample, "time_to_0p1pct_s": t_train + t_sample,
         "points_to_0p1pct": int(n), "reached_0p1pct": bool(err <= 1e-3 * mean),
         "sigma_barn": mean, "sigma_err_barn": err,
@@ -142,6 +145,13 @@ def flow_leg(P, grid, args, vegas_sigma, vegas_err):
     log(f"flow: {res}")
     return res
 
+
+WORKLOADS = {
+    1: "pp->p pi+ pi- p 13 TeV, Regge continuum, unscreened, FP64",
+    2: "pp->p pi+ pi- p 13 TeV, Regge continuum, screened (16x31 loop nodes), FP32 amplitude",
+    3: "pp->p rho0 rho0 p -> p 2(pi+ pi-) p 13 TeV, tensor pomeron 2xV, screened (16x31 loop "
+       "nodes), FP64 amplitude",
+}
 
 COPY_BYTES = 256 << 20
 
@@ -189,6 +199,11 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run 256 MiB device copies (counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--flow-steps", type=int, default=500)
+    ap.add_argument("--flow-layers", type=int, default=6)
+    ap.add_argument("--flow-bins", type=int, default=16)
+    ap.add_argument("--flow-hidden", type=int, default=128)
+    ap.add_argument("--flow-depth", type=int, default=2)
+    ap.add_argument("--flow-batch", type=int, default=1 << 18)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: every box here has one MI355X; "
@@ -286,8 +301,8 @@ def main():
         "value": value, "unit": "unweighted events/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
-        "config": {"workload": "pp->p pi+ pi- p 13 TeV, screened (16x31 loop nodes), "
-                               "VEGAS 2^20 points/iteration, FP32 amplitude",
+        "config": {"workload": WORKLOADS.get(args.preset, f"preset {args.preset}") +
+                   f", VEGAS {N} points/iteration",
                    "preset": args.preset, "points_per_step": N, "integrator": "VEGAS"},
         "points_per_s": pts_per_s, "unweighting_efficiency": eff,
         "time_to_0p1pct_s": t_prec, "precision_run": prec_res,
