@@ -401,11 +401,7 @@ int build_eikonal(gra_handle* h) {
 
 }  // namespace
 
-// The reduced FP32-amplitude kernels implement the Regge continuum; the
-// tensor-pomeron process runs the FP64 kernels in either precision mode.
-static bool f32_kernels(const RunParams& P) {
-  return P.amp_mode == GRA_AMP_FP32 && P.process == GRA_PROC_REGGE_CON;
-}
+static bool f32_kernels(const RunParams& P) { return P.amp_mode == GRA_AMP_FP32; }
 
 int ensure_point_buffers(gra_handle* h, uint64_t n) {
   const int D = h->P.dim;

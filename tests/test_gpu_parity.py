@@ -241,3 +241,18 @@ def test_config3_tensor_fp64_screened():
     Wo, ao, fo = ol.OracleRun(cfg).eval(u)
     rel = compare(Wg, Wo, fg, fo, TENSOR_RTOL, "cfg3 tensor fp64 screened")
     assert np.median(rel) < 1e-12
+
+
+def test_config3_tensor_fp32_screened_error():
+    # FP64 kinematics, FP32 helicity amplitudes through the screening loop with
+    # compensated FP32 sums; tree amplitudes in FP64
+    cfg = tensor_cfg(1)
+    cfg.amp_precision = abi.GRA_AMP_FP32
+    say("case: preset 3 (tensor pomeron rho rho) FP32 amplitude, screened, 400 points, seed 7")
+    u = points(400, cfg.dim, 7)
+    with MContinuum(cfg) as P:
+        Wg, ag, fg = P.eval(u)
+    Wo, ao, fo = ol.OracleRun(tensor_cfg(1)).eval(u)
+    rel = compare(Wg, Wo, fg, fo, 1e-3, "cfg3 tensor fp32 screened")
+    say(f"cfg3 fp32 |M|^2 rel err: median {np.median(rel):.2e} p99 {np.quantile(rel, 0.99):.2e}")
+    assert np.median(rel) < 1e-5
