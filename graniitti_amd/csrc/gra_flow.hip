@@ -296,6 +296,149 @@ __global__ __launch_bounds__(256) void k_rqs_backward(
   if (logq_chk && m == 0) logq_chk[b] += ljsum;
 }
 
+// Training direction, backward of layer l with one thread per (point,
+// transformed dimension): the block's spline-parameter rows are staged in LDS
+// (coalesced), each thread overwrites its own 3K - 1 parameters with their
+// gradients, and the rows are written back as FP32 and BF16 (coalesced). The
+// softmax backward needs sum_m sw_m G_m, which for the width (height) gradient
+// G_m = G_X (m < k), G_w (m = k), 0 (m > k) is
+//   G_X (X_k - k eps) / (1 - K eps) + G_w sw_k,
+// so no per-bin arrays are kept.
+__global__ __launch_bounds__(256) void k_rqs_backward_t(
+    const float* __restrict__ prm, int ldp, const float* __restrict__ u_prev,
+    const float* __restrict__ g_in, const float* __restrict__ coef, float* __restrict__ g_out,
+    float* __restrict__ gp, __bf16* __restrict__ gp16, int n, int D, LayerIdx L, int K, int R,
+    float* __restrict__ logq_chk) {
+  extern __shared__ float tile[];  // [R][ldp + 1]
+  __shared__ float sLj[256];
+  const int ldt = ldp + 1;
+  const int row0 = blockIdx.x * R;
+  // the block's rows are contiguous in prm: walk them with an incremental (row, column)
+  const int nrow = min(R, n - row0);
+  {
+    int r = threadIdx.x / ldp, c = threadIdx.x % ldp;
+    const int dr = blockDim.x / ldp, dc = blockDim.x % ldp;
+    for (int e = threadIdx.x; e < nrow * ldp; e += blockDim.x) {
+      tile[r * ldt + c] = prm[(size_t)row0 * ldp + e];
+      r += dr;
+      c += dc;
+      if (c >= ldp) {
+        c -= ldp;
+        ++r;
+      }
+    }
+  }
+  __syncthreads();
+  const int t = threadIdx.x;
+  const int r = t / L.dB, j = t % L.dB;
+  const int b = row0 + r;
+  const int np = 3 * K - 1;
+  const float cw = 1.f - K * kEpsW;
+  float lj = 0.f;
+  if (r < R && b < n) {
+    const float* up = u_prev + (size_t)b * D;
+    const float* gi = g_in + (size_t)b * D;
+    float* go = g_out + (size_t)b * D;
+    if (j == 0)
+      for (int i = 0; i < L.dA; ++i) go[L.A[i]] = gi[L.A[i]];
+    const float gld = -coef[b];
+    float* p = tile + r * ldt + j * np;
+    float mw = p[0], mh = p[K];
+    for (int m = 1; m < K; ++m) {
+      mw = fmaxf(mw, p[m]);
+      mh = fmaxf(mh, p[K + m]);
+    }
+    float sw = 0.f, sh = 0.f;
+    for (int m = 0; m < K; ++m) {
+      sw += __expf(p[m] - mw);
+      sh += __expf(p[K + m] - mh);
+    }
+    const float iw = 1.f / sw, ih = 1.f / sh;
+    const float x = fminf(fmaxf(up[L.B[j]], 0.f), 1.f);
+    float X = 0.f, Y = 0.f, swk = 0.f, shk = 0.f;
+    int k = 0;
+    for (; k < K; ++k) {
+      swk = __expf(p[k] - mw) * iw;
+      shk = __expf(p[K + k] - mh) * ih;
+      const float w = kEpsW + cw * swk;
+      if (k == K - 1 || X + w > x) break;
+      X += w;
+      Y += kEpsW + cw * shk;
+    }
+    Bin e;
+    e.k = k;
+    e.X = X;
+    e.Y = Y;
+    e.w = kEpsW + cw * swk;
+    e.h = kEpsW + cw * shk;
+    const float sgk = (k > 0) ? sigmoidf(p[2 * K + k - 1]) : 0.f;
+    const float sgk1 = (k < K - 1) ? sigmoidf(p[2 * K + k]) : 0.f;
+    e.dk = (k == 0) ? 1.f : kEpsD + softplusf(p[2 * K + k - 1]);
+    e.dk1 = (k == K - 1) ? 1.f : kEpsD + softplusf(p[2 * K + k]);
+    e.s = e.h / e.w;
+    const float xi = fminf(fmaxf((x - e.X) / e.w, 0.f), 1.f);
+    const float tt = xi * (1.f - xi), om = 1.f - xi;
+    const float Dl = e.dk1 + e.dk - 2.f * e.s;
+    const float Dn = e.s + Dl * tt;
+    const float N = e.h * (e.s * xi * xi + e.dk * tt);
+    const float Pn = e.dk1 * xi * xi + 2.f * e.s * tt + e.dk * om * om;
+    lj = 2.f * __logf(e.s) + __logf(Pn) - 2.f * __logf(Dn);
+    const float gu = gi[L.B[j]];
+    const float N_xi = e.h * (2.f * e.s * xi + e.dk * (1.f - 2.f * xi));
+    const float N_s = e.h * xi * xi, N_h = e.s * xi * xi + e.dk * tt, N_dk = e.h * tt;
+    const float D_xi = Dl * (1.f - 2.f * xi), D_s = 1.f - 2.f * tt, D_dk = tt, D_dk1 = tt;
+    const float P_xi = 2.f * e.dk1 * xi + 2.f * e.s * (1.f - 2.f * xi) - 2.f * e.dk * om;
+    const float P_s = 2.f * tt, P_dk = om * om, P_dk1 = xi * xi;
+    const float iD = 1.f / Dn, iD2 = iD * iD, iP = 1.f / Pn;
+    const float G_xi = gu * (N_xi * Dn - N * D_xi) * iD2 + gld * (P_xi * iP - 2.f * D_xi * iD);
+    const float G_s = gu * (N_s * Dn - N * D_s) * iD2 + gld * (2.f / e.s + P_s * iP - 2.f * D_s * iD);
+    const float G_h0 = gu * N_h * iD;
+    const float G_dk = gu * (N_dk * Dn - N * D_dk) * iD2 + gld * (P_dk * iP - 2.f * D_dk * iD);
+    const float G_dk1 = gu * (-N * D_dk1) * iD2 + gld * (P_dk1 * iP - 2.f * D_dk1 * iD);
+    const float G_Y = gu;
+    const float iwk = 1.f / e.w;
+    const float G_X = -G_xi * iwk;
+    const float G_w = -G_xi * xi * iwk - G_s * e.s * iwk;
+    const float G_h = G_h0 + G_s * iwk;
+    go[L.B[j]] = G_xi * iwk;
+    const float icw = 1.f / cw;
+    const float sdw = G_X * (X - k * kEpsW) * icw + G_w * swk;
+    const float sdh = G_Y * (Y - k * kEpsW) * icw + G_h * shk;
+    for (int m = 0; m < K; ++m) {
+      const float swm = __expf(p[m] - mw) * iw;
+      const float shm = __expf(p[K + m] - mh) * ih;
+      const float Gwm = (m < k) ? G_X : (m == k ? G_w : 0.f);
+      const float Ghm = (m < k) ? G_Y : (m == k ? G_h : 0.f);
+      p[m] = cw * swm * (Gwm - sdw);
+      p[K + m] = cw * shm * (Ghm - sdh);
+    }
+    for (int m = 0; m < K - 1; ++m)
+      p[2 * K + m] = (m == k - 1 ? G_dk * sgk : 0.f) + (m == k ? G_dk1 * sgk1 : 0.f);
+  }
+  sLj[t] = lj;
+  __syncthreads();
+  {
+    int rr = threadIdx.x / ldp, c = threadIdx.x % ldp;
+    const int dr = blockDim.x / ldp, dc = blockDim.x % ldp;
+    for (int e2 = threadIdx.x; e2 < nrow * ldp; e2 += blockDim.x) {
+      const float v = tile[rr * ldt + c];
+      gp[(size_t)row0 * ldp + e2] = v;
+      gp16[(size_t)row0 * ldp + e2] = (__bf16)v;
+      rr += dr;
+      c += dc;
+      if (c >= ldp) {
+        c -= ldp;
+        ++rr;
+      }
+    }
+  }
+  if (logq_chk && t < R && row0 + t < n) {
+    float acc = 0.f;
+    for (int jj = 0; jj < L.dB; ++jj) acc += sLj[t * L.dB + jj];
+    logq_chk[row0 + t] += acc;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // MLP kernels
 // ---------------------------------------------------------------------------
@@ -313,33 +456,43 @@ __global__ void k_mlp_in(const float* __restrict__ u, int D, LayerIdx L, const f
   if (h32) h32[(size_t)b * H + j] = acc;
 }
 
-// Input layer backward, dx part: du[b][A_i] += sum_j dh[b][j] W1[i][j]. One
-// wave per point: lanes stride the H columns (coalesced row reads), then a
-// butterfly sum over the wave per conditioning dimension.
+// Input layer backward, dx part: du[b][A_i] += sum_j dh[b][j] W1[i][j]. A
+// block takes 256 points; 32-column slices of dh are staged in LDS with
+// coalesced loads (rows padded to 33 words), then each thread accumulates the
+// dot products of its own point.
 __global__ __launch_bounds__(256) void k_mlp_in_dx(const float* __restrict__ dh,
                                                    const float* __restrict__ W1, int H, int n, int D,
                                                    LayerIdx L, float* __restrict__ g) {
-  const int lane = threadIdx.x & 63;
-  const int b = (int)(((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  if (b >= n) return;
-  const float* row = dh + (size_t)b * H;
+  __shared__ float tile[256][33];
+  __shared__ float wsl[kMaxDB][32];
+  const int t = threadIdx.x;
+  const int b0 = blockIdx.x * 256;
   float acc[kMaxDB];
 #pragma unroll
   for (int i = 0; i < kMaxDB; ++i) acc[i] = 0.f;
-  for (int j = lane; j < H; j += 64) {
-    const float d = row[j];
+  for (int c0 = 0; c0 < H; c0 += 32) {
+    for (int e = t; e < 256 * 32; e += 256) {
+      const int r = e >> 5, c = e & 31;
+      tile[r][c] = (b0 + r < n) ? dh[(size_t)(b0 + r) * H + c0 + c] : 0.f;
+    }
+    if (t < kMaxDB * 32) {
+      const int i = t >> 5, c = t & 31;
+      wsl[i][c] = (i < L.dA) ? W1[i * H + c0 + c] : 0.f;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int c = 0; c < 32; ++c) {
+      const float d = tile[t][c];
+#pragma unroll
+      for (int i = 0; i < kMaxDB; ++i) acc[i] += d * wsl[i][c];
+    }
+    __syncthreads();
+  }
+  const int b = b0 + t;
+  if (b < n) {
 #pragma unroll
     for (int i = 0; i < kMaxDB; ++i)
-      if (i < L.dA) acc[i] += d * W1[i * H + j];
-  }
-#pragma unroll
-  for (int i = 0; i < kMaxDB; ++i) {
-    if (i < L.dA) {
-      float v = acc[i];
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-      if (lane == 0) g[(size_t)b * D + L.A[i]] += v;
-    }
+      if (i < L.dA) g[(size_t)b * D + L.A[i]] += acc[i];
   }
 }
 
@@ -1010,6 +1163,13 @@ static void mlp_forward(gra_handle* h, FlowState* F, int l, const float* u, int 
   }
 }
 
+// Points per block of the spline backward: one thread per transformed
+// dimension, the parameter tile within 159 KB of LDS
+static int bwd_rows(const FlowState* F, int dB) {
+  const int by_lds = (int)((159 * 1024) / (4 * (size_t)(F->ldp + 1)));
+  return std::max(1, std::min(256 / dB, by_lds));
+}
+
 static size_t fused_lds_bytes(const FlowState* F) {
   return (size_t)2 * kRows * (F->H + 8) * sizeof(__bf16) + (size_t)kRows * (F->ldp + 1) * 4 +
          (size_t)kRows * 8 * 4 + (size_t)kRows * F->D * 4;
@@ -1087,8 +1247,14 @@ static int flow_backward(gra_handle* h, FlowState* F, int n, float* logq_chk) {
   for (int l = L - 1; l >= 0; --l) {
     const FlowLayerHost& Ly = F->layers[l];
     // g = dL/du_{l+1}; spline backward gives dL/dprm and dL/du_l (B dims; A passes through)
-    LAUNCH_SPLINE(k_rqs_backward, F->K, n, prm_of(F, l), ldp, u_of(F, l), g, F->coef.as<float>(),
-                  g2, F->gprm.as<float>(), F->gprm16.as<__bf16>(), n, D, Ly.idx, F->K, logq_chk);
+    {
+      const int R = bwd_rows(F, Ly.idx.dB);
+      const size_t lds = sizeof(float) * (size_t)R * (ldp + 1);
+      hipLaunchKernelGGL(k_rqs_backward_t, dim3((n + R - 1) / R), dim3(256), lds, h->stream,
+                         prm_of(F, l), ldp, u_of(F, l), g, F->coef.as<float>(), g2,
+                         F->gprm.as<float>(), F->gprm16.as<__bf16>(), n, D, Ly.idx, F->K, R,
+                         logq_chk);
+    }
     // MLP backward from the output layer down
     const float* dy32 = F->gprm.as<float>();
     const __bf16* dy16 = F->gprm16.as<__bf16>();
@@ -1110,7 +1276,9 @@ static int flow_backward(gra_handle* h, FlowState* F, int n, float* logq_chk) {
       ep.mask = act16_of(F, l, i - 1);
       ep.ldm = H;
       ep.out16 = F->dact16.as<__bf16>() + (size_t)cur * F->cap * H;
-      ep.out32 = F->dact32.as<float>() + (size_t)cur * F->cap * H;
+      // the FP32 copy is read only by the input-layer dx kernel (BF16 path)
+      ep.out32 = (F->mlp_fp32 || i == 1) ? F->dact32.as<float>() + (size_t)cur * F->cap * H
+                                         : nullptr;
       ep.ldc = H;
       ep.ncols_valid = H;
       if (F->mlp_fp32)
@@ -1129,7 +1297,7 @@ static int flow_backward(gra_handle* h, FlowState* F, int n, float* logq_chk) {
       input_weight_grad(h, F, u_of(F, l), Ly.idx, dy32, H, H, n, grad + Ly.oW1, grad + Ly.ob1);
     else
       input_weight_grad(h, F, u_of(F, l), Ly.idx, dy16, H, H, n, grad + Ly.oW1, grad + Ly.ob1);
-    hipLaunchKernelGGL(k_mlp_in_dx, dim3((unsigned)(((size_t)n * 64 + 255) / 256)), dim3(256), 0,
+    hipLaunchKernelGGL(k_mlp_in_dx, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                        h->stream, dy32, th + Ly.oW1, H, n, D, Ly.idx, g2);
     std::swap(g, g2);
   }
@@ -1315,6 +1483,16 @@ extern "C" int gra_hip_flow_init(gra_handle* h, const gra_flow_cfg* fc_in) {
         hipSuccess) {
       flow_release(F);
       return fail(h, GRA_E_HIP, "flow: cannot reserve %zu B of LDS per block", lds);
+    }
+  }
+  {
+    size_t bl = 0;
+    for (int l = 0; l < F->L; ++l)
+      bl = std::max(bl, sizeof(float) * (size_t)bwd_rows(F, F->layers[l].idx.dB) * (F->ldp + 1));
+    if (hipFuncSetAttribute((const void*)k_rqs_backward_t,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bl) != hipSuccess) {
+      flow_release(F);
+      return fail(h, GRA_E_HIP, "flow: cannot reserve %zu B of LDS for the spline backward", bl);
     }
   }
   h->flow = F;
