@@ -13,6 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgra_hip.so")
 
 GRA_OK = 0
+GRA_E_ARG, GRA_E_HIP, GRA_E_NOMEM, GRA_E_STATE, GRA_E_NUMERIC = -1, -2, -3, -4, -5
 GRA_MAX_DIM = 16
 GRA_VEGAS_BINS = 128
 GRA_PROC_REGGE_CON = 15
@@ -94,6 +95,19 @@ class GraFlowStats(C.Structure):
     ]
 
 
+class GraEvent(C.Structure):
+    _fields_ = [("p", D * 4 * 8), ("f", D), ("index", U64), ("n_p", I32), ("pad0", I32)]
+
+
+class GraRecordPdg(C.Structure):
+    _fields_ = [("beam", I32 * 2), ("central", I32 * 2), ("daughter", I32 * 2 * 2)]
+
+
+class GraGenStats(C.Structure):
+    _fields_ = [("trials", U64), ("valid", U64), ("accepted", U64), ("overflow", U64),
+                ("stored", U64)]
+
+
 # Every function the header declares, with its ctypes signature
 P = C.POINTER
 VP = C.c_void_p
@@ -109,6 +123,8 @@ SIGNATURES = {
     "gra_hip_vegas_iterate": (C.c_int, [VP, P(GraGrid), U64, U64, C.c_uint32, P(GraIterStats)]),
     "gra_hip_vegas_run": (C.c_int, [VP, P(GraVegasCfg), P(GraGrid), P(GraVegasResult)]),
     "gra_hip_vegas_map": (C.c_int, [VP, P(GraGrid), P(D), C.c_size_t, P(D), P(D), P(I32)]),
+    "gra_hip_generate": (C.c_int, [VP, P(GraGrid), U64, U64, C.c_uint32, D, P(GraEvent), U64,
+                                   P(GraGenStats)]),
     "gra_hip_unweight": (C.c_int, [VP, P(D), C.c_size_t, D, U64, P(C.c_uint8), P(U64), P(U64)]),
     "gra_hip_eikonal_tables": (C.c_int, [VP, P(D), P(D), P(D)]),
     "gra_hip_flow_init": (C.c_int, [VP, P(GraFlowCfg)]),
@@ -122,6 +138,9 @@ SIGNATURES = {
     "gra_hip_flow_grad": (C.c_int, [VP, P(C.c_float), P(C.c_float), C.c_size_t, P(C.c_float),
                                     P(C.c_float)]),
     "gra_hip_kernel_time": (C.c_int, [VP, I32, P(D), P(U64), I32]),
+    "gra_hepmc3_open": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, P(VP)]),
+    "gra_hepmc3_write": (C.c_int, [VP, P(GraCfg), P(GraRecordPdg), P(GraEvent), U64, D, D, I32]),
+    "gra_hepmc3_close": (C.c_int, [VP]),
 }
 
 _lib = None

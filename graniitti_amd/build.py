@@ -31,6 +31,7 @@ UNITS = {
     "gra_capi.hip": ["-ffp-contract=off"],
     "gra_flow.hip": ["-ffp-contract=fast"],
     "gra_config.cpp": ["-ffp-contract=off", "-x", "c++"],
+    "gra_hepmc3.cpp": ["-ffp-contract=off", "-x", "c++"],
 }
 
 

@@ -752,6 +752,69 @@ extern "C" int gra_hip_vegas_map(gra_handle* h, const gra_grid* grid, const doub
   return GRA_OK;
 }
 
+extern "C" int gra_hip_generate(gra_handle* h, const gra_grid* grid, uint64_t n_calls,
+                                uint64_t seed, uint32_t iter, double maxf, gra_event* events,
+                                uint64_t max_events, gra_gen_stats* st) {
+  if (!h) return set_global_error(GRA_E_ARG, "handle is NULL");
+  int r = check_grid(h, grid);
+  if (r) return r;
+  if (!st) return fail(h, GRA_E_ARG, "generate: st is NULL");
+  if (max_events > 0 && !events) return fail(h, GRA_E_ARG, "generate: events is NULL");
+  if (!(maxf > 0.0)) return fail(h, GRA_E_ARG, "generate: maxf must be > 0");
+  if (n_calls == 0 || n_calls > 0xFFFFFFFFull)
+    return fail(h, GRA_E_ARG, "generate: n_calls %llu outside [1, 2^32)",
+                (unsigned long long)n_calls);
+  const int D = h->P.dim;
+  r = ensure_point_buffers(h, n_calls);
+  if (r) return r;
+  // every accepted index is kept (at most n_calls), so that the events written
+  // are always the accepted points of smallest index, independent of the schedule
+  const uint64_t cap = std::max<uint64_t>(1, std::min<uint64_t>(max_events, n_calls));
+  DevBuf d_idx, d_cnt, d_ev;
+  if (h->d_grid.ensure(sizeof(double) * GRA_MAX_DIM * GRA_VEGAS_BINS) ||
+      d_idx.ensure(sizeof(uint64_t) * n_calls) || d_cnt.ensure(4 * sizeof(unsigned long long)) ||
+      d_ev.ensure(sizeof(gra_event) * cap))
+    return fail(h, GRA_E_NOMEM, "generation buffers");
+  GRA_HIP_CHECK(h, hipMemcpyAsync(h->d_grid.p, grid->xmat, sizeof(double) * D * GRA_VEGAS_BINS,
+                                  hipMemcpyHostToDevice, h->stream));
+  r = run_integrand(h, h->d_grid.as<double>(), (unsigned int)grid->bins, seed, iter, 0, nullptr,
+                    n_calls, h->d_W.as<double>(), nullptr, h->d_flags.as<uint8_t>(),
+                    h->d_vw.as<double>(), h->d_bins.as<uint8_t>());
+  if (r) return r;
+  GRA_HIP_CHECK(h, hipMemsetAsync(d_cnt.p, 0, 4 * sizeof(unsigned long long), h->stream));
+  GRA_HIP_CHECK(h, launch_accept(h->d_W.as<double>(), h->d_vw.as<double>(),
+                                 h->d_flags.as<uint8_t>(), n_calls, maxf, seed, iter, 0,
+                                 d_idx.as<uint64_t>(), n_calls, d_cnt.as<unsigned long long>(),
+                                 h->stream));
+  unsigned long long c[4] = {0, 0, 0, 0};
+  GRA_HIP_CHECK(h, hipMemcpyAsync(c, d_cnt.p, sizeof(c), hipMemcpyDeviceToHost, h->stream));
+  GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  const uint64_t stored = std::min<uint64_t>(c[2], max_events);
+  std::memset(st, 0, sizeof(*st));
+  st->trials = c[0];
+  st->valid = c[1];
+  st->accepted = c[2];
+  st->overflow = c[3];
+  st->stored = stored;
+  if (stored == 0) return GRA_OK;
+  // the append order depends on the schedule: sort the accepted indices and keep
+  // the first max_events
+  std::vector<uint64_t> idx(c[2]);
+  GRA_HIP_CHECK(h, hipMemcpy(idx.data(), d_idx.p, sizeof(uint64_t) * c[2],
+                             hipMemcpyDeviceToHost));
+  std::sort(idx.begin(), idx.end());
+  GRA_HIP_CHECK(h, hipMemcpy(d_idx.p, idx.data(), sizeof(uint64_t) * stored,
+                             hipMemcpyHostToDevice));
+  GRA_HIP_CHECK(h, k64::launch_event_record(h->P, h->d_grid.as<double>(),
+                                            (unsigned int)grid->bins, seed, iter, 0,
+                                            d_idx.as<uint64_t>(), h->d_W.as<double>(),
+                                            h->d_vw.as<double>(), stored, d_ev.p, h->stream));
+  GRA_HIP_CHECK(h, hipMemcpyAsync(events, d_ev.p, sizeof(gra_event) * stored,
+                                  hipMemcpyDeviceToHost, h->stream));
+  GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  return GRA_OK;
+}
+
 extern "C" int gra_hip_unweight(gra_handle* h, const double* w, size_t n, double wmax,
                                 uint64_t seed, uint8_t* accept, uint64_t* n_accepted,
                                 uint64_t* n_overflow) {

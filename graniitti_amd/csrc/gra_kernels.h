@@ -19,6 +19,10 @@ namespace gra_hip {
   hipError_t launch_screen(const RunParams& P, const void* nodes, const void* rec,                 \
                            const uint32_t* n_cand, uint64_t n_max, double* W, double* amp2,        \
                            uint8_t* flags, hipStream_t st);                                        \
+  hipError_t launch_event_record(const RunParams& P, const double* grid, unsigned int bins,         \
+                                 uint64_t seed, uint32_t iter, uint64_t first, const uint64_t* idx, \
+                                 const double* W, const double* vw, uint64_t n_ev, void* out,       \
+                                 hipStream_t st);                                                  \
   }
 
 GRA_DECLARE_INTEGRAND_LAUNCHERS(k64)
@@ -29,6 +33,9 @@ hipError_t launch_accumulate(const double* W, const double* vw, const uint8_t* b
                              double* part_scal, double* out, hipStream_t st);
 hipError_t launch_map(const double* grid, int D, unsigned int bins, const double* u, uint64_t n,
                       double* x, double* vw, int32_t* b, hipStream_t st);
+hipError_t launch_accept(const double* W, const double* vw, const uint8_t* flags, uint64_t n,
+                         double maxf, uint64_t seed, uint32_t iter, uint64_t first, uint64_t* idx,
+                         uint64_t max_out, unsigned long long* counts, hipStream_t st);
 hipError_t launch_unweight(const double* w, uint64_t n, double wmax, uint64_t seed, uint8_t* acc,
                            unsigned long long* counts, hipStream_t st);
 

@@ -222,9 +222,61 @@ __global__ void k_unweight(const double* __restrict__ w, uint64_t n, double wmax
   }
 }
 
+// Hit-or-miss of the generation pass: f = W vw of valid points against maxf;
+// accepted point indices are appended (wave-aggregated) up to max_out.
+// counts: trials, valid, accepted, overflow.
+__global__ void k_accept(const double* __restrict__ W, const double* __restrict__ vw,
+                         const uint8_t* __restrict__ flags, uint64_t n, double maxf, uint64_t seed,
+                         uint32_t iter, uint64_t first, uint64_t* __restrict__ idx, uint64_t max_out,
+                         unsigned long long* __restrict__ counts) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool valid = false, hit = false, over = false;
+  if (i < n) {
+    const uint8_t fl = flags[i];
+    valid = (fl & (GRA_FLAG_KIN_OK | GRA_FLAG_FID_OK | GRA_FLAG_VETO_OK)) ==
+            (GRA_FLAG_KIN_OK | GRA_FLAG_FID_OK | GRA_FLAG_VETO_OK);
+    const double f = W[i] * vw[i];
+    double u;
+    point_uniforms(seed, 0xC0000000u | (iter & 0x3FFFFFFFu), first + i, 1, &u);
+    hit = u < (f / maxf);
+    if (f > maxf) {
+      over = true;
+      hit = false;
+    }
+    hit = hit && valid;
+  }
+  const uint64_t mt = __ballot(i < n);
+  const uint64_t mv = __ballot(valid), mh = __ballot(hit), mo = __ballot(over);
+  const int lane = threadIdx.x & 63;
+  if (lane == 0) {
+    if (mt) atomicAdd(&counts[0], (unsigned long long)__popcll(mt));
+    if (mv) atomicAdd(&counts[1], (unsigned long long)__popcll(mv));
+    if (mo) atomicAdd(&counts[3], (unsigned long long)__popcll(mo));
+  }
+  if (mh) {
+    const int leader = __ffsll((unsigned long long)mh) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(&counts[2], (unsigned long long)__popcll(mh));
+    base = __shfl(base, leader);
+    if (hit) {
+      const uint64_t pos = base + (uint64_t)__popcll(mh & ((1ull << lane) - 1ull));
+      if (pos < max_out) idx[pos] = i;
+    }
+  }
+}
+
 }  // namespace
 
 namespace gra_hip {
+
+hipError_t launch_accept(const double* W, const double* vw, const uint8_t* flags, uint64_t n,
+                         double maxf, uint64_t seed, uint32_t iter, uint64_t first, uint64_t* idx,
+                         uint64_t max_out, unsigned long long* counts, hipStream_t st) {
+  const unsigned int nb = (unsigned int)((n + 255) / 256);
+  hipLaunchKernelGGL(k_accept, dim3(nb), dim3(256), 0, st, W, vw, flags, n, maxf, seed, iter, first,
+                     idx, max_out, counts);
+  return hipGetLastError();
+}
 
 hipError_t launch_accumulate(const double* W, const double* vw, const uint8_t* bins,
                              const uint8_t* flags, uint64_t n, int D, int nblk, double* part_hist,

@@ -157,3 +157,90 @@ class MGraniitti:
 
     def GetXS(self):
         return self.result.sigma, self.result.sigma_err
+
+    def Generate(self, n_events, calls=1 << 20, seed=12345678, maxf=None, max_passes=64):
+        """Unweighted events on the integrated grid (MGraniitti::Generate with the
+        VEGAS generation mode, src/MGraniitti.cc:1424-1436, 1651-1726). Passes of
+        `calls` points are drawn until n_events are accepted; maxf defaults to the
+        largest f seen during integration. Returns (momenta (n, n_p, 4) as
+        E, px, py, pz in GeV, f (n,), counters summed over the passes)."""
+        if self.result is None:
+            raise RuntimeError("Integrate() before Generate()")
+        wmax = float(maxf if maxf is not None else self.result.maxf)
+        lib, h = self.proc.lib, self.proc.h
+        evs, fs = [], []
+        tot = {"trials": 0, "valid": 0, "accepted": 0, "overflow": 0}
+        for it in range(max_passes):
+            need = n_events - sum(len(e) for e in evs)
+            if need <= 0:
+                break
+            buf = (abi.GraEvent * need)()
+            st = abi.GraGenStats()
+            self.proc._chk(lib.gra_hip_generate(h, C.byref(self.grid), calls, seed, it, wmax, buf,
+                                                need, C.byref(st)))
+            for k in tot:
+                tot[k] += getattr(st, k)
+            if st.stored:
+                a = np.ctypeslib.as_array(buf)[: st.stored]
+                n_p = int(a["n_p"][0])
+                evs.append(np.array(a["p"])[:, :n_p, :])
+                fs.append(np.array(a["f"]))
+        p = np.concatenate(evs) if evs else np.zeros((0, 4, 4))
+        f = np.concatenate(fs) if fs else np.zeros(0)
+        return p, f, tot
+
+
+def record_pdg(cfg):
+    """PDG ids of the event record of the built processes: p p beams; pi+ pi- for
+    PP[CON], rho0 rho0 -> (pi+ pi-)(pi+ pi-) for PP[CONTENSOR]."""
+    ids = abi.GraRecordPdg()
+    ids.beam[0] = ids.beam[1] = 2212
+    if cfg.m_daughter > 0.0:
+        ids.central[0] = ids.central[1] = 113
+        for i in range(2):
+            ids.daughter[i][0], ids.daughter[i][1] = 211, -211
+    else:
+        ids.central[0], ids.central[1] = 211, -211
+    return ids
+
+
+class HepMC3Writer:
+    """HepMC3 ASCII output of generated events (the WriterAscii the reference
+    opens in MGraniitti::InitFileOutput, src/MGraniitti.cc:190-207, written by
+    SaveEvent, 1665-1712, with the record of MProcess::CommonRecord,
+    src/MProcess.cc:1588-1706). Host-only calls into libgra_hip.so."""
+
+    def __init__(self, path, cfg, generator="GRANIITTI (TUNE0)", version="1.0", card=None,
+                 pdg=None):
+        self.lib = abi.load()
+        self.cfg = cfg
+        self.pdg = pdg if pdg is not None else record_pdg(cfg)
+        w = C.c_void_p()
+        abi.check(self.lib.gra_hepmc3_open(str(path).encode(), generator.encode(),
+                                           version.encode(),
+                                           card.encode() if card else None, C.byref(w)))
+        self.w = w
+
+    def write(self, p, f, sigma, sigma_err, weighted=False):
+        """Append events: p (n, n_p, 4) as (E, px, py, pz) in the gra_event order,
+        f (n,) the VEGAS-weighted integrand; sigma and sigma_err in barn."""
+        p = np.asarray(p, dtype=np.float64)
+        n, n_p = p.shape[0], p.shape[1]
+        buf = (abi.GraEvent * max(n, 1))()
+        a = np.ctypeslib.as_array(buf)
+        a["p"][:n, :n_p, :] = p
+        a["f"][:n] = np.asarray(f, dtype=np.float64)
+        a["n_p"][:n] = n_p
+        abi.check(self.lib.gra_hepmc3_write(self.w, C.byref(self.cfg), C.byref(self.pdg), buf, n,
+                                            float(sigma), float(sigma_err), int(weighted)))
+
+    def close(self):
+        if getattr(self, "w", None):
+            w, self.w = self.w, None
+            abi.check(self.lib.gra_hepmc3_close(w))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

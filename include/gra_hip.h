@@ -197,6 +197,29 @@ typedef struct gra_flow_stats {
   int32_t pad0;
 } gra_flow_stats;
 
+/* One unweighted event: the final-state 4-momenta (E, px, py, pz) in GeV as
+ * MProcess::EventRecord lists them (src/MProcess.cc:1287-1314): p[0], p[1] the
+ * forward protons, p[2], p[3] the central particles, p[4..7] their two-body
+ * decay daughters (p[4], p[5] from p[2]; p[6], p[7] from p[3]) when the central
+ * particles decay. n_p = 4 or 8. f is the point's VEGAS-weighted integrand
+ * (barn), index its Philox point index within the iteration. */
+typedef struct {
+  double p[8][4];
+  double f;
+  uint64_t index;
+  int32_t n_p;
+  int32_t pad0;
+} gra_event;
+
+/* Counters of one generation pass (MGraniitti::Stats, include/Graniitti/MGraniitti.h:46-93) */
+typedef struct {
+  uint64_t trials;    /* points sampled */
+  uint64_t valid;     /* AuxIntData::Valid() points */
+  uint64_t accepted;  /* hit-or-miss hits among valid points */
+  uint64_t overflow;  /* points with f > maxf (never accepted) */
+  uint64_t stored;    /* events written (min(accepted, max_events)) */
+} gra_gen_stats;
+
 /* ------------------------------------------------------------------------- */
 /* Entry points                                                              */
 /* ------------------------------------------------------------------------- */
@@ -256,11 +279,51 @@ int gra_hip_vegas_run(gra_handle* h, const gra_vegas_cfg* vc, gra_grid* grid,
 int gra_hip_vegas_map(gra_handle* h, const gra_grid* grid, const double* u, size_t n, double* x,
                       double* vw, int32_t* bins);
 
+/* Unweighted event generation (generation mode of MGraniitti::VEGASMultiThread
+ * with MGraniitti::SaveEvent, src/MGraniitti.cc:1424-1436, 1651-1726): n_calls
+ * points of iteration iter are drawn through the grid and weighted; a valid
+ * point is accepted when U < f / maxf (U from Philox, its own counter stream)
+ * and never when f > maxf (counted as overflow). The first max_events accepted
+ * points, in increasing point index, are written to events (host memory). */
+int gra_hip_generate(gra_handle* h, const gra_grid* grid, uint64_t n_calls, uint64_t seed,
+                     uint32_t iter, double maxf, gra_event* events, uint64_t max_events,
+                     gra_gen_stats* st);
+
 /* Hit-or-miss unweighting of n weights against wmax (MGraniitti::SaveEvent,
  * src/MGraniitti.cc:1651-1663): accept[i] = U_i < w_i / wmax with U from
  * Philox keyed by (seed, i). Returns accepted count and overflow count. */
 int gra_hip_unweight(gra_handle* h, const double* w, size_t n, double wmax, uint64_t seed,
                      uint8_t* accept, uint64_t* n_accepted, uint64_t* n_overflow);
+
+/* HepMC3 ASCII event output (host only, no GPU). Replaces the HepMC3 3.2
+ * WriterAscii the reference opens in MGraniitti::InitFileOutput
+ * (src/MGraniitti.cc:190-207) and fills in SaveEvent (1665-1712), with the event
+ * record of MProcess::CommonRecord (src/MProcess.cc:1588-1706) and
+ * WriteDecayKinematics (1287-1314): beams, forward protons, the two generic
+ * propagators (pdg 99), the central system (pdg 90), the central particles and
+ * their decay daughters. */
+typedef struct gra_hepmc3 gra_hepmc3;
+
+/* PDG ids of an event record */
+typedef struct {
+  int32_t beam[2];        /* beam 1 (+z), beam 2 (-z); the forward protons keep these ids */
+  int32_t central[2];     /* central particles in decaytree order (gra_event p[2], p[3]) */
+  int32_t daughter[2][2]; /* daughters of central[0] (p[4], p[5]) and central[1] (p[6], p[7]) */
+} gra_record_pdg;
+
+/* Open path for writing and write the header: format version and the run's
+ * tools (generator name and version, steering card; card may be NULL). */
+int gra_hepmc3_open(const char* path, const char* generator, const char* version,
+                    const char* card, gra_hepmc3** out);
+/* Append n events. sigma, sigma_err in barn are written in picobarn as each
+ * event's GenCrossSection; the event weight is 1 (unweighted) or, when weighted
+ * is nonzero, the event's f. Beam 4-momenta come from cfg (beam_E1, beam_E2,
+ * m_beam). Events are numbered consecutively from 0 across calls. */
+int gra_hepmc3_write(gra_hepmc3* w, const gra_cfg* cfg, const gra_record_pdg* pdg,
+                     const gra_event* events, uint64_t n, double sigma, double sigma_err,
+                     int32_t weighted);
+/* Write the footer, close the file and free the writer. */
+int gra_hepmc3_close(gra_hepmc3* w);
 
 /* Copy the screening-node eikonal amplitudes A_eik(kt_j^2) (complex, interleaved
  * re,im; n_loop_kt+1 values) and the density / amplitude tables (MBT, MSA of
