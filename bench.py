@@ -193,7 +193,7 @@ def main():
     ap.add_argument("--preset", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-precision-run", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=1 << 17)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 20)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-flow", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",

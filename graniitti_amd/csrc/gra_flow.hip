@@ -304,11 +304,14 @@ __global__ __launch_bounds__(256) void k_rqs_backward(
 // G_m = G_X (m < k), G_w (m = k), 0 (m > k) is
 //   G_X (X_k - k eps) / (1 - K eps) + G_w sw_k,
 // so no per-bin arrays are kept.
+template <int KT>
 __global__ __launch_bounds__(256) void k_rqs_backward_t(
     const float* __restrict__ prm, int ldp, const float* __restrict__ u_prev,
     const float* __restrict__ g_in, const float* __restrict__ coef, float* __restrict__ g_out,
-    float* __restrict__ gp, __bf16* __restrict__ gp16, int n, int D, LayerIdx L, int K, int R,
+    float* __restrict__ gp, __bf16* __restrict__ gp16, int n, int D, LayerIdx L, int Kin, int R,
     float* __restrict__ logq_chk) {
+  // KT > 0: compile-time K, logits and their exponentials held in registers
+  const int K = KT > 0 ? KT : Kin;
   extern __shared__ float tile[];  // [R][ldp + 1]
   __shared__ float sLj[256];
   const int ldt = ldp + 1;
@@ -343,27 +346,74 @@ __global__ __launch_bounds__(256) void k_rqs_backward_t(
       for (int i = 0; i < L.dA; ++i) go[L.A[i]] = gi[L.A[i]];
     const float gld = -coef[b];
     float* p = tile + r * ldt + j * np;
-    float mw = p[0], mh = p[K];
-    for (int m = 1; m < K; ++m) {
-      mw = fmaxf(mw, p[m]);
-      mh = fmaxf(mh, p[K + m]);
-    }
-    float sw = 0.f, sh = 0.f;
-    for (int m = 0; m < K; ++m) {
-      sw += __expf(p[m] - mw);
-      sh += __expf(p[K + m] - mh);
-    }
-    const float iw = 1.f / sw, ih = 1.f / sh;
     const float x = fminf(fmaxf(up[L.B[j]], 0.f), 1.f);
+    float ew[KT > 0 ? KT : 1], eh[KT > 0 ? KT : 1];
+    float mw, mh, iw, ih;
     float X = 0.f, Y = 0.f, swk = 0.f, shk = 0.f;
     int k = 0;
-    for (; k < K; ++k) {
-      swk = __expf(p[k] - mw) * iw;
-      shk = __expf(p[K + k] - mh) * ih;
-      const float w = kEpsW + cw * swk;
-      if (k == K - 1 || X + w > x) break;
-      X += w;
-      Y += kEpsW + cw * shk;
+    if constexpr (KT > 0) {
+#pragma unroll
+      for (int m = 0; m < KT; ++m) {
+        ew[m] = p[m];
+        eh[m] = p[KT + m];
+      }
+      mw = ew[0];
+      mh = eh[0];
+#pragma unroll
+      for (int m = 1; m < KT; ++m) {
+        mw = fmaxf(mw, ew[m]);
+        mh = fmaxf(mh, eh[m]);
+      }
+      float sw = 0.f, sh = 0.f;
+#pragma unroll
+      for (int m = 0; m < KT; ++m) {
+        ew[m] = __expf(ew[m] - mw);
+        eh[m] = __expf(eh[m] - mh);
+        sw += ew[m];
+        sh += eh[m];
+      }
+      iw = 1.f / sw;
+      ih = 1.f / sh;
+      float Xa = 0.f, Ya = 0.f;
+      bool found = false;
+#pragma unroll
+      for (int m = 0; m < KT; ++m) {
+        const float swm = ew[m] * iw, shm = eh[m] * ih;
+        const float w = kEpsW + cw * swm;
+        const bool hit = !found && (m == KT - 1 || Xa + w > x);
+        if (hit) {
+          k = m;
+          X = Xa;
+          Y = Ya;
+          swk = swm;
+          shk = shm;
+        }
+        found = found || hit;
+        Xa += w;
+        Ya += kEpsW + cw * shm;
+      }
+    } else {
+      mw = p[0];
+      mh = p[K];
+      for (int m = 1; m < K; ++m) {
+        mw = fmaxf(mw, p[m]);
+        mh = fmaxf(mh, p[K + m]);
+      }
+      float sw = 0.f, sh = 0.f;
+      for (int m = 0; m < K; ++m) {
+        sw += __expf(p[m] - mw);
+        sh += __expf(p[K + m] - mh);
+      }
+      iw = 1.f / sw;
+      ih = 1.f / sh;
+      for (; k < K; ++k) {
+        swk = __expf(p[k] - mw) * iw;
+        shk = __expf(p[K + k] - mh) * ih;
+        const float w = kEpsW + cw * swk;
+        if (k == K - 1 || X + w > x) break;
+        X += w;
+        Y += kEpsW + cw * shk;
+      }
     }
     Bin e;
     e.k = k;
@@ -404,9 +454,16 @@ __global__ __launch_bounds__(256) void k_rqs_backward_t(
     const float icw = 1.f / cw;
     const float sdw = G_X * (X - k * kEpsW) * icw + G_w * swk;
     const float sdh = G_Y * (Y - k * kEpsW) * icw + G_h * shk;
+#pragma unroll
     for (int m = 0; m < K; ++m) {
-      const float swm = __expf(p[m] - mw) * iw;
-      const float shm = __expf(p[K + m] - mh) * ih;
+      float swm, shm;
+      if constexpr (KT > 0) {
+        swm = ew[m] * iw;
+        shm = eh[m] * ih;
+      } else {
+        swm = __expf(p[m] - mw) * iw;
+        shm = __expf(p[K + m] - mh) * ih;
+      }
       const float Gwm = (m < k) ? G_X : (m == k ? G_w : 0.f);
       const float Ghm = (m < k) ? G_Y : (m == k ? G_h : 0.f);
       p[m] = cw * swm * (Gwm - sdw);
@@ -836,12 +893,92 @@ __device__ __forceinline__ void fused_product(const __bf16* sA, int lda, const _
   }
 }
 
-template <int H>
+// Bin of the inverse spline (sampling direction) for one (point, dimension):
+// the softmax of the width and height logits p[0..K), p[K..2K), and the first
+// bin k whose cumulative height passes z (the last bin otherwise). KT > 0 is a
+// compile-time K: the logits are read once into registers and the loops are
+// unrolled; KT = 0 loops over a runtime K. Both orders of operations are the
+// same, so both give the same bin and values.
+template <int KT>
+__device__ __forceinline__ void inverse_bin(const float* p, int K, float zv, Bin& e) {
+  float X = 0.f, Y = 0.f, w = 0.f, h = 0.f;
+  int k = 0;
+  if constexpr (KT > 0) {
+    float ew[KT], eh[KT];
+#pragma unroll
+    for (int m = 0; m < KT; ++m) {
+      ew[m] = p[m];
+      eh[m] = p[KT + m];
+    }
+    float mw = ew[0], mh = eh[0];
+#pragma unroll
+    for (int m = 1; m < KT; ++m) {
+      mw = fmaxf(mw, ew[m]);
+      mh = fmaxf(mh, eh[m]);
+    }
+    float sw = 0.f, sh = 0.f;
+#pragma unroll
+    for (int m = 0; m < KT; ++m) {
+      ew[m] = __expf(ew[m] - mw);
+      eh[m] = __expf(eh[m] - mh);
+      sw += ew[m];
+      sh += eh[m];
+    }
+    const float cw = 1.f - KT * kEpsW;
+    const float iw = cw / sw, ih = cw / sh;
+    float Xa = 0.f, Ya = 0.f;
+    bool found = false;
+#pragma unroll
+    for (int m = 0; m < KT; ++m) {
+      const float wm = kEpsW + ew[m] * iw;
+      const float hm = kEpsW + eh[m] * ih;
+      const bool hit = !found && (m == KT - 1 || Ya + hm > zv);
+      if (hit) {
+        k = m;
+        X = Xa;
+        Y = Ya;
+        w = wm;
+        h = hm;
+      }
+      found = found || hit;
+      Xa += wm;
+      Ya += hm;
+    }
+  } else {
+    float mw = p[0], mh = p[K];
+    for (int m = 1; m < K; ++m) {
+      mw = fmaxf(mw, p[m]);
+      mh = fmaxf(mh, p[K + m]);
+    }
+    float sw = 0.f, sh = 0.f;
+    for (int m = 0; m < K; ++m) {
+      sw += __expf(p[m] - mw);
+      sh += __expf(p[K + m] - mh);
+    }
+    const float cw = 1.f - K * kEpsW;
+    const float iw = cw / sw, ih = cw / sh;
+    for (; k < K; ++k) {
+      w = kEpsW + __expf(p[k] - mw) * iw;
+      h = kEpsW + __expf(p[K + k] - mh) * ih;
+      if (k == K - 1 || Y + h > zv) break;
+      X += w;
+      Y += h;
+    }
+  }
+  e.k = k;
+  e.X = X;
+  e.w = w;
+  e.Y = Y;
+  e.h = h;
+}
+
+template <int H, int KT>
 __global__ __launch_bounds__(256) void k_flow_layer_inverse(const float* __restrict__ u_in,
                                                             float* __restrict__ u_out,
                                                             float* __restrict__ ld, int n, int D,
-                                                            LayerIdx L, int K, int depth, int ldp,
+                                                            LayerIdx L, int Kin, int depth, int ldp,
                                                             FusedLayer W) {
+  const int K = KT > 0 ? KT : Kin;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   constexpr int lda = H + 8;  // BF16 row stride (16-byte multiple, staggers banks)
   const int ldpp = ldp + 1;
@@ -897,34 +1034,10 @@ __global__ __launch_bounds__(256) void k_flow_layer_inverse(const float* __restr
     if (j == 0)
       for (int i = 0; i < L.dA; ++i) u_out[(size_t)row * D + L.A[i]] = sU[r * D + L.A[i]];
     const float* p = sP + r * ldpp + j * np;
-    float mw = p[0], mh = p[K];
-    for (int k = 1; k < K; ++k) {
-      mw = fmaxf(mw, p[k]);
-      mh = fmaxf(mh, p[K + k]);
-    }
-    float sw = 0.f, sh = 0.f;
-    for (int k = 0; k < K; ++k) {
-      sw += __expf(p[k] - mw);
-      sh += __expf(p[K + k] - mh);
-    }
-    const float cw = 1.f - K * kEpsW;
-    const float iw = cw / sw, ih = cw / sh;
     const float zv = fminf(fmaxf(sU[r * D + L.B[j]], 0.f), 1.f);
-    float X = 0.f, Y = 0.f, w = 0.f, h = 0.f;
-    int k = 0;
-    for (; k < K; ++k) {
-      w = kEpsW + __expf(p[k] - mw) * iw;
-      h = kEpsW + __expf(p[K + k] - mh) * ih;
-      if (k == K - 1 || Y + h > zv) break;
-      X += w;
-      Y += h;
-    }
     Bin e;
-    e.k = k;
-    e.X = X;
-    e.w = w;
-    e.Y = Y;
-    e.h = h;
+    inverse_bin<KT>(p, K, zv, e);
+    const int k = e.k;
     e.dk = (k == 0) ? 1.f : kEpsD + softplusf(p[2 * K + k - 1]);
     e.dk1 = (k == K - 1) ? 1.f : kEpsD + softplusf(p[2 * K + k]);
     e.s = e.h / e.w;
@@ -1170,6 +1283,35 @@ static int bwd_rows(const FlowState* F, int dB) {
   return std::max(1, std::min(256 / dB, by_lds));
 }
 
+// Kernel instances of the fused layer by (hidden, bins) and of the spline
+// backward by bins: 8, 16 and 32 bins run the unrolled register forms, other
+// bin counts the runtime loops
+template <int H>
+static const void* layer_inverse_kernel_h(int K) {
+  switch (K) {
+    case 8: return (const void*)k_flow_layer_inverse<H, 8>;
+    case 16: return (const void*)k_flow_layer_inverse<H, 16>;
+    case 32: return (const void*)k_flow_layer_inverse<H, 32>;
+    default: return (const void*)k_flow_layer_inverse<H, 0>;
+  }
+}
+static const void* layer_inverse_kernel(int H, int K) {
+  return H == 64 ? layer_inverse_kernel_h<64>(K)
+                 : (H == 128 ? layer_inverse_kernel_h<128>(K) : layer_inverse_kernel_h<256>(K));
+}
+static const void* spline_backward_kernel(int K) {
+  switch (K) {
+    case 8: return (const void*)k_rqs_backward_t<8>;
+    case 16: return (const void*)k_rqs_backward_t<16>;
+    case 32: return (const void*)k_rqs_backward_t<32>;
+    default: return (const void*)k_rqs_backward_t<0>;
+  }
+}
+
+// Bin count that selects the kernel instance: 0 (the runtime-K loops) when the
+// configuration asks for them
+static int spline_k(const FlowState* F) { return F->fc.spline_loops ? 0 : F->K; }
+
 static size_t fused_lds_bytes(const FlowState* F) {
   return (size_t)2 * kRows * (F->H + 8) * sizeof(__bf16) + (size_t)kRows * (F->ldp + 1) * 4 +
          (size_t)kRows * 8 * 4 + (size_t)kRows * F->D * 4;
@@ -1209,15 +1351,12 @@ static int flow_sample_pass(gra_handle* h, FlowState* F, int n, uint64_t seed, u
       const float* uin = u_of(F, l + 1);
       float* uout = u_of(F, l);
       float* ldp_ = F->ld.as<float>();
-      if (F->H == 64)
-        hipLaunchKernelGGL(k_flow_layer_inverse<64>, grd, dim3(256), lds, h->stream, uin, uout,
-                           ldp_, n, D, Ly.idx, F->K, F->depth, F->ldp, W);
-      else if (F->H == 128)
-        hipLaunchKernelGGL(k_flow_layer_inverse<128>, grd, dim3(256), lds, h->stream, uin, uout,
-                           ldp_, n, D, Ly.idx, F->K, F->depth, F->ldp, W);
-      else
-        hipLaunchKernelGGL(k_flow_layer_inverse<256>, grd, dim3(256), lds, h->stream, uin, uout,
-                           ldp_, n, D, Ly.idx, F->K, F->depth, F->ldp, W);
+      LayerIdx idx = Ly.idx;
+      int nn = n, DD = D, KK = F->K, dep = F->depth, ldp = F->ldp;
+      void* args[] = {(void*)&uin, (void*)&uout, (void*)&ldp_, (void*)&nn, (void*)&DD,
+                      (void*)&idx,  (void*)&KK,   (void*)&dep,  (void*)&ldp, (void*)&W};
+      GRA_HIP_CHECK(h, hipLaunchKernel(layer_inverse_kernel(F->H, spline_k(F)), grd, dim3(256),
+                                       args, lds, h->stream));
     }
     GRA_HIP_CHECK(h, hipGetLastError());
     return GRA_OK;
@@ -1250,10 +1389,20 @@ static int flow_backward(gra_handle* h, FlowState* F, int n, float* logq_chk) {
     {
       const int R = bwd_rows(F, Ly.idx.dB);
       const size_t lds = sizeof(float) * (size_t)R * (ldp + 1);
-      hipLaunchKernelGGL(k_rqs_backward_t, dim3((n + R - 1) / R), dim3(256), lds, h->stream,
-                         prm_of(F, l), ldp, u_of(F, l), g, F->coef.as<float>(), g2,
-                         F->gprm.as<float>(), F->gprm16.as<__bf16>(), n, D, Ly.idx, F->K, R,
-                         logq_chk);
+      const float* a_prm = prm_of(F, l);
+      const float* a_up = u_of(F, l);
+      const float* a_gin = g;
+      const float* a_coef = F->coef.as<float>();
+      float* a_gp = F->gprm.as<float>();
+      __bf16* a_gp16 = F->gprm16.as<__bf16>();
+      LayerIdx idx = Ly.idx;
+      int a_ldp = ldp, nn = n, DD = D, KK = F->K, RR = R;
+      void* args[] = {(void*)&a_prm, (void*)&a_ldp, (void*)&a_up,   (void*)&a_gin,
+                      (void*)&a_coef, (void*)&g2,   (void*)&a_gp,   (void*)&a_gp16,
+                      (void*)&nn,     (void*)&DD,   (void*)&idx,    (void*)&KK,
+                      (void*)&RR,     (void*)&logq_chk};
+      GRA_HIP_CHECK(h, hipLaunchKernel(spline_backward_kernel(spline_k(F)), dim3((n + R - 1) / R),
+                                       dim3(256), args, lds, h->stream));
     }
     // MLP backward from the output layer down
     const float* dy32 = F->gprm.as<float>();
@@ -1476,9 +1625,7 @@ extern "C" int gra_hip_flow_init(gra_handle* h, const gra_flow_cfg* fc_in) {
       flow_release(F);
       return fail(h, GRA_E_ARG, "flow hidden %d: the BF16 path supports 64, 128, 256", fc.hidden);
     }
-    const void* kern = fc.hidden == 64 ? (const void*)k_flow_layer_inverse<64>
-                       : fc.hidden == 128 ? (const void*)k_flow_layer_inverse<128>
-                                          : (const void*)k_flow_layer_inverse<256>;
+    const void* kern = layer_inverse_kernel(fc.hidden, spline_k(F));
     if (hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
         hipSuccess) {
       flow_release(F);
@@ -1489,7 +1636,7 @@ extern "C" int gra_hip_flow_init(gra_handle* h, const gra_flow_cfg* fc_in) {
     size_t bl = 0;
     for (int l = 0; l < F->L; ++l)
       bl = std::max(bl, sizeof(float) * (size_t)bwd_rows(F, F->layers[l].idx.dB) * (F->ldp + 1));
-    if (hipFuncSetAttribute((const void*)k_rqs_backward_t,
+    if (hipFuncSetAttribute(spline_backward_kernel(spline_k(F)),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bl) != hipSuccess) {
       flow_release(F);
       return fail(h, GRA_E_HIP, "flow: cannot reserve %zu B of LDS for the spline backward", bl);

@@ -183,6 +183,9 @@ typedef struct gra_flow_cfg {
   double beta1, beta2, eps;
   uint64_t seed;
   double ess_fallback; /* ESS fraction below which sampling falls back to VEGAS */
+  int32_t spline_loops; /* nonzero: spline kernels loop over a runtime bin count for every K
+                           (0: unrolled kernels for K = 8, 16, 32; used to check them) */
+  int32_t pad0;
 } gra_flow_cfg;
 
 typedef struct gra_flow_stats {

@@ -82,14 +82,24 @@ def proc():
     P.close()
 
 
-@pytest.mark.parametrize("fp32,tol", [(True, 2e-4), (False, 6e-2)])
-def test_flow_forward_matches_reference(proc, fp32, tol):
-    D, L, K, H, depth = 8, 4, 8, 64, 2
-    say(f"case: flow forward fp32={fp32} D={D} L={L} K={K} H={H}")
+# K = 8 runs the unrolled spline kernels, K = 10 the runtime-K loops; the BF16
+# cases run the fused sampling layer
+@pytest.mark.parametrize("fp32,tol,D,L,K,H,depth", [
+    (True, 2e-4, 8, 4, 8, 64, 2),
+    (False, 6e-2, 8, 4, 8, 64, 2),
+    (False, 6e-2, 8, 4, 10, 64, 2),
+])
+def test_flow_forward_matches_reference(proc, fp32, tol, D, L, K, H, depth):
+    say(f"case: flow forward fp32={fp32} D={D} L={L} K={K} H={H} depth={depth}")
     fc = flow_cfg(D, L, K, H, depth, fp32)
-    th = init_random(proc, fc, 0.05, 11)
-    z = np.random.default_rng(1).random((256, D)).astype(np.float32)
-    u, lq = forward(proc, z)
+    P = proc if D == proc.cfg.dim else MContinuum(rho_cfg())
+    try:
+        th = init_random(P, fc, 0.05, 11)
+        z = np.random.default_rng(1).random((256, D)).astype(np.float32)
+        u, lq = forward(P, z)
+    finally:
+        if P is not proc:
+            P.close()
     ref = FlowRef(th, D, L, K, H, depth)
     import torch
     with torch.no_grad():
@@ -101,9 +111,10 @@ def test_flow_forward_matches_reference(proc, fp32, tol):
     assert np.all((u >= 0) & (u < 1))
 
 
-def test_flow_gradient_matches_reference(proc):
-    D, L, K, H, depth = 8, 4, 8, 64, 2
-    say("case: flow gradient fp32 vs torch autograd")
+@pytest.mark.parametrize("K", [8, 10, 16])
+def test_flow_gradient_matches_reference(proc, K):
+    D, L, H, depth = 8, 4, 64, 2
+    say(f"case: flow gradient fp32 vs torch autograd, K={K}")
     fc = flow_cfg(D, L, K, H, depth, True)
     th = init_random(proc, fc, 0.05, 12)
     rng = np.random.default_rng(2)
@@ -132,6 +143,41 @@ def test_flow_gradient_bf16_direction(proc):
     cos = float(g @ gr / (np.linalg.norm(g) * np.linalg.norm(gr)))
     say(f"flow grad bf16: cosine {cos:.5f}")
     assert cos > 0.995
+
+
+@pytest.mark.parametrize("D,L,K,H,depth", [(8, 6, 16, 128, 2), (14, 4, 32, 256, 3)])
+def test_unrolled_splines_match_runtime_loops(proc, D, L, K, H, depth):
+    """The unrolled spline kernels (K = 16, 32) against the runtime-K loops, which
+    the K = 10 cases check against the float64 reference: fused BF16 sampling
+    layer (u, log q) and spline backward (FP32 gradient) on the same parameters."""
+    say(f"case: unrolled vs runtime-K splines D={D} L={L} K={K} H={H} depth={depth}")
+    P = proc if D == proc.cfg.dim else MContinuum(rho_cfg())
+    rng = np.random.default_rng(5)
+    z = rng.random((1024, D)).astype(np.float32)
+    uu = rng.random((1024, D)).astype(np.float32)
+    c = rng.random(1024).astype(np.float32)
+    out = []
+    try:
+        for loops in (0, 1):
+            fc = flow_cfg(D, L, K, H, depth, False)
+            fc.spline_loops = loops
+            init_random(P, fc, 0.05, 21)
+            u, lq = forward(P, z)
+            fc = flow_cfg(D, L, K, H, depth, True)
+            fc.spline_loops = loops
+            th = init_random(P, fc, 0.05, 22)
+            g, lqg = grad(P, uu, c, len(th))
+            out.append((u, lq, g, lqg))
+    finally:
+        if P is not proc:
+            P.close()
+    (u0, lq0, g0, lg0), (u1, lq1, g1, lg1) = out
+    du, dl = np.abs(u0 - u1).max(), np.abs(lq0 - lq1).max()
+    dg = np.abs(g0 - g1).max() / np.abs(g1).max()
+    exact = np.array_equal(u0, u1) and np.array_equal(lq0, lq1) and np.array_equal(g0, g1)
+    say(f"unrolled vs loops: max |du| {du:.2e}, |dlogq| {dl:.2e}, grad {dg:.2e}, bit-exact {exact}")
+    assert du <= 1e-6 and dl <= 1e-5 and dg <= 1e-6
+    assert np.abs(lg0 - lg1).max() <= 1e-5
 
 
 def rho_cfg():
