@@ -6,12 +6,15 @@
 
 namespace {
 
-// Deterministic accumulation of one iteration. Each block (one wave) owns a
-// contiguous range of points. Lane l owns histogram bins l and l + 64 of every
-// dimension and keeps them in registers; the wave walks its non-zero points in
-// point order and broadcasts each point's (f, bins) with v_readlane, so every
-// bin is summed in the same order on every run without atomics.
+// Deterministic accumulation of one iteration. Each block owns a contiguous
+// range of points and has kAccWaves waves that walk the same points; wave w
+// owns the dimensions w, w + kAccWaves, ... Lane l of a wave owns histogram
+// bins l and l + 64 of its dimensions and keeps them in registers; the wave
+// walks the non-zero points in point order and broadcasts each point's (f, bin)
+// with v_readlane, so every bin is summed by one lane in point order, the same
+// on every run, without atomics. Wave 0 also sums the scalars.
 constexpr int kAccWave = 64;
+constexpr int kAccWaves = 4;
 
 __device__ __forceinline__ double readlane_d(double v, int src) {
   const uint64_t u = __double_as_longlong(v);
@@ -21,17 +24,16 @@ __device__ __forceinline__ double readlane_d(double v, int src) {
 }
 
 template <int DT>
-__global__ __launch_bounds__(kAccWave) void k_accumulate(const double* __restrict__ W,
-                                                         const double* __restrict__ vw,
-                                                         const uint8_t* __restrict__ bins,
-                                                         const uint8_t* __restrict__ flags,
-                                                         uint64_t n, uint64_t chunk, int D,
-                                                         double* __restrict__ part_hist,
-                                                         double* __restrict__ part_scal) {
-  const int lane = threadIdx.x;
-  double hlo[DT], hhi[DT], h2lo[DT], h2hi[DT];
+__global__ __launch_bounds__(kAccWave* kAccWaves) void k_accumulate(
+    const double* __restrict__ W, const double* __restrict__ vw, const uint8_t* __restrict__ bins,
+    const uint8_t* __restrict__ flags, uint64_t n, uint64_t chunk, int D,
+    double* __restrict__ part_hist, double* __restrict__ part_scal) {
+  constexpr int DPW = (DT + kAccWaves - 1) / kAccWaves;  // dimensions per wave
+  const int lane = threadIdx.x & (kAccWave - 1);
+  const int wave = threadIdx.x / kAccWave;  // wave-uniform
+  double hlo[DPW], hhi[DPW], h2lo[DPW], h2hi[DPW];
 #pragma unroll
-  for (int j = 0; j < DT; ++j) hlo[j] = hhi[j] = h2lo[j] = h2hi[j] = 0.0;
+  for (int q = 0; q < DPW; ++q) hlo[q] = hhi[q] = h2lo[q] = h2hi[q] = 0.0;
   const uint64_t p0 = (uint64_t)blockIdx.x * chunk;
   uint64_t p1 = p0 + chunk;
   if (p1 > n) p1 = n;
@@ -40,30 +42,34 @@ __global__ __launch_bounds__(kAccWave) void k_accumulate(const double* __restric
   for (uint64_t base = p0; base < p1; base += kAccWave) {
     const uint64_t i = base + lane;
     double f = 0.0;
-    int bb[DT];
+    int bb[DPW];
 #pragma unroll
-    for (int j = 0; j < DT; ++j) bb[j] = 0;
+    for (int q = 0; q < DPW; ++q) bb[q] = 0;
     if (i < p1) {
       const double Wi = W[i];
       f = Wi * vw[i];
-      const double f2 = f * f;
-      double y = f - fc, t = fs + y;
-      fc = (t - fs) - y;
-      fs = t;
-      y = f2 - f2c;
-      t = f2s + y;
-      f2c = (t - f2s) - y;
-      f2s = t;
-      mW = Wi > mW ? Wi : mW;
-      mf = f > mf ? f : mf;
-      const uint8_t fl = flags[i];
-      c_kin += (fl & GRA_FLAG_KIN_OK) ? 1 : 0;
-      c_fid += (fl & GRA_FLAG_FID_OK) ? 1 : 0;
-      c_val += ((fl & GRA_FLAG_KIN_OK) && (fl & GRA_FLAG_FID_OK)) ? 1 : 0;
-      c_nan += (fl & GRA_FLAG_AMP_OK) ? 0 : 1;
+      if (wave == 0) {
+        const double f2 = f * f;
+        double y = f - fc, t = fs + y;
+        fc = (t - fs) - y;
+        fs = t;
+        y = f2 - f2c;
+        t = f2s + y;
+        f2c = (t - f2s) - y;
+        f2s = t;
+        mW = Wi > mW ? Wi : mW;
+        mf = f > mf ? f : mf;
+        const uint8_t fl = flags[i];
+        c_kin += (fl & GRA_FLAG_KIN_OK) ? 1 : 0;
+        c_fid += (fl & GRA_FLAG_FID_OK) ? 1 : 0;
+        c_val += ((fl & GRA_FLAG_KIN_OK) && (fl & GRA_FLAG_FID_OK)) ? 1 : 0;
+        c_nan += (fl & GRA_FLAG_AMP_OK) ? 0 : 1;
+      }
 #pragma unroll
-      for (int j = 0; j < DT; ++j)
-        if (j < D) bb[j] = bins[i * D + j];
+      for (int q = 0; q < DPW; ++q) {
+        const int j = wave + q * kAccWaves;
+        if (j < D) bb[q] = bins[i * D + j];
+      }
     }
     uint64_t m = __ballot(f != 0.0);
     while (m) {
@@ -72,28 +78,30 @@ __global__ __launch_bounds__(kAccWave) void k_accumulate(const double* __restric
       const double fv = readlane_d(f, src);
       const double fv2 = fv * fv;
 #pragma unroll
-      for (int j = 0; j < DT; ++j) {
-        const int b = __builtin_amdgcn_readlane(bb[j], src);
+      for (int q = 0; q < DPW; ++q) {
+        const int b = __builtin_amdgcn_readlane(bb[q], src);
         // f >= 0, so adding +0.0 to the bins a lane does not own leaves them unchanged
         const bool lo = b == lane, hi = b == lane + 64;
-        hlo[j] += lo ? fv : 0.0;
-        h2lo[j] += lo ? fv2 : 0.0;
-        hhi[j] += hi ? fv : 0.0;
-        h2hi[j] += hi ? fv2 : 0.0;
+        hlo[q] += lo ? fv : 0.0;
+        h2lo[q] += lo ? fv2 : 0.0;
+        hhi[q] += hi ? fv : 0.0;
+        h2hi[q] += hi ? fv2 : 0.0;
       }
     }
   }
   const uint64_t H = 2 * (uint64_t)D * GRA_VEGAS_BINS;
   double* ph = part_hist + (uint64_t)blockIdx.x * H;
 #pragma unroll
-  for (int j = 0; j < DT; ++j) {
+  for (int q = 0; q < DPW; ++q) {
+    const int j = wave + q * kAccWaves;
     if (j < D) {
-      ph[j * GRA_VEGAS_BINS + lane] = hlo[j];
-      ph[j * GRA_VEGAS_BINS + lane + 64] = hhi[j];
-      ph[(D + j) * GRA_VEGAS_BINS + lane] = h2lo[j];
-      ph[(D + j) * GRA_VEGAS_BINS + lane + 64] = h2hi[j];
+      ph[j * GRA_VEGAS_BINS + lane] = hlo[q];
+      ph[j * GRA_VEGAS_BINS + lane + 64] = hhi[q];
+      ph[(D + j) * GRA_VEGAS_BINS + lane] = h2lo[q];
+      ph[(D + j) * GRA_VEGAS_BINS + lane + 64] = h2hi[q];
     }
   }
+  if (wave != 0) return;
   // Wave reduction of the scalars in a fixed lane order
   for (int off = 32; off > 0; off >>= 1) {
     const double ofs = __shfl_down(fs, off), ofc = __shfl_down(fc, off);
@@ -283,14 +291,14 @@ hipError_t launch_accumulate(const double* W, const double* vw, const uint8_t* b
                              double* part_scal, double* out, hipStream_t st) {
   const uint64_t chunk = (n + nblk - 1) / nblk;
   if (D <= 8)
-    hipLaunchKernelGGL(k_accumulate<8>, dim3(nblk), dim3(kAccWave), 0, st, W, vw, bins, flags, n,
-                       chunk, D, part_hist, part_scal);
+    hipLaunchKernelGGL(k_accumulate<8>, dim3(nblk), dim3(kAccWave * kAccWaves), 0, st, W, vw,
+                       bins, flags, n, chunk, D, part_hist, part_scal);
   else if (D <= 14)
-    hipLaunchKernelGGL(k_accumulate<14>, dim3(nblk), dim3(kAccWave), 0, st, W, vw, bins, flags, n,
-                       chunk, D, part_hist, part_scal);
+    hipLaunchKernelGGL(k_accumulate<14>, dim3(nblk), dim3(kAccWave * kAccWaves), 0, st, W, vw,
+                       bins, flags, n, chunk, D, part_hist, part_scal);
   else
-    hipLaunchKernelGGL(k_accumulate<16>, dim3(nblk), dim3(kAccWave), 0, st, W, vw, bins, flags, n,
-                       chunk, D, part_hist, part_scal);
+    hipLaunchKernelGGL(k_accumulate<16>, dim3(nblk), dim3(kAccWave * kAccWaves), 0, st, W, vw,
+                       bins, flags, n, chunk, D, part_hist, part_scal);
   const int H = 2 * D * GRA_VEGAS_BINS;
   // stage-1 partial sums use the tail of part_hist's allocation: [G][H] then [G][10]
   double* tmp = part_hist + (uint64_t)nblk * H;
