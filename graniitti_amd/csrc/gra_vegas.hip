@@ -8,19 +8,28 @@ namespace {
 
 // Deterministic accumulation of one iteration. Each block owns a contiguous
 // range of points and has kAccWaves waves that walk the same points; wave w
-// owns the dimensions w, w + kAccWaves, ... Lane l of a wave owns histogram
-// bins l and l + 64 of its dimensions and keeps them in registers; the wave
-// walks the non-zero points in point order and broadcasts each point's (f, bin)
-// with v_readlane, so every bin is summed by one lane in point order, the same
-// on every run, without atomics. Wave 0 also sums the scalars.
+// owns the dimensions w, w + kAccWaves, ... and keeps their histograms of f and
+// f^2 in its own LDS slice. For each batch of 64 points and each owned
+// dimension the wave sorts the keys (bin, lane) with a bitonic network, sums f
+// and f^2 over each run of equal bins with a segmented scan, and the last lane
+// of each run adds the run's sums to its bin. Keys are unique, so the order of
+// every addition is fixed and the histograms are the same on every run, without
+// atomics. Wave 0 also sums the scalars.
 constexpr int kAccWave = 64;
 constexpr int kAccWaves = 4;
 
-__device__ __forceinline__ double readlane_d(double v, int src) {
-  const uint64_t u = __double_as_longlong(v);
-  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, src);
-  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), src);
-  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+// Ascending bitonic sort of one int key per lane across the wave
+__device__ __forceinline__ int wave_sort(int key, int lane) {
+#pragma unroll
+  for (int k = 2; k <= kAccWave; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const int other = __shfl_xor(key, j);
+      const bool up = (lane & k) == 0, lower = (lane & j) == 0;
+      key = (lower == up) ? min(key, other) : max(key, other);
+    }
+  }
+  return key;
 }
 
 template <int DT>
@@ -29,11 +38,14 @@ __global__ __launch_bounds__(kAccWave* kAccWaves) void k_accumulate(
     const uint8_t* __restrict__ flags, uint64_t n, uint64_t chunk, int D,
     double* __restrict__ part_hist, double* __restrict__ part_scal) {
   constexpr int DPW = (DT + kAccWaves - 1) / kAccWaves;  // dimensions per wave
+  constexpr int NB = GRA_VEGAS_BINS;
+  __shared__ double hist[kAccWaves][DPW][2][NB];
   const int lane = threadIdx.x & (kAccWave - 1);
   const int wave = threadIdx.x / kAccWave;  // wave-uniform
-  double hlo[DPW], hhi[DPW], h2lo[DPW], h2hi[DPW];
+  double(*hw)[2][NB] = hist[wave];
 #pragma unroll
-  for (int q = 0; q < DPW; ++q) hlo[q] = hhi[q] = h2lo[q] = h2hi[q] = 0.0;
+  for (int q = 0; q < DPW; ++q)
+    for (int b = lane; b < NB; b += kAccWave) hw[q][0][b] = hw[q][1][b] = 0.0;
   const uint64_t p0 = (uint64_t)blockIdx.x * chunk;
   uint64_t p1 = p0 + chunk;
   if (p1 > n) p1 = n;
@@ -71,35 +83,42 @@ __global__ __launch_bounds__(kAccWave* kAccWaves) void k_accumulate(
         if (j < D) bb[q] = bins[i * D + j];
       }
     }
-    uint64_t m = __ballot(f != 0.0);
-    while (m) {
-      const int src = __ffsll((unsigned long long)m) - 1;
-      m &= m - 1;
-      const double fv = readlane_d(f, src);
-      const double fv2 = fv * fv;
+    if (__ballot(f != 0.0) == 0) continue;  // wave-uniform
 #pragma unroll
-      for (int q = 0; q < DPW; ++q) {
-        const int b = __builtin_amdgcn_readlane(bb[q], src);
-        // f >= 0, so adding +0.0 to the bins a lane does not own leaves them unchanged
-        const bool lo = b == lane, hi = b == lane + 64;
-        hlo[q] += lo ? fv : 0.0;
-        h2lo[q] += lo ? fv2 : 0.0;
-        hhi[q] += hi ? fv : 0.0;
-        h2hi[q] += hi ? fv2 : 0.0;
+    for (int q = 0; q < DPW; ++q) {
+      if (wave + q * kAccWaves >= D) break;  // wave-uniform
+      // zero contributions sort behind every bin (key >= 128 * 64) and are dropped
+      const int key = wave_sort(f != 0.0 ? (bb[q] << 6) | lane : (NB << 6) | lane, lane);
+      const int bin = key >> 6;
+      double s = __shfl(f, key & 63);
+      double s2 = s * s;
+      // segmented inclusive scan over runs of equal bins (sorted, so a run is contiguous)
+#pragma unroll
+      for (int d = 1; d < kAccWave; d <<= 1) {
+        const int ob = __shfl_up(bin, d);
+        const double os = __shfl_up(s, d), os2 = __shfl_up(s2, d);
+        if (lane >= d && ob == bin) {
+          s += os;
+          s2 += os2;
+        }
+      }
+      const int nb = __shfl_down(bin, 1);
+      if (bin < NB && (lane == kAccWave - 1 || nb != bin)) {
+        hw[q][0][bin] += s;
+        hw[q][1][bin] += s2;
       }
     }
   }
-  const uint64_t H = 2 * (uint64_t)D * GRA_VEGAS_BINS;
+  const uint64_t H = 2 * (uint64_t)D * NB;
   double* ph = part_hist + (uint64_t)blockIdx.x * H;
 #pragma unroll
   for (int q = 0; q < DPW; ++q) {
     const int j = wave + q * kAccWaves;
-    if (j < D) {
-      ph[j * GRA_VEGAS_BINS + lane] = hlo[q];
-      ph[j * GRA_VEGAS_BINS + lane + 64] = hhi[q];
-      ph[(D + j) * GRA_VEGAS_BINS + lane] = h2lo[q];
-      ph[(D + j) * GRA_VEGAS_BINS + lane + 64] = h2hi[q];
-    }
+    if (j < D)
+      for (int b = lane; b < NB; b += kAccWave) {
+        ph[j * NB + b] = hw[q][0][b];
+        ph[(D + j) * NB + b] = hw[q][1][b];
+      }
   }
   if (wave != 0) return;
   // Wave reduction of the scalars in a fixed lane order
