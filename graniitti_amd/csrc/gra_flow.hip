@@ -381,13 +381,11 @@ __global__ __launch_bounds__(256) void k_rqs_backward_t(
         const float swm = ew[m] * iw, shm = eh[m] * ih;
         const float w = kEpsW + cw * swm;
         const bool hit = !found && (m == KT - 1 || Xa + w > x);
-        if (hit) {
-          k = m;
-          X = Xa;
-          Y = Ya;
-          swk = swm;
-          shk = shm;
-        }
+        k = hit ? m : k;
+        X = hit ? Xa : X;
+        Y = hit ? Ya : Y;
+        swk = hit ? swm : swk;
+        shk = hit ? shm : shk;
         found = found || hit;
         Xa += w;
         Ya += kEpsW + cw * shm;
@@ -814,15 +812,36 @@ __global__ void k_coef(const double* __restrict__ w, int n, double inv_s2, float
 
 // ---------------------------------------------------------------------------
 // Fused coupling layer, sampling direction (BF16 MFMA MLP): one block of 256
-// threads takes kRows points through the whole layer. The conditioner MLP runs
+// threads takes R points through the whole layer. The conditioner MLP runs
 // out of LDS (input layer on the VALU in FP32, hidden and output products on
 // v_mfma_f32_16x16x32_bf16 with the transposed BF16 weights read through L2),
 // the spline parameters stay in LDS, and each thread then inverts one
 // (point, transformed dimension) spline with the knots accumulated in order.
 // Only u_{l-1}, log q and, when the layer's activations are needed for the
 // backward pass, the activations and spline parameters are written to HBM.
+// LDS holds two activation tiles; the FP32 parameter tile of the output
+// product overlays the one that product does not read. R = 32 rows per block
+// (GRA_FUSED_ROWS_256 = 16 halves the tile at H = 256 for two blocks per CU,
+// which measured slower: the B fragments are then read twice as often).
 // ---------------------------------------------------------------------------
-constexpr int kRows = 32;
+#ifndef GRA_FUSED_ROWS_256
+#define GRA_FUSED_ROWS_256 32
+#endif
+#ifndef GRA_FUSED_THREADS_256
+#define GRA_FUSED_THREADS_256 512
+#endif
+#ifndef GRA_FUSED_THREADS
+#define GRA_FUSED_THREADS 256
+#endif
+__host__ __device__ constexpr int fused_rows(int H) { return H == 256 ? GRA_FUSED_ROWS_256 : 32; }
+// Threads per block: more waves share the block's LDS tiles and split its
+// n-tiles, so each SIMD has several waves to switch between while the B
+// fragments are in flight (one wave per SIMD leaves it waiting most of the time)
+__host__ __device__ constexpr int fused_threads(int H) {
+  return H == 256 ? GRA_FUSED_THREADS_256 : GRA_FUSED_THREADS;
+}
+// Minimum waves per SIMD the register allocation must allow: two blocks per CU
+__host__ __device__ constexpr int fused_waves(int H) { return 2 * fused_threads(H) / 256; }
 
 struct FusedLayer {
   const float* W1;            // [dA x H]
@@ -833,59 +852,84 @@ struct FusedLayer {
   float* prm_out;             // optional store of the spline parameters ([n x ldp])
 };
 
-// One product of the MLP on the block's kRows rows: out = A[kRows x H] Wt^T,
+// One product of the MLP on the block's R rows: out = A[R x H] Wt^T,
 // with bias (+ ReLU into the next BF16 activation buffer, or into the FP32
 // parameter tile for the output product)
-template <int H>
+// TT n-tiles nt0, nt0 + NW, ... of one product on the block's R rows: every
+// load is unconditional, so the compiler keeps the B fragments of a k-step in
+// flight together instead of waiting for each before its MFMA
+template <int H, int R, int NW, int TT>
+__device__ __forceinline__ void product_tiles(const __bf16* sA, int lda,
+                                              const __bf16* __restrict__ wt, int nt0,
+                                              f32x4 (&acc)[R / 16][4]) {
+  constexpr int MT = R / 16;
+  const int lane = threadIdx.x & 63;
+  const int kq = 8 * (lane >> 4);
+#pragma unroll
+  for (int k0 = 0; k0 < H; k0 += 32) {
+    bf16x8 a[MT], b[TT];
+#pragma unroll
+    for (int t = 0; t < TT; ++t)
+      b[t] = *(const bf16x8*)(wt + (size_t)((nt0 + NW * t) * 16 + (lane & 15)) * H + k0 + kq);
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+      a[m] = *(const bf16x8*)(sA + (size_t)(16 * m + (lane & 15)) * lda + k0 + kq);
+#pragma unroll
+    for (int t = 0; t < TT; ++t)
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+        acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b[t], acc[m][t], 0, 0, 0);
+  }
+}
+
+template <int H, int R, int NW>
 __device__ __forceinline__ void fused_product(const __bf16* sA, int lda, const __bf16* __restrict__ wt,
-                                              const float* __restrict__ bias, int N,
+                                              const float* __restrict__ bias, int tb, int N,
                                               int nvalid, bool out_layer, __bf16* sOut, int ldo,
                                               float* sP, int ldpp, __bf16* gact, float* gprm,
                                               int row0, int n, int ldp) {
+  // output n-tiles tb .. tb + N/16 - 1; tile tb is column 0 of sP / sOut
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int ntiles = N / 16;
-  const int kq = 8 * (lane >> 4);
-  for (int nt0 = wave; nt0 < ntiles; nt0 += 16) {
-    // each wave owns n-tiles nt0, nt0 + 4, nt0 + 8, nt0 + 12 of this chunk
-    f32x4 acc[2][4];
+  wt += (size_t)tb * 16 * H;
+  for (int nt0 = wave; nt0 < ntiles; nt0 += 4 * NW) {
+    // each of the NW waves owns n-tiles nt0 + NW t (t < 4) of this chunk; nv of
+    // them exist (wave-uniform)
+    constexpr int MT = R / 16;
+    const int nv = min(4, (ntiles - nt0 + NW - 1) / NW);
+    f32x4 acc[MT][4];
 #pragma unroll
-    for (int m = 0; m < 2; ++m)
+    for (int m = 0; m < MT; ++m)
 #pragma unroll
       for (int t = 0; t < 4; ++t) acc[m][t] = (f32x4){0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int k0 = 0; k0 < H; k0 += 32) {
-      const bf16x8 a0 = *(const bf16x8*)(sA + (size_t)(lane & 15) * lda + k0 + kq);
-      const bf16x8 a1 = *(const bf16x8*)(sA + (size_t)(16 + (lane & 15)) * lda + k0 + kq);
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int nt = nt0 + 4 * t;
-        if (nt < ntiles) {
-          const bf16x8 b = *(const bf16x8*)(wt + (size_t)(nt * 16 + (lane & 15)) * H + k0 + kq);
-          acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b, acc[0][t], 0, 0, 0);
-          acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b, acc[1][t], 0, 0, 0);
-        }
-      }
-    }
+    if (nv == 4)
+      product_tiles<H, R, NW, 4>(sA, lda, wt, nt0, acc);
+    else if (nv == 3)
+      product_tiles<H, R, NW, 3>(sA, lda, wt, nt0, acc);
+    else if (nv == 2)
+      product_tiles<H, R, NW, 2>(sA, lda, wt, nt0, acc);
+    else
+      product_tiles<H, R, NW, 1>(sA, lda, wt, nt0, acc);
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      const int nt = nt0 + 4 * t;
-      if (nt >= ntiles) continue;
-      const int col = nt * 16 + (lane & 15);
+      if (t >= nv) break;
+      const int nt = nt0 + NW * t;
+      const int cl = nt * 16 + (lane & 15), col = tb * 16 + cl;
       const float bv = bias[col];
 #pragma unroll
-      for (int m = 0; m < 2; ++m)
+      for (int m = 0; m < MT; ++m)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int row = 16 * m + 4 * (lane >> 4) + r;
           float v = acc[m][t][r] + bv;
           if (out_layer) {
             if (col >= nvalid) v = 0.f;
-            sP[row * ldpp + col] = v;
+            sP[row * ldpp + cl] = v;
             if (gprm && row0 + row < n) gprm[(size_t)(row0 + row) * ldp + col] = v;
           } else {
             v = fmaxf(v, 0.f);
             const __bf16 hv = (__bf16)v;
-            sOut[row * ldo + col] = hv;
+            sOut[row * ldo + cl] = hv;
             if (gact && row0 + row < n) gact[(size_t)(row0 + row) * H + col] = hv;
           }
         }
@@ -904,45 +948,43 @@ __device__ __forceinline__ void inverse_bin(const float* p, int K, float zv, Bin
   float X = 0.f, Y = 0.f, w = 0.f, h = 0.f;
   int k = 0;
   if constexpr (KT > 0) {
-    float ew[KT], eh[KT];
+    // heights in registers: their softmax and the bin search; the widths are
+    // then walked from LDS (two exponentials each) to keep the registers low
+    float eh[KT];
 #pragma unroll
-    for (int m = 0; m < KT; ++m) {
-      ew[m] = p[m];
-      eh[m] = p[KT + m];
-    }
-    float mw = ew[0], mh = eh[0];
+    for (int m = 0; m < KT; ++m) eh[m] = p[KT + m];
+    float mw = p[0], mh = eh[0];
 #pragma unroll
     for (int m = 1; m < KT; ++m) {
-      mw = fmaxf(mw, ew[m]);
+      mw = fmaxf(mw, p[m]);
       mh = fmaxf(mh, eh[m]);
     }
     float sw = 0.f, sh = 0.f;
 #pragma unroll
     for (int m = 0; m < KT; ++m) {
-      ew[m] = __expf(ew[m] - mw);
+      sw += __expf(p[m] - mw);
       eh[m] = __expf(eh[m] - mh);
-      sw += ew[m];
       sh += eh[m];
     }
     const float cw = 1.f - KT * kEpsW;
     const float iw = cw / sw, ih = cw / sh;
-    float Xa = 0.f, Ya = 0.f;
+    float Ya = 0.f;
     bool found = false;
 #pragma unroll
     for (int m = 0; m < KT; ++m) {
-      const float wm = kEpsW + ew[m] * iw;
       const float hm = kEpsW + eh[m] * ih;
       const bool hit = !found && (m == KT - 1 || Ya + hm > zv);
-      if (hit) {
-        k = m;
-        X = Xa;
-        Y = Ya;
-        w = wm;
-        h = hm;
-      }
+      k = hit ? m : k;
+      Y = hit ? Ya : Y;
+      h = hit ? hm : h;
       found = found || hit;
-      Xa += wm;
       Ya += hm;
+    }
+#pragma unroll
+    for (int m = 0; m < KT; ++m) {
+      const float wm = kEpsW + __expf(p[m] - mw) * iw;
+      w = (m == k) ? wm : w;
+      X = (m < k) ? X + wm : X;
     }
   } else {
     float mw = p[0], mh = p[K];
@@ -973,89 +1015,106 @@ __device__ __forceinline__ void inverse_bin(const float* p, int K, float zv, Bin
 }
 
 template <int H, int KT>
-__global__ __launch_bounds__(256) void k_flow_layer_inverse(const float* __restrict__ u_in,
-                                                            float* __restrict__ u_out,
-                                                            float* __restrict__ ld, int n, int D,
-                                                            LayerIdx L, int Kin, int depth, int ldp,
-                                                            FusedLayer W) {
+__global__ __launch_bounds__(fused_threads(H), fused_waves(H)) void k_flow_layer_inverse(
+    const float* __restrict__ u_in, float* __restrict__ u_out, float* __restrict__ ld, int n,
+    int D, LayerIdx L, int Kin, int depth, int ldp, int gd, int ldg, FusedLayer W) {
   const int K = KT > 0 ? KT : Kin;
+  constexpr int R = fused_rows(H);
+  constexpr int NT = fused_threads(H);
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   constexpr int lda = H + 8;  // BF16 row stride (16-byte multiple, staggers banks)
-  const int ldpp = ldp + 1;
-  __bf16* sAct0 = (__bf16*)smem;
-  __bf16* sAct1 = sAct0 + kRows * lda;
-  float* sP = (float*)(sAct1 + kRows * lda);
-  float* sLj = sP + kRows * ldpp;
-  float* sU = sLj + kRows * 8;  // the block's input rows [kRows x D]
-  const int row0 = blockIdx.x * kRows;
-  for (int e = threadIdx.x; e < kRows * D; e += blockDim.x) {
+  const int ldpp = ldg;  // parameter tile row stride (one group of gd dimensions)
+  // X: activation tile; Y: activation tile, overlaid by the parameter tile
+  __bf16* sX = (__bf16*)smem;
+  __bf16* sY = sX + R * lda;
+  float* sP = (float*)sY;
+  const int ybytes = max(R * lda * (int)sizeof(__bf16), R * ldpp * (int)sizeof(float));
+  float* sLj = (float*)((unsigned char*)sY + ybytes);
+  float* sU = sLj + R * 8;  // the block's input rows [R x D]
+  const int row0 = blockIdx.x * R;
+  for (int e = threadIdx.x; e < R * D; e += blockDim.x) {
     const int row = row0 + e / D;
     sU[e] = row < n ? u_in[(size_t)row0 * D + e] : 0.f;
   }
   __syncthreads();
+  // activation i sits in X when depth - 1 - i is even, so the output product
+  // reads X and writes the parameter tile over Y
+  __bf16* cur = (depth & 1) ? sX : sY;
+  __bf16* nxt = (depth & 1) ? sY : sX;
   // input layer (FP32 VALU): act0 = relu(u[A] W1 + b1); each thread owns one
-  // column, with its weights in registers, for kRows / (256 / H) rows
+  // column, with its weights in registers, for R / (NT / H) rows
   {
-    constexpr int nrg = 256 / H;
+    constexpr int nrg = NT / H;
     const int j = threadIdx.x % H, rg = threadIdx.x / H;
+    // indices past dA read column 0 with weight 0, so no load is conditional
     float w[kMaxDB];
+    int ai[kMaxDB];
 #pragma unroll
-    for (int i = 0; i < kMaxDB; ++i) w[i] = i < L.dA ? W.W1[i * H + j] : 0.f;
+    for (int i = 0; i < kMaxDB; ++i) {
+      const bool on = i < L.dA;
+      w[i] = on ? W.W1[(on ? i : 0) * H + j] : 0.f;
+      ai[i] = on ? L.A[i] : 0;
+    }
     const float bj = W.b1[j];
-    for (int r = rg; r < kRows; r += nrg) {
+    for (int r = rg; r < R; r += nrg) {
       float acc = bj;
 #pragma unroll
-      for (int i = 0; i < kMaxDB; ++i)
-        if (i < L.dA) acc += sU[r * D + L.A[i]] * w[i];
+      for (int i = 0; i < kMaxDB; ++i) acc += sU[r * D + ai[i]] * w[i];
       const __bf16 hv = (__bf16)fmaxf(acc, 0.f);
-      sAct0[r * lda + j] = hv;
+      cur[r * lda + j] = hv;
       if (W.act_out[0] && row0 + r < n) W.act_out[0][(size_t)(row0 + r) * H + j] = hv;
     }
   }
   __syncthreads();
-  __bf16* cur = sAct0;
-  __bf16* nxt = sAct1;
   const int np = 3 * K - 1;
-  for (int i = 1; i <= depth; ++i) {
-    const bool out = i == depth;
-    fused_product<H>(cur, lda, W.wt[i - 1], W.bias[i - 1], out ? ldp : H, out ? L.dB * np : H, out,
-                  nxt, lda, sP, ldpp, out ? nullptr : W.act_out[i], out ? W.prm_out : nullptr,
-                  row0, n, ldp);
+  for (int i = 1; i < depth; ++i) {
+    fused_product<H, R, NT / 64>(cur, lda, W.wt[i - 1], W.bias[i - 1], 0, H, H, false, nxt, lda,
+                                 sP, ldpp, W.act_out[i], nullptr, row0, n, ldp);
     __syncthreads();
-    __bf16* t = cur;
+    __bf16* tmp = cur;
     cur = nxt;
-    nxt = t;
+    nxt = tmp;
   }
-  // spline inverses: thread -> (row, transformed dimension)
+  // output product and spline inverses by groups of gd transformed dimensions:
+  // a group's columns [j0 np, j1 np) span the n-tiles [tb, te); thread -> (row,
+  // dimension of the group)
   const int t = threadIdx.x;
-  const int r = t / L.dB, j = t % L.dB;
-  const int row = row0 + r;
-  if (r < kRows && row < n) {
-    if (j == 0)
-      for (int i = 0; i < L.dA; ++i) u_out[(size_t)row * D + L.A[i]] = sU[r * D + L.A[i]];
-    const float* p = sP + r * ldpp + j * np;
-    const float zv = fminf(fmaxf(sU[r * D + L.B[j]], 0.f), 1.f);
-    Bin e;
-    inverse_bin<KT>(p, K, zv, e);
-    const int k = e.k;
-    e.dk = (k == 0) ? 1.f : kEpsD + softplusf(p[2 * K + k - 1]);
-    e.dk1 = (k == K - 1) ? 1.f : kEpsD + softplusf(p[2 * K + k]);
-    e.s = e.h / e.w;
-    const float zeta = zv - e.Y;
-    const float Dl = e.dk1 + e.dk - 2.f * e.s;
-    const float qa = e.h * (e.s - e.dk) + zeta * Dl;
-    const float qb = e.h * e.dk - zeta * Dl;
-    const float qc = -e.s * zeta;
-    const float disc = fmaxf(qb * qb - 4.f * qa * qc, 0.f);
-    float xi = (2.f * qc) / (-qb - sqrtf(disc));
-    xi = fminf(fmaxf(xi, 0.f), 1.f);
-    float zz, lj;
-    rqs_value(e, xi, zz, lj);
-    u_out[(size_t)row * D + L.B[j]] = fminf(fmaxf(e.X + xi * e.w, 0.f), 0.99999994f);
-    sLj[r * 8 + j] = lj;
+  for (int j0 = 0; j0 < L.dB; j0 += gd) {
+    const int j1 = min(L.dB, j0 + gd), g = j1 - j0;
+    const int tb = (j0 * np) / 16, te = (j1 * np + 15) / 16;
+    fused_product<H, R, NT / 64>(cur, lda, W.wt[depth - 1], W.bias[depth - 1], tb, 16 * (te - tb),
+                                 L.dB * np, true, nxt, lda, sP, ldpp, nullptr, W.prm_out, row0, n,
+                                 ldp);
+    __syncthreads();
+    const int r = t / g, j = j0 + t % g;
+    const int row = row0 + r;
+    if (r < R && row < n) {
+      if (j == 0)
+        for (int i = 0; i < L.dA; ++i) u_out[(size_t)row * D + L.A[i]] = sU[r * D + L.A[i]];
+      const float* p = sP + r * ldpp + j * np - 16 * tb;
+      const float zv = fminf(fmaxf(sU[r * D + L.B[j]], 0.f), 1.f);
+      Bin e;
+      inverse_bin<KT>(p, K, zv, e);
+      const int k = e.k;
+      e.dk = (k == 0) ? 1.f : kEpsD + softplusf(p[2 * K + k - 1]);
+      e.dk1 = (k == K - 1) ? 1.f : kEpsD + softplusf(p[2 * K + k]);
+      e.s = e.h / e.w;
+      const float zeta = zv - e.Y;
+      const float Dl = e.dk1 + e.dk - 2.f * e.s;
+      const float qa = e.h * (e.s - e.dk) + zeta * Dl;
+      const float qb = e.h * e.dk - zeta * Dl;
+      const float qc = -e.s * zeta;
+      const float disc = fmaxf(qb * qb - 4.f * qa * qc, 0.f);
+      float xi = (2.f * qc) / (-qb - sqrtf(disc));
+      xi = fminf(fmaxf(xi, 0.f), 1.f);
+      float zz, lj;
+      rqs_value(e, xi, zz, lj);
+      u_out[(size_t)row * D + L.B[j]] = fminf(fmaxf(e.X + xi * e.w, 0.f), 0.99999994f);
+      sLj[r * 8 + j] = lj;
+    }
+    __syncthreads();
   }
-  __syncthreads();
-  if (t < kRows && row0 + t < n) {
+  if (t < R && row0 + t < n) {
     float acc = 0.f;
     for (int jj = 0; jj < L.dB; ++jj) acc += sLj[t * 8 + jj];
     ld[row0 + t] += acc;
@@ -1083,6 +1142,8 @@ static FlowState* flow_of(gra_handle* h);
 
 struct FlowLayerHost {
   LayerIdx idx;
+  int gd = 1, ldg = 1;  // fused sampling layer: dimensions per output group, tile stride
+  size_t lds = 0;       // and its LDS bytes
   size_t oW1, ob1, oWo, obo;
   size_t oWh[4], obh[4];
 };
@@ -1312,9 +1373,33 @@ static const void* spline_backward_kernel(int K) {
 // configuration asks for them
 static int spline_k(const FlowState* F) { return F->fc.spline_loops ? 0 : F->K; }
 
-static size_t fused_lds_bytes(const FlowState* F) {
-  return (size_t)2 * kRows * (F->H + 8) * sizeof(__bf16) + (size_t)kRows * (F->ldp + 1) * 4 +
-         (size_t)kRows * 8 * 4 + (size_t)kRows * F->D * 4;
+// LDS bytes of the fused layer with a parameter tile of row stride ldg
+static size_t fused_lds_bytes(const FlowState* F, int ldg) {
+  const size_t R = (size_t)fused_rows(F->H);
+  const size_t act = R * (F->H + 8) * sizeof(__bf16);
+  return act + std::max(act, R * (size_t)ldg * sizeof(float)) + R * 8 * 4 + R * F->D * 4;
+}
+
+// Output groups of the fused layer for a layer with dB transformed dimensions:
+// the group of gd dimensions [j0, j1) computes the n-tiles covering columns
+// [j0 np, j1 np); the tile stride is that of the widest group. Without a
+// requested size the largest group that keeps two blocks per CU is taken.
+constexpr size_t kFusedLdsTarget = 80 * 1024 - 512;
+static void fused_groups(const FlowState* F, int dB, int want, int& gd, int& ldg, size_t& lds) {
+  const int np = 3 * F->K - 1;
+  auto stride = [&](int g) {
+    int w = 0;
+    for (int j0 = 0; j0 < dB; j0 += g) {
+      const int j1 = std::min(dB, j0 + g);
+      w = std::max(w, (j1 * np + 15) / 16 - (j0 * np) / 16);
+    }
+    return 16 * w + 1;
+  };
+  gd = want > 0 ? std::min(want, dB) : dB;
+  if (want <= 0)
+    while (gd > 1 && fused_lds_bytes(F, stride(gd)) > kFusedLdsTarget) --gd;
+  ldg = stride(gd);
+  lds = fused_lds_bytes(F, ldg);
 }
 
 // Sampling pass: z -> u_0 (stored per layer), ld = log q(u_0). With store set,
@@ -1331,9 +1416,9 @@ static int flow_sample_pass(gra_handle* h, FlowState* F, int n, uint64_t seed, u
                        n, D, u_of(F, L));
   GRA_HIP_CHECK(h, hipMemsetAsync(F->ld.p, 0, sizeof(float) * n, h->stream));
   if (!F->mlp_fp32) {
-    const size_t lds = fused_lds_bytes(F);
     for (int l = L - 1; l >= 0; --l) {
       const FlowLayerHost& Ly = F->layers[l];
+      const size_t lds = Ly.lds;
       FusedLayer W;
       std::memset(&W, 0, sizeof(W));
       const float* th = F->theta.as<float>();
@@ -1347,16 +1432,17 @@ static int flow_sample_pass(gra_handle* h, FlowState* F, int n, uint64_t seed, u
         for (int i = 0; i < F->depth; ++i) W.act_out[i] = act16_of(F, l, i);
         W.prm_out = prm_of(F, l);
       }
-      const dim3 grd((n + kRows - 1) / kRows);
+      const int R = fused_rows(F->H);
+      const dim3 grd((n + R - 1) / R);
       const float* uin = u_of(F, l + 1);
       float* uout = u_of(F, l);
       float* ldp_ = F->ld.as<float>();
       LayerIdx idx = Ly.idx;
-      int nn = n, DD = D, KK = F->K, dep = F->depth, ldp = F->ldp;
-      void* args[] = {(void*)&uin, (void*)&uout, (void*)&ldp_, (void*)&nn, (void*)&DD,
-                      (void*)&idx,  (void*)&KK,   (void*)&dep,  (void*)&ldp, (void*)&W};
-      GRA_HIP_CHECK(h, hipLaunchKernel(layer_inverse_kernel(F->H, spline_k(F)), grd, dim3(256),
-                                       args, lds, h->stream));
+      int nn = n, DD = D, KK = F->K, dep = F->depth, ldp = F->ldp, gd = Ly.gd, ldg = Ly.ldg;
+      void* args[] = {(void*)&uin, (void*)&uout, (void*)&ldp_, (void*)&nn,  (void*)&DD, (void*)&idx,
+                      (void*)&KK,  (void*)&dep,  (void*)&ldp,  (void*)&gd, (void*)&ldg, (void*)&W};
+      GRA_HIP_CHECK(h, hipLaunchKernel(layer_inverse_kernel(F->H, spline_k(F)), grd,
+                                       dim3(fused_threads(F->H)), args, lds, h->stream));
     }
     GRA_HIP_CHECK(h, hipGetLastError());
     return GRA_OK;
@@ -1615,7 +1701,11 @@ extern "C" int gra_hip_flow_init(gra_handle* h, const gra_flow_cfg* fc_in) {
     return fail(h, GRA_E_HIP, "flow parameter upload");
   }
   if (!F->mlp_fp32) {
-    const size_t lds = fused_lds_bytes(F);
+    size_t lds = 0;
+    for (FlowLayerHost& Ly : F->layers) {
+      fused_groups(F, Ly.idx.dB, fc.out_groups, Ly.gd, Ly.ldg, Ly.lds);
+      lds = std::max(lds, Ly.lds);
+    }
     if (lds > 160 * 1024) {
       flow_release(F);
       return fail(h, GRA_E_ARG, "flow layer tile needs %zu B of LDS (hidden %d, %d spline columns)",

@@ -185,7 +185,8 @@ typedef struct gra_flow_cfg {
   double ess_fallback; /* ESS fraction below which sampling falls back to VEGAS */
   int32_t spline_loops; /* nonzero: spline kernels loop over a runtime bin count for every K
                            (0: unrolled kernels for K = 8, 16, 32; used to check them) */
-  int32_t pad0;
+  int32_t out_groups;   /* transformed dimensions per output group of the fused sampling layer
+                           (0: the largest group that fits two blocks per CU in LDS) */
 } gra_flow_cfg;
 
 typedef struct gra_flow_stats {
