@@ -390,6 +390,10 @@ int build_eikonal(gra_handle* h) {
       nd.aeik_im = A.im;
       nd.cf_re = (float)(nd.w * kt * A.re);
       nd.cf_im = (float)(nd.w * kt * A.im);
+      nd.kxf = (float)nd.kx;
+      nd.kyf = (float)nd.ky;
+      nd.k2f = (float)(kt * kt);
+      nd.pad = 0.f;
     }
   }
   if (h->d_nodes.ensure(sizeof(NodeConst) * h->nodes.size()))

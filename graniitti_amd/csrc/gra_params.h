@@ -56,4 +56,5 @@ struct NodeConst {
   double w;        // Simpson 3/8 weight U_i V_j
   double aeik_re, aeik_im;  // A_eik(kt_j^2) from the interpolated table
   float cf_re, cf_im;       // FP32: w * kt * A_eik (scaled by the run's table scale)
+  float kxf, kyf, k2f, pad; // FP32 kx, ky, kt^2 for the FP32 node kinematics
 };
