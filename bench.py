@@ -36,13 +36,13 @@ sys.path.insert(0, ROOT)
 from graniitti_amd import abi  # noqa: E402
 from graniitti_amd.process import MContinuum, MGraniitti, vegas_cfg  # noqa: E402
 
-# Algorithmic work of one screening-loop node, counted on this build's kernel
-# formulation (reduced loop kinematics and invariants in FP64; s_ij, the Regge
-# amplitude and the compensated sum in FP32, each transcendental / reciprocal
-# counted as one operation). FP64 and non-packed FP32 vector instructions issue
-# at the same rate on gfx950, so the bound is the 78.6 TFLOP/s vector peak.
-FP64_FLOP_PER_NODE = 102
-FP32_FLOP_PER_NODE = 99
+# Algorithmic work of one screening-loop node of the FP32 amplitude kernel
+# (kinematics as changes from the tree point, invariants, Regge amplitude, row
+# sum), counted on the source: add / mul = 1, fma = 2, each transcendental or
+# reciprocal = 1. Non-packed FP32 vector instructions issue at the FP64 vector
+# rate on gfx950 (the 157.3 TFLOP/s FP32 vector figure needs packed FP32, which
+# these kernels do not use), so the bound is the 78.6 TFLOP/s vector peak.
+FLOP_PER_NODE = 157
 VALU_PEAK_TFLOPS = 78.6
 
 
@@ -261,19 +261,20 @@ def main():
         f"{sigma_err:.2e} barn")
 
     roofline = None
-    # per-node operation counts are those of the Regge continuum node
-    if cfg.screening and n_scr > 0 and cfg.process == abi.GRA_PROC_REGGE_CON:
+    # per-node operation counts are those of the FP32 Regge continuum node
+    if (cfg.screening and n_scr > 0 and cfg.process == abi.GRA_PROC_REGGE_CON
+            and cfg.amp_precision == abi.GRA_AMP_FP32):
         nodes = (cfg.n_loop_phi + 1) * (cfg.n_loop_kt + 1)
         cand_per_launch = n_valid / n_scr
         avg_ms = ms_scr / n_scr
-        flop = (FP64_FLOP_PER_NODE + FP32_FLOP_PER_NODE) * nodes * cand_per_launch
+        flop = FLOP_PER_NODE * nodes * cand_per_launch
         achieved = flop / (avg_ms * 1e-3) / 1e12
         roofline = {
             "bound": "valu", "kernel": "k_screen (screening loop)",
             "achieved": achieved, "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / VALU_PEAK_TFLOPS, "traffic": screen_traffic(),
             "avg_launch_ms": avg_ms, "launches": n_scr,
-            "fp64_flop_per_node": FP64_FLOP_PER_NODE, "fp32_flop_per_node": FP32_FLOP_PER_NODE,
+            "flop_per_node": FLOP_PER_NODE, "node_dtype": "f32",
             "nodes_per_point": nodes, "screened_points_per_launch": cand_per_launch,
         }
     # time to 0.1% relative sigma error with plain VEGAS (GPU)

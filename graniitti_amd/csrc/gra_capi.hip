@@ -396,10 +396,24 @@ int build_eikonal(gra_handle* h) {
       nd.pad = 0.f;
     }
   }
-  if (h->d_nodes.ensure(sizeof(NodeConst) * h->nodes.size()))
-    return fail(h, GRA_E_NOMEM, "node table");
-  GRA_HIP_CHECK(h, hipMemcpy(h->d_nodes.p, h->nodes.data(), sizeof(NodeConst) * h->nodes.size(),
-                             hipMemcpyHostToDevice));
+  // device table: the NodeConst entries, then the compact FP32 nodes
+  const size_t NN = h->nodes.size();
+  std::vector<NodeF32> n32(NN);
+  for (size_t k = 0; k < NN; ++k) {
+    const NodeConst& nd = h->nodes[k];
+    const double cre = nd.w * nd.kt * nd.aeik_re, cim = nd.w * nd.kt * nd.aeik_im;
+    NodeF32& f = n32[k];
+    f.kx = nd.kxf;
+    f.ky = nd.kyf;
+    f.k2 = nd.k2f;
+    f.arg = (float)(std::atan2(cim, cre) / (2.0 * GRA_PI));
+    f.mod = (float)std::hypot(cre, cim);
+    f.pad[0] = f.pad[1] = f.pad[2] = 0.f;
+  }
+  const size_t b0 = sizeof(NodeConst) * NN, b1 = sizeof(NodeF32) * NN;
+  if (h->d_nodes.ensure(b0 + b1)) return fail(h, GRA_E_NOMEM, "node table");
+  GRA_HIP_CHECK(h, hipMemcpy(h->d_nodes.p, h->nodes.data(), b0, hipMemcpyHostToDevice));
+  GRA_HIP_CHECK(h, hipMemcpy((char*)h->d_nodes.p + b0, n32.data(), b1, hipMemcpyHostToDevice));
   return GRA_OK;
 }
 

@@ -56,5 +56,15 @@ struct NodeConst {
   double w;        // Simpson 3/8 weight U_i V_j
   double aeik_re, aeik_im;  // A_eik(kt_j^2) from the interpolated table
   float cf_re, cf_im;       // FP32: w * kt * A_eik (scaled by the run's table scale)
-  float kxf, kyf, k2f, pad; // FP32 kx, ky, kt^2 for the FP32 node kinematics
+  float kxf, kyf, k2f, pad; // FP32 kx, ky, kt^2
+};
+
+// Compact FP32 node of the Regge screening loop (FP32 amplitude kernels). On the
+// device the node table holds the NodeConst entries followed by one NodeF32 per
+// node in the same order.
+struct NodeF32 {
+  float kx, ky, k2;  // kt cos(phi_i), kt sin(phi_i), kt^2
+  float arg;         // arg(w kt A_eik) / (2 pi), in revolutions
+  float mod;         // |w kt A_eik|
+  float pad[3];
 };

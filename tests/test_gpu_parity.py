@@ -63,8 +63,9 @@ def test_config1_fp32_amplitude_error():
     with MContinuum(cfg) as P:
         Wg, ag, fg = P.eval(u)
     Wo, ao, fo = ol.OracleRun(abi.cfg_default(1)).eval(u)
-    rel = compare(Wg, Wo, fg, fo, 1e-3, "cfg1 fp32")
-    assert np.median(rel) < 1e-5
+    # FP32 amplitude at the tree point: measured max 1.1e-6, median 1.6e-7
+    rel = compare(Wg, Wo, fg, fo, 2e-5, "cfg1 fp32")
+    assert np.median(rel) < 2e-6
 
 
 def screened_cfg(amp):
@@ -109,9 +110,10 @@ def test_config2_fp32_screened_error():
     with MContinuum(cfg) as P:
         Wg, ag, fg = P.eval(u)
     Wo, ao, fo = ol.OracleRun(screened_cfg(abi.GRA_AMP_FP64)).eval(u)
-    rel = compare(Wg, Wo, fg, fo, 1e-2, "cfg2 fp32 screened")
+    # FP32 loop nodes, FP32 row sums, FP64 sum of rows: measured max 5.5e-6, median 3.9e-7
+    rel = compare(Wg, Wo, fg, fo, 1e-4, "cfg2 fp32 screened")
     say(f"cfg2 fp32 |M|^2 rel err: median {np.median(rel):.2e} p99 {np.quantile(rel, 0.99):.2e}")
-    assert np.median(rel) < 1e-4
+    assert np.median(rel) < 4e-6
 
 
 def test_vegas_map_matches_oracle():
