@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgra_hip.so")
 
 GRA_OK = 0
-GRA_E_ARG, GRA_E_HIP, GRA_E_NOMEM, GRA_E_STATE, GRA_E_NUMERIC = -1, -2, -3, -4, -5
+GRA_E_ARG, GRA_E_HIP, GRA_E_NOMEM, GRA_E_STATE, GRA_E_NUMERIC, GRA_E_IO = -1, -2, -3, -4, -5, -6
 GRA_MAX_DIM = 16
 GRA_VEGAS_BINS = 128
 GRA_PROC_REGGE_CON = 15
@@ -114,6 +114,11 @@ VP = C.c_void_p
 SIGNATURES = {
     "gra_hip_cfg_default": (C.c_int, [P(GraCfg), I32]),
     "gra_hip_create": (C.c_int, [P(GraCfg), P(VP)]),
+    "gra_hip_create_cached": (C.c_int, [P(GraCfg), C.c_char_p, P(VP)]),
+    "gra_hip_eikonal_cache_name": (C.c_int, [P(GraCfg), I32, C.c_char_p, C.c_size_t]),
+    "gra_eikonal_table_write": (C.c_int, [C.c_char_p, P(D), I32]),
+    "gra_eikonal_table_read": (C.c_int, [C.c_char_p, P(D), I32]),
+    "gra_djb2": (U64, [C.c_char_p]),
     "gra_hip_destroy": (None, [VP]),
     "gra_hip_last_error": (C.c_char_p, [VP]),
     "gra_hip_eval_points": (C.c_int, [VP, P(D), C.c_size_t, P(D), P(D), P(C.c_uint8)]),

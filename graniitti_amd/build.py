@@ -32,6 +32,7 @@ UNITS = {
     "gra_flow.hip": ["-ffp-contract=fast"],
     "gra_config.cpp": ["-ffp-contract=off", "-x", "c++"],
     "gra_hepmc3.cpp": ["-ffp-contract=off", "-x", "c++"],
+    "gra_eikcache.cpp": ["-ffp-contract=off", "-x", "c++"],
 }
 
 

@@ -142,10 +142,6 @@ double gev2barn() {
   const double GeV2m2 = GeV2m * GeV2m;
   return GeV2m2 / 1E-28;
 }
-double gev2fm() {
-  const double c = 2.99792458E8, hbar = 6.582119514E-25;
-  return hbar * c * 1E15;
-}
 
 double simpson38_w(int i, int N) {
   if (i == 0 || i == N) return 1.0;
@@ -313,8 +309,30 @@ void fill_params(const gra_cfg& c, RunParams& P) {
   P.m_rho = c.m_central;
 }
 
+// One eikonal table through the cache directory dir (none: nullptr), as
+// MEikonal::S3Constructor handles MBT and MSA (src/MEikonal.cc:62-93, 98-129):
+// read the table's file; if it is missing or corrupt, compute the table, write
+// it and read it back. compute() fills T already rounded as that write-read
+// cycle rounds it (15 significant digits).
+template <typename F>
+int cached_table(gra_handle* h, const char* dir, int table, int n_rows, std::vector<double>& T,
+                 F compute) {
+  std::string path;
+  if (dir) {
+    path = std::string(dir) + "/" + eikonal_cache_name(h->cfg, h->P.s, table);
+    const int r = eikonal_table_read(path.c_str(), n_rows, T);
+    if (r == GRA_OK) return GRA_OK;
+    if (r != GRA_E_IO) return fail(h, r, "%s", global_error());
+  }
+  const int r = compute();
+  if (r != GRA_OK || !dir) return r;
+  int w = eikonal_table_write(path.c_str(), T);
+  if (w == GRA_OK) w = eikonal_table_read(path.c_str(), n_rows, T);
+  return w == GRA_OK ? GRA_OK : fail(h, w, "%s", global_error());
+}
+
 // MEikonal::S3Constructor tables on the GPU, src/MEikonal.cc:46-132
-int build_eikonal(gra_handle* h) {
+int build_eikonal(gra_handle* h, const char* cache_dir) {
   const gra_cfg& c = h->cfg;
   const RunParams& P = h->P;
   const int NF = c.n_fb;
@@ -333,19 +351,24 @@ int build_eikonal(gra_handle* h) {
     pre[2 * q + 1] = a.im;
   }
   const double bmin = 1E-6, bmax = 10.0 / gev2fm(), bstep = (bmax - bmin) / c.n_bt;
-  GRA_HIP_CHECK(h, hipMemcpyAsync(d_pre.p, pre.data(), sizeof(double) * pre.size(),
-                                  hipMemcpyHostToDevice, h->stream));
-  GRA_HIP_CHECK(h, launch_fb_table(0, d_pre.as<double>(), NF, kmin, kstep, bmin, bstep, c.n_bt + 1,
-                                   1 / (2.0 * GRA_PI * P.s), d_out.as<double>(), h->stream));
-  GRA_HIP_CHECK(h, hipMemcpyAsync(out.data(), d_out.p, sizeof(double) * 2 * (c.n_bt + 1),
-                                  hipMemcpyDeviceToHost, h->stream));
-  GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
-  h->mbt.resize(3 * (size_t)(c.n_bt + 1));
-  for (int i = 0; i <= c.n_bt; ++i) {
-    h->mbt[3 * i] = round15(bmin + i * bstep);
-    h->mbt[3 * i + 1] = round15(out[2 * i]);
-    h->mbt[3 * i + 2] = round15(out[2 * i + 1]);
-  }
+  int r = cached_table(h, cache_dir, 0, c.n_bt + 1, h->mbt, [&]() -> int {
+    GRA_HIP_CHECK(h, hipMemcpyAsync(d_pre.p, pre.data(), sizeof(double) * pre.size(),
+                                    hipMemcpyHostToDevice, h->stream));
+    GRA_HIP_CHECK(h, launch_fb_table(0, d_pre.as<double>(), NF, kmin, kstep, bmin, bstep,
+                                     c.n_bt + 1, 1 / (2.0 * GRA_PI * P.s), d_out.as<double>(),
+                                     h->stream));
+    GRA_HIP_CHECK(h, hipMemcpyAsync(out.data(), d_out.p, sizeof(double) * 2 * (c.n_bt + 1),
+                                    hipMemcpyDeviceToHost, h->stream));
+    GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    h->mbt.resize(3 * (size_t)(c.n_bt + 1));
+    for (int i = 0; i <= c.n_bt; ++i) {
+      h->mbt[3 * i] = round15(bmin + i * bstep);
+      h->mbt[3 * i + 1] = round15(out[2 * i]);
+      h->mbt[3 * i + 2] = round15(out[2 * i + 1]);
+    }
+    return GRA_OK;
+  });
+  if (r != GRA_OK) return r;
   // Screening amplitude A_eik(kt^2): pre_q = i (1 - exp(i Omega(bt_q) / 2))
   const double STEP = (bmax - bmin) / NF;
   for (int q = 0; q <= NF; ++q) {
@@ -358,19 +381,24 @@ int build_eikonal(gra_handle* h) {
     pre[2 * q + 1] = 0.0 * om.im + 1.0 * om.re;
   }
   const double amin = 1E-6, amax = 25.0, astep = (amax - amin) / c.n_kt2;
-  GRA_HIP_CHECK(h, hipMemcpyAsync(d_pre.p, pre.data(), sizeof(double) * pre.size(),
-                                  hipMemcpyHostToDevice, h->stream));
-  GRA_HIP_CHECK(h, launch_fb_table(1, d_pre.as<double>(), NF, bmin, STEP, amin, astep, c.n_kt2 + 1,
-                                   (2.0 * P.s) * (2.0 * GRA_PI), d_out.as<double>(), h->stream));
-  GRA_HIP_CHECK(h, hipMemcpyAsync(out.data(), d_out.p, sizeof(double) * 2 * (c.n_kt2 + 1),
-                                  hipMemcpyDeviceToHost, h->stream));
-  GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
-  h->msa.resize(3 * (size_t)(c.n_kt2 + 1));
-  for (int i = 0; i <= c.n_kt2; ++i) {
-    h->msa[3 * i] = round15(amin + i * astep);
-    h->msa[3 * i + 1] = round15(out[2 * i]);
-    h->msa[3 * i + 2] = round15(out[2 * i + 1]);
-  }
+  r = cached_table(h, cache_dir, 1, c.n_kt2 + 1, h->msa, [&]() -> int {
+    GRA_HIP_CHECK(h, hipMemcpyAsync(d_pre.p, pre.data(), sizeof(double) * pre.size(),
+                                    hipMemcpyHostToDevice, h->stream));
+    GRA_HIP_CHECK(h, launch_fb_table(1, d_pre.as<double>(), NF, bmin, STEP, amin, astep,
+                                     c.n_kt2 + 1, (2.0 * P.s) * (2.0 * GRA_PI),
+                                     d_out.as<double>(), h->stream));
+    GRA_HIP_CHECK(h, hipMemcpyAsync(out.data(), d_out.p, sizeof(double) * 2 * (c.n_kt2 + 1),
+                                    hipMemcpyDeviceToHost, h->stream));
+    GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    h->msa.resize(3 * (size_t)(c.n_kt2 + 1));
+    for (int i = 0; i <= c.n_kt2; ++i) {
+      h->msa[3 * i] = round15(amin + i * astep);
+      h->msa[3 * i + 1] = round15(out[2 * i]);
+      h->msa[3 * i + 2] = round15(out[2 * i + 1]);
+    }
+    return GRA_OK;
+  });
+  if (r != GRA_OK) return r;
   d_pre.release();
   d_out.release();
   // Loop nodes (phi_i, kt_j), i-major as the reference's (i, j) loops
@@ -487,7 +515,8 @@ extern "C" const char* gra_hip_last_error(const gra_handle* h) {
   return h ? h->err.c_str() : global_error();
 }
 
-extern "C" int gra_hip_create(const gra_cfg* cfg, gra_handle** out) {
+namespace {
+int create_handle(const gra_cfg* cfg, const char* eikonal_dir, gra_handle** out) {
   if (!out) return set_global_error(GRA_E_ARG, "gra_hip_create: out is NULL");
   *out = nullptr;
   const int v = validate(cfg);
@@ -513,7 +542,7 @@ extern "C" int gra_hip_create(const gra_cfg* cfg, gra_handle** out) {
     }
   }
   if (h->cfg.screening) {
-    const int r = build_eikonal(h);
+    const int r = build_eikonal(h, eikonal_dir);
     if (r != GRA_OK) {
       const std::string m = h->err;
       gra_hip_destroy(h);
@@ -521,6 +550,34 @@ extern "C" int gra_hip_create(const gra_cfg* cfg, gra_handle** out) {
     }
   }
   *out = h;
+  return GRA_OK;
+}
+}  // namespace
+
+extern "C" int gra_hip_create(const gra_cfg* cfg, gra_handle** out) {
+  return create_handle(cfg, nullptr, out);
+}
+
+extern "C" int gra_hip_create_cached(const gra_cfg* cfg, const char* eikonal_dir,
+                                     gra_handle** out) {
+  if (!eikonal_dir || !*eikonal_dir)
+    return set_global_error(GRA_E_ARG, "gra_hip_create_cached: eikonal_dir is empty");
+  return create_handle(cfg, eikonal_dir, out);
+}
+
+extern "C" int gra_hip_eikonal_cache_name(const gra_cfg* cfg, int32_t table, char* name,
+                                          size_t len) {
+  const int v = validate(cfg);
+  if (v != GRA_OK) return v;
+  if ((table != 0 && table != 1) || !name || len == 0)
+    return set_global_error(GRA_E_ARG, "gra_hip_eikonal_cache_name: table must be 0 or 1");
+  RunParams P;
+  fill_params(*cfg, P);
+  const std::string n = eikonal_cache_name(*cfg, P.s, table);
+  if (n.size() + 1 > len)
+    return set_global_error(GRA_E_ARG, "gra_hip_eikonal_cache_name: buffer of %zu < %zu", len,
+                            n.size() + 1);
+  std::memcpy(name, n.c_str(), n.size() + 1);
   return GRA_OK;
 }
 

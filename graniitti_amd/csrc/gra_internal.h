@@ -35,6 +35,21 @@ struct DevBuf {
 struct FlowState;  // normalizing-flow sampler (gra_flow.hip)
 void flow_release(FlowState* f);
 
+// 1 GeV^-1 in fermi: hbar c (include/Graniitti/MPDG.h:65-74)
+inline double gev2fm() {
+  const double c = 2.99792458E8, hbar = 6.582119514E-25;
+  return hbar * c * 1E15;
+}
+
+// Eikonal table cache in the reference's file format (gra_eikcache.cpp):
+// file name of table 0 (MBT) or 1 (MSA) for a configuration of Mandelstam s;
+// a table of n_rows rows "x,re,im" (GRA_E_IO: missing, short or malformed;
+// GRA_E_ARG: a field that is not a number); a table written with 15 significant
+// digits per value
+std::string eikonal_cache_name(const gra_cfg& c, double s, int table);
+int eikonal_table_read(const char* path, int n_rows, std::vector<double>& rows);
+int eikonal_table_write(const char* path, const std::vector<double>& rows);
+
 // Kernel timing slots
 enum TimeSlot { TS_TREE = 0, TS_SCREEN = 1, TS_ACCUM = 2, TS_FLOW = 3, TS_N = 4 };
 

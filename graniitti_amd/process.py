@@ -6,6 +6,7 @@ Initialize, GetXS; include/Graniitti/MGraniitti.h:105-202).
 All evaluation runs in libgra_hip.so; a missing library or GPU raises.
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -26,14 +27,29 @@ class AuxIntData:
         return self.kinematics_ok and self.fidcuts_ok and self.vetocuts_ok
 
 
+def eikonal_cache_name(cfg, table):
+    """File name under which table 0 (MBT) or 1 (MSA) of cfg is cached."""
+    lib = abi.load()
+    buf = C.create_string_buffer(256)
+    abi.check(lib.gra_hip_eikonal_cache_name(C.byref(cfg), table, buf, len(buf)))
+    return buf.value.decode()
+
+
 class MContinuum:
     """The <C> continuum process on the GPU."""
 
-    def __init__(self, cfg):
+    def __init__(self, cfg, eikonal_dir=None):
+        """eikonal_dir: a directory that caches the eikonal tables in the
+        reference's file format (MEikonal::S3Constructor, src/MEikonal.cc:46-132);
+        tables found there are read, missing ones computed and written."""
         self.lib = abi.load()
         self.cfg = cfg
         h = C.c_void_p()
-        abi.check(self.lib.gra_hip_create(C.byref(cfg), C.byref(h)))
+        if eikonal_dir is None:
+            abi.check(self.lib.gra_hip_create(C.byref(cfg), C.byref(h)))
+        else:
+            abi.check(self.lib.gra_hip_create_cached(C.byref(cfg), os.fsencode(eikonal_dir),
+                                                     C.byref(h)))
         self.h = h
 
     @classmethod

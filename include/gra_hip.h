@@ -40,6 +40,7 @@ typedef struct gra_handle gra_handle; /* opaque: device state of one process */
 #define GRA_E_NOMEM (-3)   /* device allocation failed */
 #define GRA_E_STATE (-4)   /* call not valid in the handle's state */
 #define GRA_E_NUMERIC (-5) /* too many NaN/Inf weights (src/MProcess.h:317-340) */
+#define GRA_E_IO (-6)      /* file missing, unreadable or malformed */
 
 /* ------------------------------------------------------------------------- */
 /* Limits                                                                    */
@@ -240,6 +241,33 @@ int gra_hip_cfg_default(gra_cfg* cfg, int32_t preset);
  * screening is on (MEikonal::S3Constructor, src/MEikonal.cc:46-132), and sets
  * up the process (MContinuum ctor + post_Constructor, src/MContinuum.cc:43-89). */
 int gra_hip_create(const gra_cfg* cfg, gra_handle** out);
+
+/* gra_hip_create with an eikonal table cache directory, as MEikonal::S3Constructor
+ * keeps its tables (src/MEikonal.cc:62-93, 98-129): each of MBT and MSA is read
+ * from eikonal_dir/<gra_hip_eikonal_cache_name> when that file holds n+1 rows
+ * "x,re,im"; otherwise it is computed on the GPU, written there with 15
+ * significant digits (IArray1D::WriteArray, src/MEikonal.cc:404-436) and read
+ * back (IArray1D::ReadArray, 439-485). A field that is not a number fails with
+ * GRA_E_ARG, as the reference's parse throws. Files the reference wrote for the
+ * same parameters are read as they are. */
+int gra_hip_create_cached(const gra_cfg* cfg, const char* eikonal_dir, gra_handle** out);
+
+/* File name of table 0 (MBT, proton density) or 1 (MSA, screening amplitude)
+ * for cfg: "MBT_2212_2212_<sqrt s, %.0f>_<djb2 hash>" (MSA_... likewise), the
+ * hash (src/MAux.cc:280-287) taken over the same string of std::to_string
+ * values of s, beams, PARAM_SOFT (src/MForm.cc:94-100) and MEikonalNumerics
+ * (include/Graniitti/MEikonal.h:59-67) as S3Constructor forms (src/MEikonal.cc:70-79,
+ * 107-116). */
+int gra_hip_eikonal_cache_name(const gra_cfg* cfg, int32_t table, char* name, size_t len);
+
+/* Host-only table file helpers (no GPU): n_rows rows of (x, re, im) in the
+ * format above. Read returns GRA_E_IO for a missing file or one without
+ * 3 n_rows values in its first n_rows lines (the reference's "Corrupted file"),
+ * GRA_E_ARG for a field that is not a number. */
+int gra_eikonal_table_write(const char* path, const double* rows, int32_t n_rows);
+int gra_eikonal_table_read(const char* path, double* rows, int32_t n_rows);
+/* djb2 string hash (gra::aux::djb2hash, src/MAux.cc:280-287), 64-bit */
+uint64_t gra_djb2(const char* s);
 
 /* Release every device resource of the handle. */
 void gra_hip_destroy(gra_handle* h);
