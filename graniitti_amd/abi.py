@@ -20,6 +20,7 @@ GRA_PROC_REGGE_CON = 15
 GRA_PROC_TENSOR_CON = 12
 GRA_AMP_FP64 = 0
 GRA_AMP_FP32 = 1
+GRA_PS_C, GRA_PS_F = 0, 1
 FLAG_KIN_OK, FLAG_FID_OK, FLAG_VETO_OK, FLAG_AMP_OK = 1, 2, 4, 8
 
 D = C.c_double
@@ -50,6 +51,7 @@ class GraCfg(C.Structure):
         ("screening", I32), ("n_loop_phi", I32), ("n_loop_kt", I32), ("n_bt", I32),
         ("n_kt2", I32), ("n_fb", I32),
         ("min_loop_kt", D), ("max_loop_kt", D),
+        ("phase_space", I32), ("pad_ps", I32), ("gen_M_min", D), ("gen_M_max", D),
     ]
 
 

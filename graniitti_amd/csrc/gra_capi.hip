@@ -159,6 +159,18 @@ int validate(const gra_cfg* c) {
   if (c->dim < dmin || c->dim > GRA_MAX_DIM)
     return set_global_error(GRA_E_ARG, "dim %d: this process needs %d..%d", c->dim, dmin,
                             GRA_MAX_DIM);
+  if (c->phase_space != GRA_PS_C && c->phase_space != GRA_PS_F)
+    return set_global_error(GRA_E_ARG, "phase_space %d unknown", c->phase_space);
+  if (c->phase_space == GRA_PS_F) {
+    if (c->process != GRA_PROC_REGGE_CON || c->w_central > 0.0 || c->m_daughter > 0.0)
+      return set_global_error(GRA_E_ARG,
+                              "<F> phase space: built for PP[CON] with stable central particles");
+    const double Mmax = std::min(c->beam_E1 + c->beam_E2 - 2.0 * c->m_beam, c->gen_M_max);
+    if (!(std::max(2.0 * c->m_central + 1e-4, c->gen_M_min) < Mmax))
+      return set_global_error(GRA_E_ARG,
+                              "<F> phase space: empty system mass range [%g, %g] (set gen_M_max; "
+                              "MFactorized::B51RandomKin throws here)", c->gen_M_min, c->gen_M_max);
+  }
   if (c->amp_precision != GRA_AMP_FP64 && c->amp_precision != GRA_AMP_FP32)
     return set_global_error(GRA_E_ARG, "amp_precision %d unknown", c->amp_precision);
   if (std::fabs(c->beam_E1 - c->beam_E2) > 0.0)
@@ -258,6 +270,26 @@ void fill_params(const gra_cfg& c, RunParams& P) {
   P.invS = 1.0 / c.S_factor;
   P.GeV2barn = gev2barn();
   P.flux = 2.0 * P.s;
+  // <F>: system mass range from the decay-tree masses (pions at their pole mass)
+  // and the GENCUTS, src/MFactorized.cc:259-285
+  P.phase_space = c.phase_space;
+  if (c.phase_space == GRA_PS_F) {
+    const double MARGIN = 1e-4;
+    double Mmin = (0.0 + c.m_central) + c.m_central + MARGIN;
+    const double mb1 = P.pbeam1.e * P.pbeam1.e -
+                       (P.pbeam1.x * P.pbeam1.x + P.pbeam1.y * P.pbeam1.y + P.pbeam1.z * P.pbeam1.z);
+    const double mb2 = P.pbeam2.e * P.pbeam2.e -
+                       (P.pbeam2.x * P.pbeam2.x + P.pbeam2.y * P.pbeam2.y + P.pbeam2.z * P.pbeam2.z);
+    double Mmax = P.sqrt_s - ((mb1 > 0.0 ? msqrt(mb1) : -msqrt(-mb1)) +
+                              (mb2 > 0.0 ? msqrt(mb2) : -msqrt(-mb2)));
+    Mmin = std::max(Mmin, c.gen_M_min);
+    Mmax = std::min(Mmax, c.gen_M_max);
+    P.f_Mmin = Mmin;
+    P.f_Mmax = Mmax;
+    P.f_vol = (Mmax * Mmax - Mmin * Mmin) * (c.rap_max - c.rap_min);
+    const double tp = 2.0 * GRA_PI;
+    P.f_psw = (1.0 / 2.0) * (1.0 / (tp * ((tp * tp) * (tp * tp))));
+  }
   // Regge
   int on = 0, kk = -1;
   for (int k = 0; k < 3; ++k) {

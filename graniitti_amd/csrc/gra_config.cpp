@@ -117,6 +117,19 @@ void set_pipi(gra_cfg* c) {
   c->fid_pt_min = 0.15;
 }
 
+// PP[CON]<F>: system rapidity and mass generation ranges, GENCUTS "<F>" of
+// tests/run_compare_PS/card.json; 8 dimensions: the B51 variables (pt_1, pt_2,
+// phi_1, phi_2, Y, M^2) and the isotropic decay angles (cos theta, phi) that
+// TwoBodyPhaseSpace draws (include/Graniitti/MKinematics.h:405-459), lifted
+void set_factorized(gra_cfg* c) {
+  c->phase_space = GRA_PS_F;
+  c->dim = 8;  // LIPSDIM = 5 + 1 (src/MFactorized.cc:57), + 2 decay angles
+  c->rap_min = -0.9;
+  c->rap_max = 0.9;
+  c->gen_M_min = 0.0;
+  c->gen_M_max = 4.0;
+}
+
 void set_rhorho(gra_cfg* c) {
   c->process = GRA_PROC_TENSOR_CON;
   c->dim = 14;  // 8 phase-space + 2 BW masses + 2 x (cos theta, phi) decay angles
@@ -159,6 +172,13 @@ extern "C" int gra_hip_cfg_default(gra_cfg* cfg, int32_t preset) {
       set_rhorho(cfg);
       cfg->screening = 1;
       cfg->amp_precision = GRA_AMP_FP32;
+      break;
+    case 4:
+    case 5:
+      set_pipi(cfg);
+      set_factorized(cfg);
+      cfg->screening = preset == 5;
+      cfg->amp_precision = preset == 5 ? GRA_AMP_FP32 : GRA_AMP_FP64;
       break;
     default:
       return set_global_error(GRA_E_ARG, "gra_hip_cfg_default: unknown preset %d", preset);

@@ -210,6 +210,7 @@ struct Kin {
   dv4 q1;
   double ss13, ss24, ss14, ss23;
   double t1, t2, t_hat, u_hat, m2, Y, Pt;
+  double dw;  // <F>: two-body phase-space weight of the central system (lts.DW)
 };
 
 // MProcess::GetLorentzScalars(Nf = 4), src/MProcess.cc:1499-1584. Only the
@@ -252,8 +253,58 @@ __device__ __forceinline__ dv4 lorentz_boost(dv4 b, double M0, dv4 p) {
   return v4m(p.x + kappa2 * bx, p.y + kappa2 * by, p.z + kappa2 * bz, gamma * (p.e + kappa1));
 }
 
+// MFactorized::B51RandomKin + B51BuildKin, src/MFactorized.cc:243-393, for a
+// central system decaying to two stable particles: x[0..4] as there, x[5] the
+// system's M^2 on [M_MIN^2, M_MAX^2], and the isotropic decay angles that
+// TwoBodyPhaseSpace draws (include/Graniitti/MKinematics.h:405-459) lifted to
+// x[6] (cos theta = -1 + 2 x[6]) and x[7] (phi = 2 pi x[7]).
+__device__ __forceinline__ bool build_kin_f(const RunParams& P, const double* x, Kin& K) {
+  const double u1 = P.lmin + (P.lmax - P.lmin) * x[0];
+  const double u2 = P.lmin + (P.lmax - P.lmin) * x[1];
+  const double pt1 = exp(u1), pt2 = exp(u2);
+  const double phi1 = 2.0 * GRA_PI * x[2], phi2 = 2.0 * GRA_PI * x[3];
+  const double yX = P.rap_min + (P.rap_max - P.rap_min) * x[4];
+  const double m2X = P.f_Mmin * P.f_Mmin + (P.f_Mmax * P.f_Mmax - P.f_Mmin * P.f_Mmin) * x[5];
+  const double m1 = P.m_beam, m2 = P.m_beam;
+  dv4 p1 = v4m(pt1 * cos(phi1), pt1 * sin(phi1), 0, 0);
+  dv4 p2 = v4m(pt2 * cos(phi2), pt2 * sin(phi2), 0, 0);
+  dv4 pX = v4m(-(p1.x + p2.x), -(p1.y + p2.y), 0, 0);
+  const double mtX = msq(m2X + pt2of(pX));
+  pX.z = mtX * sinh(yX);
+  pX.e = mtX * cosh(yX);
+  if (pX.e > P.sqrt_s - (m1 + m2)) return false;
+  const double p1z = solvepz(m1, m2, pt1, pt2, pX.z, pX.e, P);
+  const double p2z = -(pX.z + p1z);
+  if (p1z < 0 || p2z > 0) return false;
+  p1.z = p1z;
+  p1.e = msq(m1 * m1 + pt1 * pt1 + p1z * p1z);
+  p2.z = p2z;
+  p2.e = msq(m2 * m2 + pt2 * pt2 + p2z * p2z);
+  K.pf[1] = p1;
+  K.pf[2] = p2;
+  K.pf[0] = pX;
+  if (!check_emc(v4sub(P.beamsum, v4add(v4add(K.pf[1], K.pf[2]), K.pf[0])))) return false;
+  // TwoBodyPhaseSpace(pX, sqrt(m2X), {m, m}): DecayMomentum, Isotropic, boost, dPhi2
+  const double M0 = msq(m2X), ma = P.m_central, mb = P.m_central;
+  const double pn = msq((M0 - ma - mb) * (M0 + ma + mb) * (M0 + ma - mb) * (M0 - ma + mb)) /
+                    (2.0 * M0);
+  const double ct = -1.0 + 2.0 * x[6];
+  const double st = msq(1.0 - ct * ct);
+  const double ph = 2.0 * GRA_PI * x[7];
+  const double kx = pn * st * cos(ph), ky = pn * st * sin(ph), kz = pn * ct;
+  K.dt[0] = lorentz_boost(pX, M0, v4m(kx, ky, kz, msq(ma * ma + pn * pn)));
+  K.dt[1] = lorentz_boost(pX, M0, v4m(-kx, -ky, -kz, msq(mb * mb + pn * pn)));
+  const double V = 2.0 * 2.0 * GRA_PI;
+  K.dw = V * pn / (16.0 * (GRA_PI * GRA_PI) * M0);
+  if (K.dw < 0) return false;
+  K.pf[3] = K.dt[0];
+  K.pf[4] = K.dt[1];
+  return lorentz_scalars(P, K, true);
+}
+
 // MContinuum::BNRandomKin + BNBuildKin (Kf = 2), src/MContinuum.cc:276-435
 __device__ __forceinline__ bool build_kin(const RunParams& P, const double* x, Kin& K) {
+  if (P.phase_space == GRA_PS_F) return build_kin_f(P, x, K);
   const double u1 = P.lmin + (P.lmax - P.lmin) * x[0];
   const double u2 = P.lmin + (P.lmax - P.lmin) * x[1];
   const double pt1 = exp(u1), pt2 = exp(u2);
@@ -372,12 +423,14 @@ __device__ __forceinline__ bool fiducial_cuts(const RunParams& P, const Kin& K) 
 struct LoopPt {
   double p1x, p1y, p2x, p2y;  // tree proton pT vectors
   double pktx, pkty;
-  double sh[2], ch[2], mc2[2];  // sinh y, cosh y, m^2 of the central particles
+  double sh[2], ch[2], mc2[2];  // sinh y, cosh y, m^2 of the central particles (<F>: [0]
+                                // holds the central system's)
   double m1, m2;                // proton masses from pfinal_orig[].M()
   dv4 dt[2];
+  double dw;  // <F>: two-body phase-space weight (unchanged by the loop)
 };
 
-__device__ __forceinline__ void loop_point(const Kin& K, LoopPt& L) {
+__device__ __forceinline__ void loop_point(const RunParams& P, const Kin& K, LoopPt& L) {
   L.p1x = K.pf[1].x;
   L.p1y = K.pf[1].y;
   L.p2x = K.pf[2].x;
@@ -395,6 +448,43 @@ __device__ __forceinline__ void loop_point(const Kin& K, LoopPt& L) {
   }
   L.m1 = mof(K.pf[1]);
   L.m2 = mof(K.pf[2]);
+  L.dw = K.dw;
+  if (P.phase_space == GRA_PS_F) {
+    // MFactorized::LoopKinematics keeps the system's mass and rapidity
+    const double mX = mof(K.pf[0]), yX = rapof(K.pf[0]);
+    L.mc2[0] = mX * mX;
+    L.sh[0] = sinh(yX);
+    L.ch[0] = cosh(yX);
+  }
+}
+
+// MFactorized::LoopKinematics, src/MFactorized.cc:77-139: the system keeps its
+// mass and rapidity, takes pT = -(p1T + p2T), and the decay products (pfinal[3],
+// pfinal[4] and the decay tree) stay those of the tree point.
+__device__ __forceinline__ bool loop_kinematics_f(const RunParams& P, const LoopPt& L, double kx,
+                                                  double ky, Kin& K, bool cut_vars) {
+  const double p1x = L.p1x - kx, p1y = L.p1y - ky;
+  const double p2x = L.p2x + kx, p2y = L.p2y + ky;
+  const double pt1 = msq(p1x * p1x + p1y * p1y), pt2 = msq(p2x * p2x + p2y * p2y);
+  dv4 pX = v4m(-(p1x + p2x), -(p1y + p2y), 0, 0);
+  const double mtX = msq(L.mc2[0] + pt2of(pX));
+  pX.z = mtX * L.sh[0];
+  pX.e = mtX * L.ch[0];
+  if (pX.e > P.sqrt_s - (L.m1 + L.m2)) return false;
+  const double p1z = solvepz(L.m1, L.m2, pt1, pt2, pX.z, pX.e, P);
+  const double p2z = -(pX.z + p1z);
+  if (isnan(p1z)) return false;
+  if (p1z < 0 || p2z > 0) return false;
+  K.pf[1] = v4m(p1x, p1y, p1z, msq(L.m1 * L.m1 + pt1 * pt1 + p1z * p1z));
+  K.pf[2] = v4m(p2x, p2y, p2z, msq(L.m2 * L.m2 + pt2 * pt2 + p2z * p2z));
+  K.pf[0] = pX;
+  if (!check_emc(v4sub(P.beamsum, v4add(v4add(K.pf[1], K.pf[2]), K.pf[0])))) return false;
+  K.dt[0] = L.dt[0];
+  K.dt[1] = L.dt[1];
+  K.pf[3] = L.dt[0];
+  K.pf[4] = L.dt[1];
+  K.dw = L.dw;
+  return lorentz_scalars(P, K, cut_vars);
 }
 
 // MContinuum::LoopKinematics (Kf = 2), src/MContinuum.cc:94-161, for the node
@@ -402,6 +492,7 @@ __device__ __forceinline__ void loop_point(const Kin& K, LoopPt& L) {
 // into BLinearSystem only through their x, y components, which are set here.
 __device__ __forceinline__ bool loop_kinematics(const RunParams& P, const LoopPt& L, double kx,
                                                 double ky, Kin& K, bool cut_vars) {
+  if (P.phase_space == GRA_PS_F) return loop_kinematics_f(P, L, kx, ky, K, cut_vars);
   const double p1x = L.p1x - kx, p1y = L.p1y - ky;
   const double p2x = L.p2x + kx, p2y = L.p2y + ky;
   // BLinearSystem: b0 = q - w, b1 = -q - w, p_i = sum_j A_ij b_j
@@ -720,6 +811,18 @@ __device__ __forceinline__ double abs2d(cxd z) {
 // src/MContinuum.cc:164-193; BNPhaseSpaceWeight 639-656; BNIntegralVolume 619-636;
 // ForwardVolume src/MProcess.cc:1390-1415; MollerFlux = 2s).
 __device__ __forceinline__ double weight_from_amp2(const RunParams& P, const Kin& K, double amp2) {
+  if (P.phase_space == GRA_PS_F) {
+    // MFactorized::EventWeight, src/MFactorized.cc:142-182: C_space = DW / (2 pi)
+    // times CascadePS (1: stable decay products); B51PhaseSpaceWeight 441-452;
+    // B51IntegralVolume 434-438 with ForwardVolume
+    const double C_space = (K.dw / (2 * GRA_PI)) * 1.0;
+    const double J = 1.0 / fabs(K.pf[1].z / K.pf[1].e - K.pf[2].z / K.pf[2].e);
+    const double pt1 = ptof(K.pf[1]), pt2 = ptof(K.pf[2]);
+    const double psw = P.f_psw * (pt1 / (2.0 * K.pf[1].e)) * (pt2 / (2.0 * K.pf[2].e)) * J;
+    const double PT_vol = (pt1 * pt2) * P.lspan2;
+    const double vol = P.f_vol * (P.PHI_vol * PT_vol);
+    return C_space * P.invS * psw * vol * amp2 * P.GeV2barn / P.flux;
+  }
   const double J = 1.0 / fabs(K.pf[1].z / K.pf[1].e - K.pf[2].z / K.pf[2].e);
   const double PROD = 1.0 * ptof(K.pkt);
   const double pt1 = ptof(K.pf[1]), pt2 = ptof(K.pf[2]);

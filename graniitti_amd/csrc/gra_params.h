@@ -31,6 +31,11 @@ struct RunParams {
   double fwd_t_min, fwd_t_max;
   // weight constants
   double psw_c1, psw_c2, lspan2, KT_vol, PHI_vol, invS, GeV2barn, flux;
+  // <F> phase space (MFactorized B51): M_MIN, M_MAX of the system (B51RandomKin,
+  // src/MFactorized.cc:256-285), B51IntegralVolume's (M_MAX^2 - M_MIN^2)(Y_max - Y_min)
+  // and B51PhaseSpaceWeight's (1/2)(2 pi)^-5
+  int32_t phase_space, pad_ps;
+  double f_Mmin, f_Mmax, f_vol, f_psw;
   // Regge amplitude
   int32_t regge_on[3];
   int32_t regge_k;  // index of the single active trajectory, -1 if several

@@ -42,6 +42,10 @@ typedef struct gra_handle gra_handle; /* opaque: device state of one process */
 #define GRA_E_NUMERIC (-5) /* too many NaN/Inf weights (src/MProcess.h:317-340) */
 #define GRA_E_IO (-6)      /* file missing, unreadable or malformed */
 
+/* Phase-space classes */
+#define GRA_PS_C 0 /* <C>: MContinuum, 2 -> 4 in (pt_i, phi_i, kt, phi_kt, y_3, y_4) */
+#define GRA_PS_F 1 /* <F>: MFactorized B51, system (M^2, Y) x isotropic 2-body decay */
+
 /* ------------------------------------------------------------------------- */
 /* Limits                                                                    */
 /* ------------------------------------------------------------------------- */
@@ -118,6 +122,14 @@ typedef struct gra_cfg {
   int32_t n_kt2;      /* NumberKT2: screening table intervals */
   int32_t n_fb;       /* FBIntegralN: Fourier-Bessel quadrature intervals (even) */
   double min_loop_kt, max_loop_kt;
+  /* Phase-space class, the <C> / <F> suffix of the process string (MProcess
+   * classes MContinuum / MFactorized). With GRA_PS_F, rap_min/rap_max bound the
+   * central system's rapidity and gen_M_min/gen_M_max its mass (GENCUTS "<F>",
+   * e.g. tests/run_compare_PS/card.json); the reference has no default for the
+   * mass range (GENCUT::M_max = 0, include/Graniitti/MKinematics.h:1565). */
+  int32_t phase_space; /* GRA_PS_C or GRA_PS_F */
+  int32_t pad_ps;
+  double gen_M_min, gen_M_max;
 } gra_cfg;
 
 /* ------------------------------------------------------------------------- */
@@ -233,6 +245,9 @@ typedef struct {
  *   1: PP[CON]<C> -> pi+ pi-, 13 TeV, |eta|<0.9, pT>0.15, no screening, FP64
  *   2: preset 1 + eikonal screening (16 phi x 31 kt nodes, MaxLoopKT 3), FP32 amplitude
  *   3: PP[CONTENSOR]<C> -> rho0 rho0 -> 4 pi, |eta|<2.5, pT>0.1, screening, FP32
+ *   4: PP[CON]<F> -> pi+ pi-, preset 1's cuts, system |Y| < 0.9 and M in [0, 4] GeV
+ *      (GENCUTS "<F>" of tests/run_compare_PS/card.json), no screening, FP64
+ *   5: preset 4 + eikonal screening as preset 2, FP32 amplitude
  * Replaces the steering-card reader MGraniitti::ReadInput (src/MGraniitti.cc:318-329)
  * for the keys these workloads use, with TUNE0 values (modeldata/TUNE0/GENERAL.json). */
 int gra_hip_cfg_default(gra_cfg* cfg, int32_t preset);

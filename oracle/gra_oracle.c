@@ -332,6 +332,8 @@ typedef struct {
   double *mbt_x, *msa_x; /* n+1 */
   cx *mbt_y, *msa_y;
   int tables_ready;
+  /* <F>: system mass range, src/MFactorized.cc:259-285 */
+  double f_Mmin, f_Mmax;
 } ora_run;
 
 /* MParticle masses and units, include/Graniitti/MPDG.h:54-78 */
@@ -441,6 +443,7 @@ typedef struct {
   double ss[5][5];
   double s1, s2, t1, t2, t_hat, u_hat, m2, Y, Pt;
   double m_off[2];
+  double dw; /* <F>: lts.DW, the two-body phase-space weight */
 } kin_t;
 
 /* MProcess::GetLorentzScalars(Nf=4), src/MProcess.cc:1499-1584 (fields used by
@@ -499,7 +502,7 @@ static v4 lorentz_boost(v4 boost, double M0, v4 p) {
 /* MContinuum::BNRandomKin + BNBuildKin for Kf = 2 central particles,
  * src/MContinuum.cc:276-435. Hidden variables of the rho decays (config 3) are
  * taken from x[8..13] (see the tensor section). Returns kinematics_ok. */
-static int build_kin(const ora_run* R, const double* x, kin_t* K) {
+static int build_kin_c(const ora_run* R, const double* x, kin_t* K) {
   const gra_cfg* c = &R->c;
   const double ZERO_EPS = 1e-12;
   const double lmin = log(c->fwd_pt_min + ZERO_EPS), lmax = log(c->fwd_pt_max);
@@ -588,6 +591,105 @@ static int build_kin(const ora_run* R, const double* x, kin_t* K) {
   return lorentz_scalars(R, K);
 }
 
+/* MFactorized::B51RandomKin + B51BuildKin, src/MFactorized.cc:243-393, system
+ * decaying to two stable particles (pions at their pole mass). x[0..4] as there,
+ * x[5] the system's M^2 uniform on [M_MIN^2, M_MAX^2] (268); TwoBodyPhaseSpace
+ * (include/Graniitti/MKinematics.h:443-459) with its FlatIsotropic draws lifted:
+ * cos theta = U(-1, 1) from x[6], phi = U(0, 2 pi) from x[7] (405-409;
+ * MRandom::U(a, b) = a + (b - a) u, src/MRandom.cc:61-64); DecayMomentum 294-297;
+ * dPhi2 267-271. */
+static int build_kin_f(const ora_run* R, const double* x, kin_t* K) {
+  const gra_cfg* c = &R->c;
+  const double ZERO_EPS = 1e-12;
+  const double lmin = log(c->fwd_pt_min + ZERO_EPS), lmax = log(c->fwd_pt_max);
+  const double u1 = lmin + (lmax - lmin) * x[0];
+  const double u2 = lmin + (lmax - lmin) * x[1];
+  const double pt1 = exp(u1), pt2 = exp(u2);
+  const double phi1 = 2.0 * PI * x[2], phi2 = 2.0 * PI * x[3];
+  const double yX = c->rap_min + (c->rap_max - c->rap_min) * x[4];
+  const double Mn = R->f_Mmin, Mx = R->f_Mmax;
+  const double m2X = Mn * Mn + (Mx * Mx - Mn * Mn) * x[5];
+  const double m1 = c->m_beam, m2 = c->m_beam;
+  v4 p1 = v4_make(pt1 * cos(phi1), pt1 * sin(phi1), 0, 0);
+  v4 p2 = v4_make(pt2 * cos(phi2), pt2 * sin(phi2), 0, 0);
+  v4 pX = v4_make(-(p1.x + p2.x), -(p1.y + p2.y), 0, 0);
+  const double mtX = msqrt(m2X + v4_pt2(pX));
+  pX.z = mtX * sinh(yX);
+  pX.e = mtX * cosh(yX);
+  if (pX.e > R->sqrt_s - (m1 + m2)) return 0;
+  const double p1z = ora_solvepz(m1, m2, pt1, pt2, pX.z, pX.e, R->s);
+  const double p2z = -(pX.z + p1z);
+  if (p1z < 0 || p2z > 0) return 0;
+  p1.z = p1z;
+  p1.e = msqrt(m1 * m1 + pt1 * pt1 + p1z * p1z);
+  p2.z = p2z;
+  p2.e = msqrt(m2 * m2 + pt2 * pt2 + p2z * p2z);
+  K->pf[1] = p1;
+  K->pf[2] = p2;
+  K->pf[0] = pX;
+  if (!check_emc(v4_sub(R->beamsum, v4_add(v4_add(K->pf[1], K->pf[2]), K->pf[0])))) return 0;
+  const double M0 = msqrt(m2X), ma = c->m_central, mb = c->m_central;
+  const double pn = msqrt((M0 - ma - mb) * (M0 + ma + mb) * (M0 + ma - mb) * (M0 - ma + mb)) /
+                    (2.0 * M0);
+  const double costh = -1.0 + (1.0 - (-1.0)) * x[6];
+  const double sinth = msqrt(1.0 - costh * costh);
+  const double phi = 0.0 + (2.0 * PI - 0.0) * x[7];
+  const double k[3] = {pn * sinth * cos(phi), pn * sinth * sin(phi), pn * costh};
+  v4 d0 = v4_make(k[0], k[1], k[2], msqrt(ma * ma + pn * pn));
+  v4 d1 = v4_make(-k[0], -k[1], -k[2], msqrt(mb * mb + pn * pn));
+  K->dt[0] = lorentz_boost(pX, M0, d0);
+  K->dt[1] = lorentz_boost(pX, M0, d1);
+  const double V = 2.0 * 2.0 * PI;
+  K->dw = V * pn / (16.0 * (PI * PI) * M0);
+  if (K->dw < 0) return 0;
+  K->m_off[0] = ma;
+  K->m_off[1] = mb;
+  K->pf[3] = K->dt[0];
+  K->pf[4] = K->dt[1];
+  return lorentz_scalars(R, K);
+}
+
+/* MFactorized::LoopKinematics, src/MFactorized.cc:77-139: the system keeps
+ * pfinal_orig[0]'s mass and rapidity with pT = -(p1T + p2T); the decay products
+ * stay those of the tree point */
+static int loop_kinematics_f(const ora_run* R, kin_t* K, const double p1p[2], const double p2p[2]) {
+  K->pf[1].x = p1p[0];
+  K->pf[1].y = p1p[1];
+  K->pf[2].x = p2p[0];
+  K->pf[2].y = p2p[1];
+  const double pt1 = v4_pt(K->pf[1]), pt2 = v4_pt(K->pf[2]);
+  const double m1 = v4_m(K->pf_orig[1]), m2 = v4_m(K->pf_orig[2]);
+  const double mX = v4_m(K->pf_orig[0]), yX = v4_rap(K->pf_orig[0]);
+  K->pf[0].x = -(K->pf[1].x + K->pf[2].x);
+  K->pf[0].y = -(K->pf[1].y + K->pf[2].y);
+  const double mtX = msqrt(mX * mX + v4_pt2(K->pf[0]));
+  K->pf[0].z = mtX * sinh(yX);
+  K->pf[0].e = mtX * cosh(yX);
+  if (K->pf[0].e > R->sqrt_s - (m1 + m2)) return 0;
+  const double p1z = ora_solvepz(m1, m2, pt1, pt2, K->pf[0].z, K->pf[0].e, R->s);
+  const double p2z = -(K->pf[0].z + p1z);
+  if (isnan(p1z)) return 0;
+  if (p1z < 0 || p2z > 0) return 0;
+  K->pf[1].z = p1z;
+  K->pf[1].e = msqrt(m1 * m1 + pt1 * pt1 + p1z * p1z);
+  K->pf[2].z = p2z;
+  K->pf[2].e = msqrt(m2 * m2 + pt2 * pt2 + p2z * p2z);
+  if (!check_emc(v4_sub(R->beamsum, v4_add(v4_add(K->pf[1], K->pf[2]), K->pf[0])))) return 0;
+  return lorentz_scalars(R, K);
+}
+
+/* The phase-space class's kinematics: <C> (MContinuum) or <F> (MFactorized) */
+static int build_kin(const ora_run* R, const double* x, kin_t* K) {
+  return R->c.phase_space == GRA_PS_F ? build_kin_f(R, x, K) : build_kin_c(R, x, K);
+}
+
+/* lts.DW of the <F> point x (-1 if its kinematics fail) */
+ORA_EXPORT double ora_two_body_weight(ora_run* R, const double* x) {
+  kin_t K;
+  memset(&K, 0, sizeof(K));
+  return build_kin_f(R, x, &K) ? K.dw : -1.0;
+}
+
 /* Fiducial check of one final-state leg, MProcess::FindDecayCuts, src/MProcess.cc:1138-1154 */
 static int leg_cuts(const gra_cfg* c, v4 p) {
   const double pt = v4_pt(p), et = v4_mt(p), eta = v4_eta(p), rap = v4_rap(p);
@@ -621,6 +723,7 @@ static int fiducial_cuts(const ora_run* R, const kin_t* K) {
 
 /* MContinuum::LoopKinematics for Kf = 2, src/MContinuum.cc:94-161 */
 static int loop_kinematics(const ora_run* R, kin_t* K, const double p1p[2], const double p2p[2]) {
+  if (R->c.phase_space == GRA_PS_F) return loop_kinematics_f(R, K, p1p, p2p);
   K->pf[1].x = p1p[0];
   K->pf[1].y = p1p[1];
   K->pf[2].x = p2p[0];
@@ -967,6 +1070,26 @@ static double event_weight(ora_run* R, const double* x, double* amp2_out, uint8_
   fl |= GRA_FLAG_VETO_OK;
   if (kin_ok && fid_ok) {
     amp2 = screened_amp2(R, &K);
+    if (c->phase_space == GRA_PS_F) {
+      /* MFactorized::EventWeight, src/MFactorized.cc:142-182: C_space = DW / (2 pi)
+       * times CascadePS (1 for stable products); B51PhaseSpaceWeight 441-452 with
+       * pow5(2 pi) = x pow2(pow2(x)) (include/Graniitti/MMath.h:162-176);
+       * B51IntegralVolume 434-438 with ForwardVolume (src/MProcess.cc:1390-1400) */
+      const double C_space = (K.dw / (2 * PI)) * 1.0;
+      const double J = 1.0 / fabs(K.pf[1].z / K.pf[1].e - K.pf[2].z / K.pf[2].e);
+      const double tp = 2.0 * PI;
+      const double psw = (1.0 / 2.0) * (1.0 / (tp * ((tp * tp) * (tp * tp)))) *
+                         (v4_pt(K.pf[1]) / (2.0 * K.pf[1].e)) *
+                         (v4_pt(K.pf[2]) / (2.0 * K.pf[2].e)) * J;
+      const double ZERO_EPS = 1e-12;
+      const double PHI_vol = (2.0 * PI) * (2.0 * PI);
+      const double J_pt = v4_pt(K.pf[1]) * v4_pt(K.pf[2]);
+      const double lspan = log(c->fwd_pt_max) - log(c->fwd_pt_min + ZERO_EPS);
+      const double fwd = PHI_vol * (J_pt * (lspan * lspan));
+      const double Mn = R->f_Mmin, Mx = R->f_Mmax;
+      const double vol = (Mx * Mx - Mn * Mn) * (c->rap_max - c->rap_min) * fwd;
+      W = C_space * (1.0 / c->S_factor) * psw * vol * amp2 * R->GeV2barn / (2.0 * R->s);
+    } else {
     const double C_space = 1.0;
     const double J = 1.0 / fabs(K.pf[1].z / K.pf[1].e - K.pf[2].z / K.pf[2].e);
     const double PROD = 1.0 * v4_pt(K.pkt);
@@ -984,6 +1107,7 @@ static double event_weight(ora_run* R, const double* x, double* amp2_out, uint8_
     KT_vol *= pow(c->rap_max - c->rap_min, 2);
     const double vol = KT_vol * (PHI_vol * PT_vol);
     W = C_space * (1.0 / c->S_factor) * psw * vol * amp2 * R->GeV2barn / (2.0 * R->s);
+    }
   }
   if (isnan(W) || isinf(W)) {
     W = 0;
@@ -1013,6 +1137,12 @@ ORA_EXPORT ora_run* ora_create(const gra_cfg* cfg, int nthreads) {
   R->nkt = cfg->n_loop_kt;
   R->step_kt = (cfg->max_loop_kt - cfg->min_loop_kt) / cfg->n_loop_kt;
   R->step_phi = (2.0 * PI - 0.0) / cfg->n_loop_phi;
+  if (cfg->phase_space == GRA_PS_F) {
+    /* M_MIN = sum of decay-tree masses + 1e-4 GeV, M_MAX = sqrt(s) - beam masses,
+     * clipped to the GENCUTS (src/MFactorized.cc:259-272) */
+    R->f_Mmin = fmax((0.0 + cfg->m_central) + cfg->m_central + 1e-4, cfg->gen_M_min);
+    R->f_Mmax = fmin(R->sqrt_s - (v4_m(R->pbeam1) + v4_m(R->pbeam2)), cfg->gen_M_max);
+  }
   if (cfg->screening) {
     R->n_bt = cfg->n_bt;
     R->n_kt2 = cfg->n_kt2;
