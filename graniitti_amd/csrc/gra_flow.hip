@@ -820,9 +820,10 @@ __global__ void k_coef(const double* __restrict__ w, int n, double inv_s2, float
 // Only u_{l-1}, log q and, when the layer's activations are needed for the
 // backward pass, the activations and spline parameters are written to HBM.
 // LDS holds two activation tiles; the FP32 parameter tile of the output
-// product overlays the one that product does not read. R = 32 rows per block
-// (GRA_FUSED_ROWS_256 = 16 halves the tile at H = 256 for two blocks per CU,
-// which measured slower: the B fragments are then read twice as often).
+// product overlays the one that product does not read. R = 64 rows per block
+// below H = 256 and 32 at H = 256 (GRA_FUSED_ROWS_256 = 16 halves the tile there
+// for two blocks per CU, which measured slower: the B fragments are then read
+// twice as often).
 // ---------------------------------------------------------------------------
 #ifndef GRA_FUSED_ROWS_256
 #define GRA_FUSED_ROWS_256 32
@@ -833,7 +834,15 @@ __global__ void k_coef(const double* __restrict__ w, int n, double inv_s2, float
 #ifndef GRA_FUSED_THREADS
 #define GRA_FUSED_THREADS 256
 #endif
-__host__ __device__ constexpr int fused_rows(int H) { return H == 256 ? GRA_FUSED_ROWS_256 : 32; }
+// Below H = 256 a block takes 64 rows: each B fragment read through L2 then
+// feeds four MFMA row tiles instead of two (+10% sampling rate at H = 128
+// against 32 rows; 128 rows is slower, at one block per CU)
+#ifndef GRA_FUSED_ROWS
+#define GRA_FUSED_ROWS 64
+#endif
+__host__ __device__ constexpr int fused_rows(int H) {
+  return H == 256 ? GRA_FUSED_ROWS_256 : GRA_FUSED_ROWS;
+}
 // Threads per block: more waves share the block's LDS tiles and split its
 // n-tiles, so each SIMD has several waves to switch between while the B
 // fragments are in flight (one wave per SIMD leaves it waiting most of the time)
