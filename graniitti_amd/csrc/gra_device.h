@@ -253,6 +253,15 @@ __device__ __forceinline__ dv4 lorentz_boost(dv4 b, double M0, dv4 p) {
   return v4m(p.x + kappa2 * bx, p.y + kappa2 * by, p.z + kappa2 * bz, gamma * (p.e + kappa1));
 }
 
+// Phase-space class of a kernel: GRA_PS_C or GRA_PS_F fixed at compile time
+// (each instantiation then holds only its own kinematics), or kPsAny, read
+// from the run's parameters
+constexpr int kPsAny = -1;
+template <int PS>
+__device__ __forceinline__ bool ps_f(const RunParams& P) {
+  return PS == kPsAny ? P.phase_space == GRA_PS_F : PS == GRA_PS_F;
+}
+
 // MFactorized::B51RandomKin + B51BuildKin, src/MFactorized.cc:243-393, for a
 // central system decaying to two stable particles: x[0..4] as there, x[5] the
 // system's M^2 on [M_MIN^2, M_MAX^2], and the isotropic decay angles that
@@ -303,8 +312,9 @@ __device__ __forceinline__ bool build_kin_f(const RunParams& P, const double* x,
 }
 
 // MContinuum::BNRandomKin + BNBuildKin (Kf = 2), src/MContinuum.cc:276-435
+template <int PS = kPsAny>
 __device__ __forceinline__ bool build_kin(const RunParams& P, const double* x, Kin& K) {
-  if (P.phase_space == GRA_PS_F) return build_kin_f(P, x, K);
+  if (ps_f<PS>(P)) return build_kin_f(P, x, K);
   const double u1 = P.lmin + (P.lmax - P.lmin) * x[0];
   const double u2 = P.lmin + (P.lmax - P.lmin) * x[1];
   const double pt1 = exp(u1), pt2 = exp(u2);
@@ -430,6 +440,7 @@ struct LoopPt {
   double dw;  // <F>: two-body phase-space weight (unchanged by the loop)
 };
 
+template <int PS = kPsAny>
 __device__ __forceinline__ void loop_point(const RunParams& P, const Kin& K, LoopPt& L) {
   L.p1x = K.pf[1].x;
   L.p1y = K.pf[1].y;
@@ -449,7 +460,7 @@ __device__ __forceinline__ void loop_point(const RunParams& P, const Kin& K, Loo
   L.m1 = mof(K.pf[1]);
   L.m2 = mof(K.pf[2]);
   L.dw = K.dw;
-  if (P.phase_space == GRA_PS_F) {
+  if (ps_f<PS>(P)) {
     // MFactorized::LoopKinematics keeps the system's mass and rapidity
     const double mX = mof(K.pf[0]), yX = rapof(K.pf[0]);
     L.mc2[0] = mX * mX;
@@ -490,9 +501,10 @@ __device__ __forceinline__ bool loop_kinematics_f(const RunParams& P, const Loop
 // MContinuum::LoopKinematics (Kf = 2), src/MContinuum.cc:94-161, for the node
 // with loop momentum (kx, ky). pf[1], pf[2] carry E and z of the previous node
 // into BLinearSystem only through their x, y components, which are set here.
+template <int PS = kPsAny>
 __device__ __forceinline__ bool loop_kinematics(const RunParams& P, const LoopPt& L, double kx,
                                                 double ky, Kin& K, bool cut_vars) {
-  if (P.phase_space == GRA_PS_F) return loop_kinematics_f(P, L, kx, ky, K, cut_vars);
+  if (ps_f<PS>(P)) return loop_kinematics_f(P, L, kx, ky, K, cut_vars);
   const double p1x = L.p1x - kx, p1y = L.p1y - ky;
   const double p2x = L.p2x + kx, p2y = L.p2y + ky;
   // BLinearSystem: b0 = q - w, b1 = -q - w, p_i = sum_j A_ij b_j
@@ -810,8 +822,9 @@ __device__ __forceinline__ double abs2d(cxd z) {
 // Event weight from the screened |M|^2 (MContinuum::EventWeight,
 // src/MContinuum.cc:164-193; BNPhaseSpaceWeight 639-656; BNIntegralVolume 619-636;
 // ForwardVolume src/MProcess.cc:1390-1415; MollerFlux = 2s).
+template <int PS = kPsAny>
 __device__ __forceinline__ double weight_from_amp2(const RunParams& P, const Kin& K, double amp2) {
-  if (P.phase_space == GRA_PS_F) {
+  if (ps_f<PS>(P)) {
     // MFactorized::EventWeight, src/MFactorized.cc:142-182: C_space = DW / (2 pi)
     // times CascadePS (1: stable decay products); B51PhaseSpaceWeight 441-452;
     // B51IntegralVolume 434-438 with ForwardVolume
