@@ -110,7 +110,7 @@ def test_config2_fp32_screened_error():
     with MContinuum(cfg) as P:
         Wg, ag, fg = P.eval(u)
     Wo, ao, fo = ol.OracleRun(screened_cfg(abi.GRA_AMP_FP64)).eval(u)
-    # FP32 loop nodes, FP32 row sums, FP64 sum of rows: measured max 5.5e-6, median 3.9e-7
+    # FP32 loop nodes, FP32 row sums, FP64 sum of rows: measured max 6.9e-6, median 3.8e-7
     rel = compare(Wg, Wo, fg, fo, 1e-4, "cfg2 fp32 screened")
     say(f"cfg2 fp32 |M|^2 rel err: median {np.median(rel):.2e} p99 {np.quantile(rel, 0.99):.2e}")
     assert np.median(rel) < 4e-6
