@@ -850,7 +850,16 @@ __host__ __device__ constexpr int fused_threads(int H) {
   return H == 256 ? GRA_FUSED_THREADS_256 : GRA_FUSED_THREADS;
 }
 // Minimum waves per SIMD the register allocation must allow: two blocks per CU
-__host__ __device__ constexpr int fused_waves(int H) { return 2 * fused_threads(H) / 256; }
+// Blocks per CU the register allocation must allow: two at H = 256; three
+// below (168 VGPRs, a 40-byte spill outside the products), with the LDS tile
+// kept to a third of the CU (fused_lds_target)
+#ifndef GRA_FUSED_BLOCKS
+#define GRA_FUSED_BLOCKS 3
+#endif
+__host__ __device__ constexpr int fused_blocks(int H) { return H == 256 ? 2 : GRA_FUSED_BLOCKS; }
+__host__ __device__ constexpr int fused_waves(int H) {
+  return fused_blocks(H) * fused_threads(H) / 256;
+}
 
 struct FusedLayer {
   const float* W1;            // [dA x H]
@@ -1392,8 +1401,11 @@ static size_t fused_lds_bytes(const FlowState* F, int ldg) {
 // Output groups of the fused layer for a layer with dB transformed dimensions:
 // the group of gd dimensions [j0, j1) computes the n-tiles covering columns
 // [j0 np, j1 np); the tile stride is that of the widest group. Without a
-// requested size the largest group that keeps two blocks per CU is taken.
-constexpr size_t kFusedLdsTarget = 80 * 1024 - 512;
+// requested size the largest group that keeps fused_blocks(H) blocks per CU is taken.
+// LDS per block that lets fused_blocks(H) blocks share a CU's 160 KB
+static size_t fused_lds_target(int H) {
+  return (size_t)(160 * 1024 / fused_blocks(H)) - 512;
+}
 static void fused_groups(const FlowState* F, int dB, int want, int& gd, int& ldg, size_t& lds) {
   const int np = 3 * F->K - 1;
   auto stride = [&](int g) {
@@ -1406,7 +1418,7 @@ static void fused_groups(const FlowState* F, int dB, int want, int& gd, int& ldg
   };
   gd = want > 0 ? std::min(want, dB) : dB;
   if (want <= 0)
-    while (gd > 1 && fused_lds_bytes(F, stride(gd)) > kFusedLdsTarget) --gd;
+    while (gd > 1 && fused_lds_bytes(F, stride(gd)) > fused_lds_target(F->H)) --gd;
   ldg = stride(gd);
   lds = fused_lds_bytes(F, ldg);
 }
