@@ -612,6 +612,54 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const void* __restrict__ Av, in
       }
     }
   }
+  if (BF16 && (ep.ldc & 7) == 0 && (ep.ldm & 7) == 0) {
+    // the wave's 16 x 64 tile through LDS, so that each lane then reads its
+    // mask and writes its outputs as 16-byte vectors: lane -> row lane / 4,
+    // columns 16 (lane % 4) .. + 15 (full 128-byte row segments)
+    __shared__ float sT[4][16][64 + 4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) sT[wave][4 * (lane >> 4) + r][16 * t + (lane & 15)] = acc[t][r];
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int rr = lane >> 2, c0 = 16 * (lane & 3);
+    const size_t row = (size_t)(m0 + rr);
+    const int col0 = n0 + c0;
+    float v[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+      v[c] = sT[wave][rr][c0 + c];
+      if (ep.bias) v[c] += ep.bias[col0 + c];
+      if (ep.relu) v[c] = fmaxf(v[c], 0.f);
+    }
+    if (ep.mask) {
+      bf16x8 mk[2];
+      mk[0] = *(const bf16x8*)(ep.mask + row * ep.ldm + col0);
+      mk[1] = *(const bf16x8*)(ep.mask + row * ep.ldm + col0 + 8);
+#pragma unroll
+      for (int c = 0; c < 16; ++c)
+        if (!((float)mk[c >> 3][c & 7] > 0.f)) v[c] = 0.f;
+    }
+#pragma unroll
+    for (int c = 0; c < 16; ++c)
+      if (col0 + c >= ep.ncols_valid) v[c] = 0.f;
+    if (ep.out16) {
+      bf16x8 o[2];
+#pragma unroll
+      for (int c = 0; c < 16; ++c) o[c >> 3][c & 7] = (__bf16)v[c];
+      *(bf16x8*)(ep.out16 + row * ep.ldc + col0) = o[0];
+      *(bf16x8*)(ep.out16 + row * ep.ldc + col0 + 8) = o[1];
+    }
+    if (ep.out32) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        *(f32x4*)(ep.out32 + row * ep.ldc + col0 + 4 * q) =
+            (f32x4){v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]};
+    }
+    return;
+  }
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     const int col = n0 + 16 * t + (lane & 15);
