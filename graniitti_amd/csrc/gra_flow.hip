@@ -553,15 +553,30 @@ __global__ __launch_bounds__(256) void k_mlp_in_dx(const float* __restrict__ dh,
 
 // Sum of the per-chunk weight-gradient partials: rows < kw go to gW[row][col],
 // row kw (the all-ones input row) to the bias gradient gb[col].
-__global__ void k_part_reduce(const float* __restrict__ part, int nchunk, int kin_t, int N, int kw,
-                              float* __restrict__ gW, float* __restrict__ gb) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (kw + 1) * N) return;
-  const int row = idx / N, col = idx % N;
+// Fixed-order sum of the per-chunk partials: 32 outputs per block of 256 threads;
+// group g of 8 sums chunks g, g + 8, g + 16, ... in order, and the 8 group sums
+// are then added in order (deterministic, independent of the schedule)
+constexpr int kReduceGroups = 8;
+__global__ __launch_bounds__(256) void k_part_reduce(const float* __restrict__ part, int nchunk,
+                                                     int kin_t, int N, int kw,
+                                                     float* __restrict__ gW,
+                                                     float* __restrict__ gb) {
+  __shared__ float s[kReduceGroups][33];
+  const int o = threadIdx.x & 31, g = threadIdx.x >> 5;
+  const int idx = blockIdx.x * 32 + o;
+  const bool on = idx < (kw + 1) * N;
+  const int row = on ? idx / N : 0, col = on ? idx % N : 0;
   float acc = 0.f;
-  for (int k = 0; k < nchunk; ++k) acc += part[((size_t)k * kin_t + row) * N + col];
-  if (row < kw) gW[(size_t)row * N + col] = acc;
-  else gb[col] = acc;
+  if (on)
+    for (int k = g; k < nchunk; k += kReduceGroups) acc += part[((size_t)k * kin_t + row) * N + col];
+  s[g][o] = acc;
+  __syncthreads();
+  if (g != 0 || !on) return;
+  float t = s[0][o];
+#pragma unroll
+  for (int j = 1; j < kReduceGroups; ++j) t += s[j][o];
+  if (row < kw) gW[(size_t)row * N + col] = t;
+  else gb[col] = t;
 }
 
 // Dense product C[M x N] = A[M x K] * Bt[N x K]^T with row-major A and Bt,
@@ -1352,7 +1367,7 @@ static void weight_grad(gra_handle* h, FlowState* F, const void* X, int ldx, con
     hipLaunchKernelGGL(k_gemm_tn_part<true>, grid, dim3(256), 0, h->stream, X, ldx, Y, ldy, Kin, N,
                        n, chunk, F->part.as<float>(), nullptr, 0, none);
   const int tot = (Kin + 1) * N;
-  hipLaunchKernelGGL(k_part_reduce, dim3((tot + 255) / 256), dim3(256), 0, h->stream,
+  hipLaunchKernelGGL(k_part_reduce, dim3((tot + 31) / 32), dim3(256), 0, h->stream,
                      F->part.as<float>(), nch, Kin + 64, N, Kin, gW, gb);
 }
 
@@ -1369,7 +1384,7 @@ static void input_weight_grad(gra_handle* h, FlowState* F, const float* u, const
     hipLaunchKernelGGL(k_gemm_tn_part<true>, grid, dim3(256), 0, h->stream, nullptr, 0, Y, ldy, 64, N,
                        n, chunk, F->part.as<float>(), u, F->D, idx);
   const int tot = (idx.dA + 1) * N;
-  hipLaunchKernelGGL(k_part_reduce, dim3((tot + 255) / 256), dim3(256), 0, h->stream,
+  hipLaunchKernelGGL(k_part_reduce, dim3((tot + 31) / 32), dim3(256), 0, h->stream,
                      F->part.as<float>(), nch, 64, N, idx.dA, gW1, gb1);
 }
 
