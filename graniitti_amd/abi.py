@@ -18,6 +18,10 @@ GRA_MAX_DIM = 16
 GRA_VEGAS_BINS = 128
 GRA_PROC_REGGE_CON = 15
 GRA_PROC_TENSOR_CON = 12
+GRA_PROC_RES_CON = 17
+GRA_MAX_RES = 8
+GRA_BW_FIXED, GRA_BW_RUNNING, GRA_BW_BARRIER, GRA_BW_SPIN = 1, 2, 3, 4
+GRA_FRAME_CM, GRA_FRAME_HX, GRA_FRAME_CS, GRA_FRAME_GJ = 0, 1, 2, 3
 GRA_AMP_FP64 = 0
 GRA_AMP_FP32 = 1
 GRA_PS_C, GRA_PS_F = 0, 1
@@ -26,6 +30,14 @@ FLAG_KIN_OK, FLAG_FID_OK, FLAG_VETO_OK, FLAG_AMP_OK = 1, 2, 4, 8
 D = C.c_double
 I32 = C.c_int32
 U64 = C.c_uint64
+
+
+class GraRes(C.Structure):
+    _fields_ = [
+        ("mass", D), ("width", D), ("g_A", D), ("g_phi", D), ("g_FF", D), ("BR", D),
+        ("alpha_re", D), ("alpha_im", D), ("rho_re", D * 25), ("rho_im", D * 25),
+        ("J", I32), ("P", I32), ("BW", I32), ("pdg", I32),
+    ]
 
 
 class GraCfg(C.Structure):
@@ -52,6 +64,8 @@ class GraCfg(C.Structure):
         ("n_kt2", I32), ("n_fb", I32),
         ("min_loop_kt", D), ("max_loop_kt", D),
         ("phase_space", I32), ("pad_ps", I32), ("gen_M_min", D), ("gen_M_max", D),
+        ("n_res", I32), ("spin_gen", I32), ("spin_dec", I32), ("spin_frame", I32),
+        ("regge_omega", D), ("res", GraRes * GRA_MAX_RES),
     ]
 
 

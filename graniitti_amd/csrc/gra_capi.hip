@@ -153,8 +153,34 @@ int validate(const gra_cfg* c) {
   if (c->abi_size != (int32_t)sizeof(gra_cfg))
     return set_global_error(GRA_E_ARG, "gra_cfg.abi_size = %d, library expects %d", c->abi_size,
                             (int)sizeof(gra_cfg));
-  if (c->process != GRA_PROC_REGGE_CON && c->process != GRA_PROC_TENSOR_CON)
+  if (c->process != GRA_PROC_REGGE_CON && c->process != GRA_PROC_TENSOR_CON &&
+      c->process != GRA_PROC_RES_CON)
     return set_global_error(GRA_E_ARG, "unknown process %d", c->process);
+  if (c->process == GRA_PROC_RES_CON) {
+    if (c->phase_space != GRA_PS_C || c->w_central > 0.0 || c->m_daughter > 0.0)
+      return set_global_error(GRA_E_ARG,
+                              "PP[RES+CON]: built for the <C> class with stable central particles");
+    if (c->amp_precision != GRA_AMP_FP64)
+      return set_global_error(GRA_E_ARG, "PP[RES+CON]: resonance amplitudes run in FP64 only");
+    if (c->n_res < 0 || c->n_res > GRA_MAX_RES)
+      return set_global_error(GRA_E_ARG, "n_res %d outside 0..%d", c->n_res, GRA_MAX_RES);
+    if (c->spin_frame < GRA_FRAME_CM || c->spin_frame > GRA_FRAME_GJ)
+      return set_global_error(GRA_E_ARG, "spin_frame %d unknown", c->spin_frame);
+    for (int r = 0; r < c->n_res; ++r) {
+      const gra_res& q = c->res[r];
+      if (q.J < 0 || q.J > 2 || q.P != ((q.J & 1) ? -1 : 1))
+        return set_global_error(GRA_E_ARG,
+                                "resonance %d: J^P = %d^%d; built for 0+, 1-, 2+ (decays to two "
+                                "spinless particles)", r, q.J, q.P);
+      if (q.BW < GRA_BW_FIXED || q.BW > GRA_BW_SPIN)
+        return set_global_error(GRA_E_ARG, "resonance %d: BW %d unknown", r, q.BW);
+      if (!(q.mass > 2.0 * c->m_central) || !(q.width > 0.0) || !(q.BR > 0.0) || !(q.g_FF > 0.0))
+        return set_global_error(GRA_E_ARG,
+                                "resonance %d: needs M > 2 m_central, W > 0, BR > 0, g_FF > 0", r);
+      if (q.alpha_re * q.alpha_re + q.alpha_im * q.alpha_im <= 0.0)
+        return set_global_error(GRA_E_ARG, "resonance %d: alpha_{J0} = 0 (decay impossible)", r);
+    }
+  }
   const int dmin = (c->m_daughter > 0.0) ? 14 : (c->w_central > 0.0 ? 10 : 8);
   if (c->dim < dmin || c->dim > GRA_MAX_DIM)
     return set_global_error(GRA_E_ARG, "dim %d: this process needs %d..%d", c->dim, dmin,
@@ -198,6 +224,48 @@ int validate(const gra_cfg* c) {
   if (c->process == GRA_PROC_REGGE_CON && c->amp_precision == GRA_AMP_FP32 && on != 1)
     return set_global_error(GRA_E_ARG, "FP32 amplitude needs exactly one active Regge trajectory");
   return GRA_OK;
+}
+
+// Run constants of the resonances: the production coupling g_A e^{i g_phi}
+// (src/MForm.cc:276-279); the 2-body decay coupling sqrt(BR W / PS) with
+// PS = PDW2body(M0^2, m^2, m^2, 1, 1) (src/MProcess.cc:668-713,
+// include/Graniitti/MKinematics.h:331-334); T_{00} = alpha_{J0}, renormalised to
+// |alpha| = 1 when it is not (spin::InitTMatrix, src/MSpin.cc:54-262: for two
+// spinless daughters only (l, s) = (J, 0) passes parity, with NORM = CG = 1);
+// PhotoProp's running slope B0 at m^2 = M0^2 (src/MRegge.cc:849-880).
+void res_table(const gra_cfg& c, std::vector<ResDev>& tab) {
+  tab.assign((size_t)c.n_res, ResDev{});
+  const double m2 = c.m_central * c.m_central;
+  for (int r = 0; r < c.n_res; ++r) {
+    const gra_res& q = c.res[r];
+    ResDev& d = tab[r];
+    d.M0 = q.mass;
+    d.W = q.width;
+    d.M02 = q.mass * q.mass;
+    d.gFF4 = (q.g_FF * q.g_FF) * (q.g_FF * q.g_FF);
+    d.g_re = q.g_A * std::cos(q.g_phi);
+    d.g_im = q.g_A * std::sin(q.g_phi);
+    const double lam = d.M02 * d.M02 + m2 * m2 + m2 * m2 - 2.0 * d.M02 * m2 - 2.0 * d.M02 * m2 -
+                       2.0 * m2 * m2;
+    const double M0 = std::sqrt(d.M02 > 0.0 ? d.M02 : 0.0);
+    const double PS = std::sqrt(lam > 0.0 ? lam : 0.0) * 1.0 / (16.0 * GRA_PI * 1.0 * (M0 * M0 * M0));
+    const double g2 = q.BR * q.width / PS;
+    d.gdec = std::sqrt(g2 > 0.0 ? g2 : 0.0);
+    const double a2 = q.alpha_re * q.alpha_re + q.alpha_im * q.alpha_im;
+    const double an = std::fabs(a2 - 1.0) > 1e-6 ? std::sqrt(a2) : 1.0;
+    d.T_re = q.alpha_re / an;
+    d.T_im = q.alpha_im / an;
+    const double mf = std::max(0.7 * 0.7, d.M02);
+    d.B0 = std::pow(mf / 2.0, -GRA_PI / 2.0) + 3.0 * GRA_PI / 2.0;
+    for (int k = 0; k < 25; ++k) {
+      d.rho_re[k] = q.rho_re[k];
+      d.rho_im[k] = q.rho_im[k];
+    }
+    d.J = q.J;
+    d.P = q.P;
+    d.BW = q.BW;
+    d.photo = (q.J == 1 && q.P == -1) ? 1 : 0;
+  }
 }
 
 void fill_params(const gra_cfg& c, RunParams& P) {
@@ -338,6 +406,13 @@ void fill_params(const gra_cfg& c, RunParams& P) {
   P.tp_gPvv0 = c.tp_gPvv[0];
   P.tp_gPvv1 = c.tp_gPvv[1];
   P.tp_mp_ff = c.tp_mp_ff;
+  P.res = nullptr;
+  P.nres = c.process == GRA_PROC_RES_CON ? c.n_res : 0;
+  P.spin_gen = c.spin_gen;
+  P.spin_dec = c.spin_dec;
+  P.spin_frame = c.spin_frame;
+  P.omega = c.regge_omega;
+  P.sgn0 = 1.0;
   P.m_rho = c.m_central;
 }
 
@@ -573,6 +648,17 @@ int create_handle(const gra_cfg* cfg, const char* eikonal_dir, gra_handle** out)
       return set_global_error(GRA_E_HIP, "gra_hip_create: event creation failed");
     }
   }
+  if (h->P.nres > 0) {
+    std::vector<ResDev> tab;
+    res_table(h->cfg, tab);
+    const size_t b = sizeof(ResDev) * tab.size();
+    if (h->d_res.ensure(b) ||
+        hipMemcpy(h->d_res.p, tab.data(), b, hipMemcpyHostToDevice) != hipSuccess) {
+      gra_hip_destroy(h);
+      return set_global_error(GRA_E_HIP, "gra_hip_create: resonance table upload failed");
+    }
+    h->P.res = (const ResDev*)h->d_res.p;
+  }
   if (h->cfg.screening) {
     const int r = build_eikonal(h, eikonal_dir);
     if (r != GRA_OK) {
@@ -617,7 +703,7 @@ extern "C" void gra_hip_destroy(gra_handle* h) {
   if (!h) return;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   if (h->flow) flow_release(h->flow);
-  DevBuf* bufs[] = {&h->d_nodes, &h->d_x,    &h->d_W,         &h->d_amp2,      &h->d_flags,
+  DevBuf* bufs[] = {&h->d_nodes, &h->d_res, &h->d_x,    &h->d_W,         &h->d_amp2,      &h->d_flags,
                     &h->d_vw,    &h->d_bins, &h->d_rec,       &h->d_ncand,     &h->d_grid,
                     &h->d_part_hist, &h->d_part_scal, &h->d_stats, &h->d_aux};
   for (DevBuf* b : bufs) b->release();

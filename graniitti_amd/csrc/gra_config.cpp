@@ -148,6 +148,52 @@ void set_rhorho(gra_cfg* c) {
   c->fid_pt_min = 0.1;
 }
 
+// One resonance: RES_<name>.json (modeldata/TUNE0) and its pi+ pi- entry of
+// BRANCHING.json (BR, alpha_ls = [[J, 0, 1, 0]]); diagonal spin density
+void add_res(gra_cfg* c, int pdg, double M, double W, int J, double g_A, double g_phi, double BR,
+             double rho_pm) {
+  gra_res& r = c->res[c->n_res++];
+  r.pdg = pdg;
+  r.mass = M;
+  r.width = W;
+  r.J = J;
+  r.P = (J & 1) ? -1 : 1;
+  r.BW = GRA_BW_RUNNING;
+  r.g_A = g_A;
+  r.g_phi = g_phi;
+  r.g_FF = 1.0;
+  r.BR = BR;
+  r.alpha_re = 1.0;
+  r.alpha_im = 0.0;
+  const int n = 2 * J + 1;
+  if (J == 0) r.rho_re[0] = 1.0;
+  else {
+    r.rho_re[0] = rho_pm;                  // m = -J
+    r.rho_re[(n - 1) * n + (n - 1)] = rho_pm;  // m = +J
+  }
+}
+
+// PP[RES+CON]<C> -> pi+ pi- of tests/processes/ALICECUTS.json: the continuum of
+// preset 1 and the eight resonances of its "RES" list. PROC_17 sums
+// lts.RESONANCES, a std::map keyed by name (include/Graniitti/MKinematics.h:1467),
+// so in name order. PARAM_SPIN: SPINGEN, SPINDEC on, FRAME "CS"; PARAM_REGGE omega 0.5.
+void set_res_con(gra_cfg* c) {
+  c->process = GRA_PROC_RES_CON;
+  c->spin_gen = 1;
+  c->spin_dec = 1;
+  c->spin_frame = GRA_FRAME_CS;
+  c->regge_omega = 0.5;
+  c->n_res = 0;
+  add_res(c, 9030221, 1.506, 0.112, 0, 0.03, -3.14159, 0.2299, 0.0);     // f0_1500
+  add_res(c, 10331, 1.704, 0.123, 0, 0.03, -3.14159, 0.0552, 0.0);       // f0_1710
+  add_res(c, 9000221, 0.5, 0.4, 0, 0.05, -3.14159, 1.0, 0.0);            // f0_500
+  add_res(c, 9010221, 0.98, 0.08, 0, 0.065, -3.14159, 0.5, 0.0);         // f0_980
+  add_res(c, 225, 1.2755, 0.1867, 2, 0.7, -1.57, 0.562, 0.5);            // f2_1270
+  add_res(c, 335, 1.525, 0.073, 2, 0.19, -1.57, 0.00546, 0.5);           // f2_1525
+  add_res(c, 333, 1.019461, 0.004249, 1, 0.165, -2.36, 7.3e-05, 0.5);    // phi_1020
+  add_res(c, 113, 0.77526, 0.1491, 1, 0.19, -2.36, 1.0, 0.5);            // rho_770
+}
+
 }  // namespace
 
 }  // namespace gra_hip
@@ -179,6 +225,13 @@ extern "C" int gra_hip_cfg_default(gra_cfg* cfg, int32_t preset) {
       set_factorized(cfg);
       cfg->screening = preset == 5;
       cfg->amp_precision = preset == 5 ? GRA_AMP_FP32 : GRA_AMP_FP64;
+      break;
+    case 6:
+    case 7:
+      set_pipi(cfg);
+      set_res_con(cfg);
+      cfg->screening = preset == 7;
+      cfg->amp_precision = GRA_AMP_FP64;
       break;
     default:
       return set_global_error(GRA_E_ARG, "gra_hip_cfg_default: unknown preset %d", preset);

@@ -66,6 +66,7 @@ struct gra_handle {
   std::vector<double> mbt, msa;
   std::vector<NodeConst> nodes;
   gra_hip::DevBuf d_nodes;
+  gra_hip::DevBuf d_res;  // ResDev table of PP[RES+CON]
 
   // per-point buffers
   gra_hip::DevBuf d_x, d_W, d_amp2, d_flags, d_vw, d_bins, d_rec, d_ncand, d_grid;

@@ -13,6 +13,18 @@ struct GV4 {
   double e, x, y, z;
 };
 
+// One resonance of PP[RES+CON] with its run constants (MRegge::ME3 / PhotoME3,
+// src/MRegge.cc:677-706, 781-843; spin::ProdAmp / DecayAmp, src/MSpin.cc:280-486)
+struct ResDev {
+  double M0, W, M02, gFF4;  // pole mass, width, M0^2, g_FF^4
+  double g_re, g_im;        // production coupling g_A e^{i g_phi}
+  double gdec;              // decay coupling sqrt(BR W / PS) (src/MProcess.cc:713)
+  double T_re, T_im;        // decay helicity amplitude T_{00} of InitTMatrix
+  double B0;                // PhotoProp slope at m^2 = M0^2
+  double rho_re[25], rho_im[25];
+  int32_t J, P, BW, photo;  // photo: J^P = 1^- through MRegge::PhotoME3
+};
+
 struct RunParams {
   int32_t process, amp_mode, dim, screening;
   // beams and invariants
@@ -52,6 +64,10 @@ struct RunParams {
   double kappa;  // nrm * hfac (FP32 loop sum normalisation)
   // tensor pomeron
   double tp_delta_P, tp_ap_P, tp_gPNN, tp_gPvv0, tp_gPvv1, tp_mp_ff, m_rho;
+  // resonances (PP[RES+CON]): device table of nres entries
+  const ResDev* res;
+  int32_t nres, spin_gen, spin_dec, spin_frame;
+  double omega, sgn0;  // PARAM_REGGE::omega; beam pdg product sign for PhotoME3 (+1: pp)
 };
 
 // Screening node constants: one entry per (phi_i, kt_j) node, i-major.

@@ -1,0 +1,218 @@
+// Resonance amplitudes of PP[RES+CON] in FP64: A = MRegge::ME4(+1) + sum_r
+// {MRegge::ME3 | MRegge::PhotoME3}(r) (PROC_17::Amp2, include/Graniitti/MSubProc.h:398-432),
+// evaluated at the tree point and at every screening-loop node. Resonances are
+// summed in table order.
+#pragma once
+
+#include "gra_device.h"
+
+__device__ __forceinline__ double fact_small(int n) {
+  double f = 1.0;
+  for (int i = 2; i <= n; ++i) f *= (double)i;
+  return f;
+}
+__device__ __forceinline__ double ipow(double x, int n) {
+  double r = 1.0;
+  for (int i = 0; i < n; ++i) r *= x;
+  return r;
+}
+// Wigner small d^J_{m m'}(theta) as the finite sum over k (spin::WignerSmalld,
+// src/MSpin.cc:927-960), integer J <= 2
+__device__ __forceinline__ double wigner_d(double th, int m, int mp, int J) {
+  const double c = cos(th / 2.0), s = sin(th / 2.0);
+  double acc = 0.0;
+  for (int k = 0; k <= 2 * J; ++k) {
+    if (J - mp - k >= 0 && J + m - k >= 0 && k + mp - m >= 0) {
+      const double sg = ((mp - m + k) & 1) ? -1.0 : 1.0;
+      const double v1 = sg / (fact_small(J - mp - k) * fact_small(J + m - k) *
+                              fact_small(k + mp - m) * fact_small(k));
+      acc += v1 * (ipow(c, 2 * J + m - mp - 2 * k) * ipow(s, mp - m + 2 * k));
+    }
+  }
+  return sqrt(fact_small(J + m) * fact_small(J - m) * fact_small(J + mp) * fact_small(J - mp)) *
+         acc;
+}
+// M4Vec::Theta / Phi (include/Graniitti/M4Vec.h:150-155)
+__device__ __forceinline__ void theta_phi(dv4 p, double& th, double& ph) {
+  ph = (p.x == 0.0 && p.y == 0.0) ? 0.0 : atan2(p.y, p.x);
+  th = (p.x == 0.0 && p.y == 0.0 && p.z == 0.0) ? 0.0 : atan2(ptof(p), p.z);
+}
+// kinematics::LorentzBoost(b, M0, p, -1): p into the rest frame of b
+// (include/Graniitti/MKinematics.h:384-401)
+__device__ __forceinline__ dv4 boost_rest(dv4 b, double M0, dv4 p) {
+  const double bx = -1.0 * b.x / b.e, by = -1.0 * b.y / b.e, bz = -1.0 * b.z / b.e;
+  const double g = b.e / M0;
+  const double k1 = (bx * p.x + by * p.y) + bz * p.z;
+  const double k2 = g * (g * k1 / (1.0 + g) + p.e);
+  return v4m(p.x + k2 * bx, p.y + k2 * by, p.z + k2 * bz, g * (p.e + k1));
+}
+// spin::GetRhoRotation (src/MSpin.cc:544-610): the angles of the quantisation
+// axis of the spin density in the system rest frame
+__device__ __forceinline__ void rho_rotation(const RunParams& P, const Kin& K, double M, dv4 q1b,
+                                             double& thR, double& phR) {
+  thR = 0.0;
+  phR = 0.0;
+  if (P.spin_frame == GRA_FRAME_HX) {
+    theta_phi(K.pf[0], thR, phR);
+  } else if (P.spin_frame == GRA_FRAME_CS) {
+    const dv4 b1 = boost_rest(K.pf[0], M, P.pbeam1), b2 = boost_rest(K.pf[0], M, P.pbeam2);
+    const double n1 = sqrt(p3mod2(b1)), n2 = sqrt(p3mod2(b2));
+    const double zx = b1.x / n1 - b2.x / n2, zy = b1.y / n1 - b2.y / n2, zz = b1.z / n1 - b2.z / n2;
+    const double nz = sqrt(zx * zx + zy * zy + zz * zz);
+    theta_phi(v4m(zx / nz, zy / nz, zz / nz, 0.0), thR, phR);
+  } else if (P.spin_frame == GRA_FRAME_GJ) {
+    theta_phi(q1b, thR, phR);
+  }
+}
+// Re Tr[f rho_rot f^dagger] with rho_rot = D^dagger rho D, D_aj = D^J(thR, phR, M_a, M_j),
+// and rows f_rj = T_r D^J(th, ph, lam_r, M_j) (spin::fMatrix, DMatrix, WignerD,
+// src/MSpin.cc:631-705, 968-970), contracted as sum_ab rho_ab g_a conj(g_b) with
+// g_a = sum_j f_rj conj(D_aj)
+__device__ __forceinline__ double spin_trace(const ResDev& R, int nrow, const int* lam,
+                                             const double* T, double th, double ph, double thR,
+                                             double phR) {
+  const int J = R.J, n = 2 * J + 1;
+  double tr = 0.0;
+  for (int r = 0; r < nrow; ++r) {
+    cxd g[5];
+    for (int a = 0; a < n; ++a) {
+      cxd acc = cd(0.0, 0.0);
+      for (int j = 0; j < n; ++j) {
+        const int Mj = j - J;
+        const double fd = T[r] * wigner_d(th, lam[r], Mj, J);
+        const double Dd = wigner_d(thR, a - J, Mj, J);
+        const cxd f = cd(fd * cos(Mj * ph), fd * sin(Mj * ph));
+        const cxd Dc = cd(Dd * cos(Mj * phR), -(Dd * sin(Mj * phR)));
+        acc = cadd(acc, cmul(f, Dc));
+      }
+      g[a] = acc;
+    }
+    for (int a = 0; a < n; ++a)
+      for (int b = 0; b < n; ++b) {
+        const cxd rab = cd(R.rho_re[a * n + b], R.rho_im[a * n + b]);
+        tr += cmul(rab, cmul(g[a], cd(g[b].re, -g[b].im))).re;
+      }
+  }
+  return tr;
+}
+// spin::ProdAmp (src/MSpin.cc:280-403) for J^P = 0+, 1-, 2+: "Pomeron effective
+// spins" s1 = s2 = 0 with T0 = 1 (tensors), s1 = 0, s2 = 1 with the fixed T0 row (vectors)
+__device__ __forceinline__ cxd prod_amp(const RunParams& P, const ResDev& R, double thq,
+                                        double phq, double thR, double phR) {
+  const cxd A0 = cd(R.g_re, R.g_im);
+  if (!P.spin_gen || R.J == 0) return A0;
+  int lam[3] = {0, 0, 0};
+  double T[3] = {1.0, 0.0, 0.0};
+  int nrow = 1;
+  if (R.J == 1) {
+    nrow = 3;
+    lam[0] = 1;
+    lam[1] = 0;
+    lam[2] = -1;
+    T[0] = -0.697;
+    T[1] = -0.169;
+    T[2] = 0.697;
+  }
+  const double tr = spin_trace(R, nrow, lam, T, thq, phq, thR, phR);
+  return cscale(A0, sqrt(2.0 * R.J + 1.0) * msq(tr));
+}
+// spin::DecayAmp (src/MSpin.cc:406-486) for two spinless daughters: f = fX, 1 x (2J+1).
+// |T_00| = 1 leaves the trace unchanged, so only J = 0 carries T_00's phase.
+__device__ __forceinline__ cxd decay_amp(const RunParams& P, const ResDev& R, double thd,
+                                         double phd, double thR, double phR) {
+  const double A0 = R.gdec;
+  if (!P.spin_dec) return cd(A0, 0.0);
+  if (R.J == 0) return cd(R.T_re * A0, R.T_im * A0);
+  const int lam[1] = {0};
+  const double T[1] = {sqrt(R.T_re * R.T_re + R.T_im * R.T_im)};
+  const double tr = spin_trace(R, 1, lam, T, thd, phd, thR, phR);
+  return cd(A0 * (sqrt(2.0 * R.J + 1.0) * msq(tr)), 0.0);
+}
+// form::CBW (src/MForm.cc:743-828)
+__device__ __forceinline__ cxd cbw(const ResDev& R, double m2, double mA, double mB) {
+  const double a = m2 - R.M0 * R.M0;
+  double b, num = -1.0;
+  if (R.BW == GRA_BW_FIXED) {
+    b = R.M0 * R.W;
+  } else if (R.BW == GRA_BW_RUNNING) {
+    b = sqrt(m2) * R.W;
+  } else if (R.BW == GRA_BW_BARRIER) {
+    const double u = msq((m2 - (mA + mB) * (mA + mB)) * (m2 - (mA - mB) * (mA - mB))) / (2 * msq(m2));
+    const double d = msq((R.M0 * R.M0 - (mA + mB) * (mA + mB)) * (R.M0 * R.M0 - (mA - mB) * (mA - mB))) /
+                     (2 * R.M0);
+    b = R.W * R.M0 * R.M0 / msq(m2) * pow(u / d, 2 * R.J + 1);
+  } else {
+    b = R.M0 * R.W;
+    const double r = m2 / (R.M0 * R.M0);
+    if (R.J == 1) num = -(1.0 - r);
+    if (R.J == 2) num = -(7.0 / 6.0 - (4.0 / 3.0) * r + (2.0 / 3.0) * (m2 * m2) / (R.M02 * R.M02));
+  }
+  const double den = a * a + b * b;
+  return cd(num * a / den, -num * b / den);
+}
+// form::G_E_KELLY, G_M_KELLY (src/MForm.cc:607-645) and form::CohFlux (src/MForm.cc:664-687)
+__device__ __forceinline__ double coh_flux(double xi, double t, double pt, double mp) {
+  const double pt2 = pt * pt, xi2 = xi * xi, mp2 = mp * mp, Q2 = fabs(t);
+  const double tau = Q2 / ((2 * mp) * (2 * mp));
+  const double GE = (1.0 + -0.24 * tau) / (1.0 + 10.98 * tau + 12.82 * (tau * tau) + 21.97 * (tau * tau * tau));
+  const double GM = 2.792847337 * ((1.0 + 0.12 * tau) /
+                                   (1.0 + 10.97 * tau + 18.86 * (tau * tau) + 6.55 * (tau * tau * tau)));
+  const double PART1 = (4.0 * mp2 * (GE * GE) + Q2 * (GM * GM)) / (4.0 * mp2 + Q2);
+  const double PART2 = GM * GM;
+  const double DELTA = pt2 / (pt2 + xi2 * mp2);
+  double f = 0.0072973525693 / GRA_PI *
+             ((1.0 - xi) * (DELTA * DELTA) * PART1 + (xi2 / 4.0) * DELTA * PART2);
+  f /= xi;
+  f /= pt2;
+  f *= 16.0 * (GRA_PI * GRA_PI);
+  return f;
+}
+// MRegge::PhotoProp, elastic forward leg (src/MRegge.cc:849-880): the leading trajectory
+__device__ __forceinline__ cxd photo_prop(const RunParams& P, double s, double t, double B0) {
+  const double alpha = P.regge_a0[0] + P.regge_ap[0] * t;
+  const double th = ((-GRA_PI / 2.0) * P.regge_ap[0]) * t;
+  const cxd eta = cmul(cd(P.eta0_re[0], P.eta0_im[0]), cd(cos(th), sin(th)));
+  const double FF = P.gN_P * exp(B0 * t / 2.0);
+  return cscale(cscale(eta, pow(s / 1.0, alpha)), FF);
+}
+
+__device__ __forceinline__ cxd res_sum(const RunParams& P, const Kin& K) {
+  const dv4 X = K.pf[0];
+  const double m2 = m2of(X), M = mof(X);
+  const double s1 = m2of(v4add(X, K.pf[1])), s2 = m2of(v4add(X, K.pf[2]));
+  const double FF_A = P.gN_P * s3f(P, K.t1), FF_B = P.gN_P * s3f(P, K.t2);
+  const cxd pr1 = regge_prop(P, s1, K.t1), pr2 = regge_prop(P, s2, K.t2);
+  const double V = pow(1.0 / m2, P.omega);
+  const dv4 q1b = boost_rest(X, M, K.q1);
+  double thq, phq, thd, phd, thR, phR;
+  theta_phi(q1b, thq, phq);
+  theta_phi(boost_rest(X, M, K.dt[0]), thd, phd);
+  rho_rotation(P, K, M, q1b, thR, phR);
+  const double mA = mof(K.dt[0]), mB = mof(K.dt[1]);
+  cxd A = cd(0.0, 0.0);
+  for (int r = 0; r < P.nres; ++r) {
+    const ResDev& R = P.res[r];
+    const cxd bw = cbw(R, m2, mA, mB);
+    const double rff = exp(-((m2 - R.M02) * (m2 - R.M02)) / R.gFF4);
+    const cxd spin = cmul(prod_amp(P, R, thq, phq, thR, phR), decay_amp(P, R, thd, phd, thR, phR));
+    if (R.photo) {
+      const dv4 q2 = v4sub(P.pbeam2, K.pf[2]);
+      const double x1 = 1 - K.pf[1].z / P.pbeam1.z, x2 = 1 - K.pf[2].z / P.pbeam2.z;
+      const double gf1 = msq(coh_flux(x1, K.t1, ptof(K.q1), P.tp_mp_ff) / x1);
+      const double gf2 = msq(coh_flux(x2, K.t2, ptof(q2), P.tp_mp_ff) / x2);
+      const cxd common = cscale(bw, rff);
+      const cxd A1 = cmul(cscale(common, gf2), photo_prop(P, s1, K.t1, R.B0));
+      const cxd A2 = cmul(cscale(common, gf1), photo_prop(P, s2, K.t2, R.B0));
+      A = cadd(A, cmul(cadd(A1, A2), spin));
+    } else {
+      cxd Ap = cscale(pr1, 2.0);
+      Ap = cscale(Ap, FF_A);
+      Ap = cmul(Ap, bw);
+      Ap = cscale(Ap, rff);
+      Ap = cmul(Ap, pr2);
+      Ap = cscale(Ap, FF_B);
+      A = cadd(A, cscale(cmul(Ap, spin), V));
+    }
+  }
+  return A;
+}

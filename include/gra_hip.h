@@ -55,6 +55,20 @@ typedef struct gra_handle gra_handle; /* opaque: device state of one process */
 /* Processes (sub-process registry, include/Graniitti/MSubProc.h) */
 #define GRA_PROC_REGGE_CON 15  /* PROC_15 "PP[CON]" 2-body: MRegge::ME4, include/Graniitti/MSubProc.h:361-380 */
 #define GRA_PROC_TENSOR_CON 12 /* PROC_12 "PP[CONTENSOR]" 2xV: MTensorPomeron::ME4, MSubProc.h:288-300 */
+#define GRA_PROC_RES_CON 17    /* PROC_17 "PP[RES+CON]" 2-body: MRegge::ME4 + resonances, MSubProc.h:398-432 */
+
+/* Resonances of PP[RES+CON] */
+#define GRA_MAX_RES 8
+/* Breit-Wigner forms, PARAM_RES "BW" (form::CBW, src/MForm.cc:743-828) */
+#define GRA_BW_FIXED 1   /* fixed-width relativistic */
+#define GRA_BW_RUNNING 2 /* running-width relativistic */
+#define GRA_BW_BARRIER 3 /* spin-barrier (Blatt-Weisskopf type) relativistic */
+#define GRA_BW_SPIN 4    /* spin-dependent relativistic */
+/* Rest frame of the spin density, PARAM_SPIN "FRAME" (spin::GetRhoRotation, src/MSpin.cc:544-610) */
+#define GRA_FRAME_CM 0 /* no rotation */
+#define GRA_FRAME_HX 1 /* helicity */
+#define GRA_FRAME_CS 2 /* Collins-Soper */
+#define GRA_FRAME_GJ 3 /* Gottfried-Jackson */
 
 /* Amplitude arithmetic */
 #define GRA_AMP_FP64 0 /* everything FP64, operation order of the reference */
@@ -69,6 +83,22 @@ typedef struct gra_handle gra_handle; /* opaque: device state of one process */
 /* ------------------------------------------------------------------------- */
 /* Configuration (plain old data, passed by value through the ABI)           */
 /* ------------------------------------------------------------------------- */
+/* One resonance (form::ReadResonance, src/MForm.cc:213-330; modeldata/TUNE0/RES_*.json)
+ * and its decay to the two central particles (MProcess::ProcessHelicityDecay,
+ * src/MProcess.cc:534-740; modeldata/TUNE0/BRANCHING.json). Spins J = 0, 1, 2
+ * with parity (-1)^J: the states that decay to two spinless central particles. */
+typedef struct gra_res {
+  double mass, width;            /* "M", "W" (GeV) */
+  double g_A, g_phi;             /* production coupling g_A exp(i g_phi) */
+  double g_FF;                   /* resonance form-factor parameter (GeV) */
+  double BR;                     /* branching fraction to the central pair */
+  double alpha_re, alpha_im;     /* decay helicity coupling alpha_{l = J, s = 0} */
+  double rho_re[25], rho_im[25]; /* spin density, (2J+1) x (2J+1) row-major, projections -J..J */
+  int32_t J, P;                  /* spin, parity */
+  int32_t BW;                    /* GRA_BW_* */
+  int32_t pdg;
+} gra_res;
+
 typedef struct gra_cfg {
   int32_t abi_size; /* = sizeof(gra_cfg), checked by every entry point */
   int32_t process;  /* GRA_PROC_* */
@@ -130,6 +160,16 @@ typedef struct gra_cfg {
   int32_t phase_space; /* GRA_PS_C or GRA_PS_F */
   int32_t pad_ps;
   double gen_M_min, gen_M_max;
+  /* Resonances of PP[RES+CON], summed with the continuum amplitude in this
+   * order (PROC_17::Amp2, include/Graniitti/MSubProc.h:398-432); the spin
+   * switches and frame of PARAM_SPIN and the flux exponent PARAM_REGGE::omega
+   * (modeldata/TUNE0/GENERAL.json) */
+  int32_t n_res;
+  int32_t spin_gen;   /* SPINGEN: 2 -> 1 production spin correlations */
+  int32_t spin_dec;   /* SPINDEC: 1 -> 2 decay spin correlations */
+  int32_t spin_frame; /* GRA_FRAME_* */
+  double regge_omega;
+  gra_res res[GRA_MAX_RES];
 } gra_cfg;
 
 /* ------------------------------------------------------------------------- */
@@ -248,6 +288,10 @@ typedef struct {
  *   4: PP[CON]<F> -> pi+ pi-, preset 1's cuts, system |Y| < 0.9 and M in [0, 4] GeV
  *      (GENCUTS "<F>" of tests/run_compare_PS/card.json), no screening, FP64
  *   5: preset 4 + eikonal screening as preset 2, FP32 amplitude
+ *   6: PP[RES+CON]<C> -> pi+ pi- (tests/processes/ALICECUTS.json): preset 1's continuum and
+ *      cuts plus f0(500), rho(770), f0(980), phi(1020), f2(1270), f0(1500), f2'(1525),
+ *      f0(1710) with spin correlations in the Collins-Soper frame, FP64 amplitude
+ *   7: preset 6 + eikonal screening as preset 2 (FP64 amplitude)
  * Replaces the steering-card reader MGraniitti::ReadInput (src/MGraniitti.cc:318-329)
  * for the keys these workloads use, with TUNE0 values (modeldata/TUNE0/GENERAL.json). */
 int gra_hip_cfg_default(gra_cfg* cfg, int32_t preset);

@@ -787,6 +787,243 @@ static cx regge_me4(const ora_run* R, const kin_t* K) {
   return cx_add(A_t, A_u);
 }
 
+/* ------------------------------------------------------------------------- */
+/* Resonances of PP[RES+CON] (PROC_17::Amp2, include/Graniitti/MSubProc.h:398-432) */
+/* ------------------------------------------------------------------------- */
+static double o_fact(int n) {
+  double f = 1.0;
+  for (int i = 2; i <= n; ++i) f *= (double)i;
+  return f;
+}
+/* spin::WignerSmalld, src/MSpin.cc:927-960 (integer J) */
+static double o_wigner_d(double th, double m, double mp, double J) {
+  const double factor = msqrt(o_fact((int)(J + m)) * o_fact((int)(J - m)) * o_fact((int)(J + mp)) *
+                              o_fact((int)(J - mp)));
+  double s = 0.0;
+  for (int k = 0; k <= 2 * J; ++k) {
+    double value = 0.0;
+    if (J - mp - k >= 0 && J + m - k >= 0 && k + mp - m >= 0 && k >= 0) {
+      const double val1 = pow(-1, mp - m + k) /
+                          (o_fact((int)(J - mp - k)) * o_fact((int)(J + m - k)) *
+                           o_fact((int)(k + mp - m)) * o_fact(k));
+      const double val2 = pow(cos(th / 2.0), 2 * J + m - mp - 2 * k) * pow(sin(th / 2.0), mp - m + 2 * k);
+      value = val1 * val2;
+    }
+    s += value;
+  }
+  return factor * s;
+}
+ORA_EXPORT double ora_wigner_d(double th, int m, int mp, int J) { return o_wigner_d(th, m, mp, J); }
+/* spin::WignerD, src/MSpin.cc:968-970 */
+static cx o_wigner_D(double th, double ph, double m, double mp, double J) {
+  const cx e = cx_exp(cx_make(0.0, mp * ph));
+  return cx_scale(e, o_wigner_d(th, m, mp, J));
+}
+static void o_theta_phi(v4 p, double* th, double* ph) {
+  *ph = (p.x == 0.0 && p.y == 0.0) ? 0.0 : atan2(p.y, p.x);
+  *th = (p.x == 0.0 && p.y == 0.0 && p.z == 0.0) ? 0.0 : atan2(v4_pt(p), p.z);
+}
+/* kinematics::LorentzBoost(b, M0, p, -1), include/Graniitti/MKinematics.h:384-401 */
+static v4 o_boost_rest(v4 b, double M0, v4 p) {
+  const double beta[3] = {-1 * b.x / b.e, -1 * b.y / b.e, -1 * b.z / b.e};
+  const double gamma = b.e / M0;
+  const double k1 = beta[0] * p.x + beta[1] * p.y + beta[2] * p.z;
+  const double k2 = gamma * (gamma * k1 / (1.0 + gamma) + p.e);
+  return v4_make(p.x + k2 * beta[0], p.y + k2 * beta[1], p.z + k2 * beta[2], gamma * (p.e + k1));
+}
+/* spin::GetRhoRotation, src/MSpin.cc:544-610 */
+static void o_rho_rotation(const ora_run* R, const kin_t* K, double* thR, double* phR) {
+  const double M = v4_m(K->pf[0]);
+  *thR = 0.0;
+  *phR = 0.0;
+  if (R->c.spin_frame == GRA_FRAME_HX) {
+    o_theta_phi(K->pf[0], thR, phR);
+  } else if (R->c.spin_frame == GRA_FRAME_CS) {
+    const v4 b1 = o_boost_rest(K->pf[0], M, R->pbeam1), b2 = o_boost_rest(K->pf[0], M, R->pbeam2);
+    const double n1 = sqrt(v4_p3mod2(b1)), n2 = sqrt(v4_p3mod2(b2));
+    double z[3] = {b1.x / n1 - b2.x / n2, b1.y / n1 - b2.y / n2, b1.z / n1 - b2.z / n2};
+    const double nz = sqrt(z[0] * z[0] + z[1] * z[1] + z[2] * z[2]);
+    o_theta_phi(v4_make(z[0] / nz, z[1] / nz, z[2] / nz, 0.0), thR, phR);
+  } else if (R->c.spin_frame == GRA_FRAME_GJ) {
+    o_theta_phi(o_boost_rest(K->pf[0], M, K->q1), thR, phR);
+  }
+}
+/* Re Tr[f rho_ROT f^dagger], rho_ROT = D^dagger rho D, in the reference's order
+ * (spin::fMatrix, DMatrix, src/MSpin.cc:631-705; ProdAmp / DecayAmp 380-398, 460-484).
+ * Rows r of f: final-state helicity lam[r] with amplitude T[r]. */
+static double o_spin_trace(int J, int nrow, const int* lam, const cx* T, double th, double ph,
+                           double thR, double phR, const double* rho_re, const double* rho_im) {
+  const int n = 2 * J + 1;
+  cx f[3][5], D[5][5], X[5][5], Rr[5][5], Y[3][5];
+  for (int i = 0; i < nrow; ++i)
+    for (int j = 0; j < n; ++j) f[i][j] = cx_mul(o_wigner_D(th, ph, lam[i], j - J, J), T[i]);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) D[i][j] = o_wigner_D(thR, phR, i - J, j - J, J);
+  for (int i = 0; i < n; ++i) /* D^dagger * rho */
+    for (int j = 0; j < n; ++j) {
+      cx a = cx_make(0, 0);
+      for (int k = 0; k < n; ++k)
+        a = cx_add(a, cx_mul(cx_make(D[k][i].re, -D[k][i].im), cx_make(rho_re[k * n + j], rho_im[k * n + j])));
+      X[i][j] = a;
+    }
+  for (int i = 0; i < n; ++i) /* (D^dagger rho) * D */
+    for (int j = 0; j < n; ++j) {
+      cx a = cx_make(0, 0);
+      for (int k = 0; k < n; ++k) a = cx_add(a, cx_mul(X[i][k], D[k][j]));
+      Rr[i][j] = a;
+    }
+  for (int i = 0; i < nrow; ++i) /* f * rho_ROT */
+    for (int j = 0; j < n; ++j) {
+      cx a = cx_make(0, 0);
+      for (int k = 0; k < n; ++k) a = cx_add(a, cx_mul(f[i][k], Rr[k][j]));
+      Y[i][j] = a;
+    }
+  double tr = 0.0; /* Tr[(f rho_ROT) f^dagger] */
+  for (int i = 0; i < nrow; ++i) {
+    cx a = cx_make(0, 0);
+    for (int k = 0; k < n; ++k) a = cx_add(a, cx_mul(Y[i][k], cx_make(f[i][k].re, -f[i][k].im)));
+    tr += a.re;
+  }
+  return tr;
+}
+ORA_EXPORT double ora_spin_trace(int J, int nrow, const int* lam, const double* T, double th,
+                                 double ph, double thR, double phR, const double* rho_re,
+                                 const double* rho_im) {
+  cx Tc[3];
+  for (int i = 0; i < nrow; ++i) Tc[i] = cx_make(T[i], 0.0);
+  return o_spin_trace(J, nrow, lam, Tc, th, ph, thR, phR, rho_re, rho_im);
+}
+/* decay helicity amplitude T_00 (spin::InitTMatrix, src/MSpin.cc:54-262, two spinless
+ * daughters: only (l, s) = (J, 0), NORM = CG = 1, alpha renormalised to |alpha| = 1) */
+static cx o_res_T(const gra_res* q) {
+  const double a2 = q->alpha_re * q->alpha_re + q->alpha_im * q->alpha_im;
+  const double an = fabs(a2 - 1.0) > 1e-6 ? sqrt(a2) : 1.0;
+  return cx_make(q->alpha_re / an, q->alpha_im / an);
+}
+/* hc.g_decay = msqrt(BR W / PS), PS = PDW2body(M0^2, m^2, m^2, 1, 1)
+ * (src/MProcess.cc:668-713, include/Graniitti/MKinematics.h:331-334) */
+static double o_res_gdecay(const ora_run* R, const gra_res* q) {
+  const double M02 = q->mass * q->mass, m2 = R->c.m_central * R->c.m_central;
+  const double lam = M02 * M02 + m2 * m2 + m2 * m2 - 2.0 * (M02 * m2 + M02 * m2 + m2 * m2);
+  const double PS = msqrt(lam) * 1.0 / (16.0 * PI * 1.0 * pow(msqrt(M02), 3));
+  return msqrt(q->BR * q->width / PS);
+}
+ORA_EXPORT double ora_res_gdecay(const ora_run* R, int r) { return o_res_gdecay(R, &R->c.res[r]); }
+/* spin::ProdAmp, src/MSpin.cc:280-403 (J^P = 0+, 1-, 2+) */
+static cx o_prod_amp(const ora_run* R, const gra_res* q, const kin_t* K, double thR, double phR) {
+  cx A0 = cx_make(q->g_A * cos(q->g_phi), q->g_A * sin(q->g_phi));
+  if (!R->c.spin_gen || q->J == 0) return A0;
+  int lam[3] = {0, 0, 0};
+  cx T[3] = {cx_make(1, 0), cx_make(0, 0), cx_make(0, 0)};
+  int nrow = 1;
+  if (q->J == 1) { /* s1 = 0, s2 = 1: lambda = 0 - lambda2, lambda2 = -1, 0, 1 */
+    nrow = 3;
+    lam[0] = 1;
+    lam[1] = 0;
+    lam[2] = -1;
+    T[0] = cx_make(-0.697, 0);
+    T[1] = cx_make(-0.169, 0);
+    T[2] = cx_make(0.697, 0);
+  }
+  double th, ph;
+  o_theta_phi(o_boost_rest(K->pf[0], v4_m(K->pf[0]), K->q1), &th, &ph);
+  const double tr = o_spin_trace(q->J, nrow, lam, T, th, ph, thR, phR, q->rho_re, q->rho_im);
+  return cx_scale(A0, msqrt(2.0 * q->J + 1.0) * msqrt(tr));
+}
+/* spin::DecayAmp, src/MSpin.cc:406-486 (two spinless daughters, no recursion) */
+static cx o_decay_amp(const ora_run* R, const gra_res* q, const kin_t* K, double thR, double phR) {
+  const double A0 = o_res_gdecay(R, q);
+  if (!R->c.spin_dec) return cx_make(A0, 0);
+  const cx T = o_res_T(q);
+  if (q->J == 0) return cx_scale(T, A0);
+  double th, ph;
+  o_theta_phi(o_boost_rest(K->pf[0], v4_m(K->pf[0]), K->dt[0]), &th, &ph);
+  const int lam[1] = {0};
+  const double tr = o_spin_trace(q->J, 1, lam, &T, th, ph, thR, phR, q->rho_re, q->rho_im);
+  return cx_make(A0 * (msqrt(2.0 * q->J + 1.0) * msqrt(tr)), 0);
+}
+ORA_EXPORT int ora_res_spin(ora_run* R, const double* u, int r, double* out);
+/* form::CBW, src/MForm.cc:743-828 */
+static cx o_cbw(const gra_res* q, double m2, double mA, double mB) {
+  const double M0 = q->mass, G = q->width;
+  cx den = cx_make(m2 - M0 * M0, M0 * G);
+  double num = -1.0;
+  if (q->BW == GRA_BW_RUNNING) den = cx_make(m2 - M0 * M0, sqrt(m2) * G);
+  if (q->BW == GRA_BW_BARRIER) {
+    const int J = q->J;
+    const double u = msqrt((m2 - (mA + mB) * (mA + mB)) * (m2 - (mA - mB) * (mA - mB))) / (2 * msqrt(m2));
+    const double d = msqrt((M0 * M0 - (mA + mB) * (mA + mB)) * (M0 * M0 - (mA - mB) * (mA - mB))) / (2 * M0);
+    den = cx_make(m2 - M0 * M0, G * M0 * M0 / msqrt(m2) * pow(u / d, 2 * J + 1));
+  }
+  if (q->BW == GRA_BW_SPIN) {
+    if (q->J == 1) num = -(1.0 - m2 / (M0 * M0));
+    if (q->J == 2)
+      num = -(7.0 / 6.0 - (4.0 / 3.0) * (m2 / (M0 * M0)) + (2.0 / 3.0) * (m2 * m2) / pow(M0, 4));
+  }
+  const double d2 = den.re * den.re + den.im * den.im;
+  return cx_make(num * den.re / d2, -num * den.im / d2);
+}
+/* form::CohFlux with the Kelly G_E, G_M (src/MForm.cc:607-687), alpha_QED(0) */
+static double o_coh_flux(double xi, double t, double pt, double mp) {
+  const double pt2 = pt * pt, xi2 = xi * xi, mp2 = mp * mp, Q2 = fabs(t);
+  const double tau = Q2 / ((2 * mp) * (2 * mp));
+  const double GE = (1.0 - 0.24 * tau) / (1.0 + 10.98 * tau + 12.82 * tau * tau + 21.97 * tau * tau * tau);
+  const double GM = 2.792847337 * (1.0 + 0.12 * tau) / (1.0 + 10.97 * tau + 18.86 * tau * tau + 6.55 * tau * tau * tau);
+  const double PART1 = (4.0 * mp2 * GE * GE + Q2 * GM * GM) / (4.0 * mp2 + Q2);
+  const double PART2 = GM * GM;
+  const double DELTA = pt2 / (pt2 + xi2 * mp2);
+  double f = 0.0072973525693 / PI * ((1.0 - xi) * DELTA * DELTA * PART1 + (xi2 / 4.0) * DELTA * PART2);
+  f /= xi;
+  f /= pt2;
+  f *= 16.0 * PI * PI;
+  return f;
+}
+/* MRegge::PhotoProp, elastic leg (src/MRegge.cc:849-880) */
+static cx o_photo_prop(const gra_cfg* c, double s, double t, double m2) {
+  const double B0 = pow(fmax(0.7 * 0.7, m2) / 2.0, -PI / 2.0) + 3 * PI / 2.0;
+  const double alpha = c->regge_a0[0] + c->regge_ap[0] * t;
+  const cx eta = regge_eta_linear(t, c->regge_a0[0], c->regge_ap[0], c->regge_sgn[0]);
+  const double FF = c->gN_P * exp(B0 * t / 2.0);
+  return cx_scale(cx_scale(eta, pow(s / 1.0, alpha)), FF);
+}
+/* sum over resonances of MRegge::ME3 (src/MRegge.cc:677-706) and, for 1-,
+ * MRegge::PhotoME3 (src/MRegge.cc:781-843) */
+static cx res_sum(const ora_run* R, const kin_t* K) {
+  const gra_cfg* c = &R->c;
+  double thR, phR;
+  o_rho_rotation(R, K, &thR, &phR);
+  const double FF_A = c->gN_P * s3f(c, K->t1), FF_B = c->gN_P * s3f(c, K->t2);
+  cx A = cx_make(0, 0);
+  for (int r = 0; r < c->n_res; ++r) {
+    const gra_res* q = &c->res[r];
+    const cx bw = o_cbw(q, K->m2, v4_m(K->dt[0]), v4_m(K->dt[1]));
+    const double M2 = q->mass * q->mass;
+    const double rff = exp(-((K->m2 - M2) * (K->m2 - M2)) / pow(q->g_FF, 4));
+    const cx spin = cx_mul(o_prod_amp(R, q, K, thR, phR), o_decay_amp(R, q, K, thR, phR));
+    if (q->J == 1 && q->P == -1) {
+      const cx common = cx_scale(bw, rff);
+      const double x1 = (1 - K->pf[1].z / R->pbeam1.z), x2 = (1 - K->pf[2].z / R->pbeam2.z);
+      double gf1 = o_coh_flux(x1, K->t1, v4_pt(K->q1), c->tp_mp_ff);
+      double gf2 = o_coh_flux(x2, K->t2, v4_pt(K->q2), c->tp_mp_ff);
+      gf1 = msqrt(gf1 / x1);
+      gf2 = msqrt(gf2 / x2);
+      const cx A1 = cx_mul(cx_scale(common, gf2), o_photo_prop(c, K->s1, K->t1, M2));
+      const cx A2 = cx_mul(cx_scale(common, gf1), o_photo_prop(c, K->s2, K->t2, M2));
+      A = cx_add(A, cx_mul(cx_add(A1, A2), spin));
+    } else {
+      cx Ap = cx_scale(regge_prop(c, K->s1, K->t1), 2.0);
+      Ap = cx_scale(Ap, FF_A);
+      Ap = cx_mul(Ap, bw);
+      Ap = cx_scale(Ap, rff);
+      Ap = cx_mul(Ap, regge_prop(c, K->s2, K->t2));
+      Ap = cx_scale(Ap, FF_B);
+      const double V = pow(1.0 / K->m2, c->regge_omega);
+      A = cx_add(A, cx_scale(cx_mul(Ap, spin), V));
+    }
+  }
+  return A;
+}
+
 /* Tensor-pomeron ME4 (defined further below). Writes nh helicity amplitudes. */
 static int tensor_me4(const ora_run* R, const kin_t* K, cx* hamp);
 
@@ -794,6 +1031,7 @@ static int tensor_me4(const ora_run* R, const kin_t* K, cx* hamp);
 static int bare_amplitudes(const ora_run* R, const kin_t* K, cx* hamp) {
   if (R->c.process == GRA_PROC_TENSOR_CON) return tensor_me4(R, K, hamp);
   hamp[0] = regge_me4(R, K);
+  if (R->c.process == GRA_PROC_RES_CON) hamp[0] = cx_add(hamp[0], res_sum(R, K));
   return 1;
 }
 
@@ -1757,4 +1995,20 @@ ORA_EXPORT double ora_phase_space_weight(ora_run* R, const double* x) {
                      pow(c->rap_max - c->rap_min, 2) *
                      ((2.0 * PI) * (2.0 * PI) * (v4_pt(K.pf[1]) * v4_pt(K.pf[2]) * (lspan * lspan)));
   return psw * vol;
+}
+
+/* Spin factors of resonance r at hypercube point x (test access):
+ * out[0] = |ProdAmp| / g_A, out[1] = |DecayAmp| / g_decay. Returns 1 if the
+ * kinematics are valid. */
+ORA_EXPORT int ora_res_spin(ora_run* R, const double* x, int r, double* out) {
+  kin_t K;
+  memset(&K, 0, sizeof(K));
+  if (!build_kin(R, x, &K)) return 0;
+  double thR, phR;
+  o_rho_rotation(R, &K, &thR, &phR);
+  const gra_res* q = &R->c.res[r];
+  const cx pa = o_prod_amp(R, q, &K, thR, phR), da = o_decay_amp(R, q, &K, thR, phR);
+  out[0] = sqrt(cx_abs2(pa)) / q->g_A;
+  out[1] = sqrt(cx_abs2(da)) / o_res_gdecay(R, q);
+  return 1;
 }

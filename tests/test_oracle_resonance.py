@@ -1,0 +1,174 @@
+"""The oracle's PP[RES+CON] resonance amplitudes (PROC_17, include/Graniitti/MSubProc.h:398-432)
+against what the mathematics fixes:
+- Wigner small-d values against their closed forms, and the orthogonality of d^J;
+- the spin-density traces Tr[f D^+ rho D f^+] (spin::ProdAmp / DecayAmp) against the
+  rotation-invariant forms sum_j rho_j |d^J_{j lam}(beta)|^2, with beta the angle
+  between the two directions; angle independence for an unpolarised rho;
+- the f2(1270) and rho(770) spin factors at generated events against the
+  Collins-Soper angles computed here from the four-vectors;
+- the decay coupling against the two-body width Gamma = g^2 p* / (8 pi M^2).
+CBW, the resonance form factor, PhotoProp and the coherent photon flux are
+formula restatements checked only GPU = oracle point by point: parity unpinned."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from graniitti_amd import abi
+import oracle_lib as ol
+
+MPI = 0.13957061
+
+
+def L():
+    lib = ol.oracle()
+    lib.ora_wigner_d.restype = C.c_double
+    lib.ora_wigner_d.argtypes = [C.c_double, C.c_int, C.c_int, C.c_int]
+    lib.ora_spin_trace.restype = C.c_double
+    lib.ora_spin_trace.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                   C.c_double, C.c_double, C.c_double, C.c_double,
+                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    lib.ora_res_spin.restype = C.c_int
+    lib.ora_res_spin.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double)]
+    lib.ora_res_gdecay.restype = C.c_double
+    lib.ora_res_gdecay.argtypes = [C.c_void_p, C.c_int]
+    return lib
+
+
+def d(th, mp, m, J):
+    """d^J_{mp m}(th) in the usual (row mp, column m) notation; the reference's
+    WignerSmalld(theta, m, mp, J) puts the column index first"""
+    return L().ora_wigner_d(th, m, mp, J)
+
+
+def test_wigner_d_closed_forms():
+    for th in np.linspace(0.05, 3.1, 13):
+        c, s = np.cos(th), np.sin(th)
+        assert np.isclose(d(th, 0, 0, 1), c, atol=1e-14)
+        assert np.isclose(d(th, 1, 1, 1), (1 + c) / 2, atol=1e-14)
+        assert np.isclose(d(th, 1, -1, 1), (1 - c) / 2, atol=1e-14)
+        assert np.isclose(d(th, 1, 0, 1), -s / np.sqrt(2), atol=1e-14)
+        assert np.isclose(d(th, 0, 1, 1), s / np.sqrt(2), atol=1e-14)
+        assert np.isclose(d(th, 0, 0, 2), (3 * c * c - 1) / 2, atol=1e-14)
+        assert np.isclose(d(th, 2, 0, 2), np.sqrt(3 / 8) * s * s, atol=1e-14)
+        assert np.isclose(d(th, 1, 0, 2), -np.sqrt(3 / 2) * s * c, atol=1e-14)
+        assert np.isclose(d(th, 2, 2, 2), ((1 + c) / 2) ** 2, atol=1e-14)
+        assert np.isclose(d(th, 2, 1, 2), -(1 + c) * s / 2, atol=1e-14)
+        for J in (1, 2):
+            M = np.array([[d(th, a - J, b - J, J) for b in range(2 * J + 1)]
+                          for a in range(2 * J + 1)])
+            assert np.allclose(M @ M.T, np.eye(2 * J + 1), atol=1e-13)
+
+
+def trace(J, lam, T, th, ph, thR, phR, rho):
+    n = 2 * J + 1
+    rho = np.asarray(rho, dtype=complex).reshape(n, n)
+    lam = (C.c_int * len(lam))(*lam)
+    T = (C.c_double * len(T))(*T)
+    rr = np.ascontiguousarray(rho.real.ravel())
+    ri = np.ascontiguousarray(rho.imag.ravel())
+    return L().ora_spin_trace(J, len(lam), lam, T, th, ph, thR, phR, abi.dptr(rr), abi.dptr(ri))
+
+
+def unit(th, ph):
+    return np.array([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)])
+
+
+def test_spin_traces_reduce_to_the_angle_between_the_axes():
+    rng = np.random.default_rng(5)
+    T1 = [-0.697, -0.169, 0.697]
+    for _ in range(200):
+        th, thR = rng.uniform(0, np.pi, 2)
+        ph, phR = rng.uniform(-np.pi, np.pi, 2)
+        cb = unit(th, ph) @ unit(thR, phR)
+        sb2 = 1 - cb * cb
+        # f2 -> pi pi (and its production with T0 = 1), rho = diag(1/2, 0, 0, 0, 1/2)
+        t2 = trace(2, [0], [1.0], th, ph, thR, phR, np.diag([0.5, 0, 0, 0, 0.5]))
+        assert np.isclose(t2, 3 / 8 * sb2 * sb2, rtol=1e-12, atol=1e-15)
+        # vector decay to two spinless particles, rho = diag(1/2, 0, 1/2)
+        t1 = trace(1, [0], [1.0], th, ph, thR, phR, np.diag([0.5, 0, 0.5]))
+        assert np.isclose(t1, sb2 / 2, rtol=1e-12, atol=1e-15)
+        # vector production rows lambda = 1, 0, -1 with the fixed T0 row
+        tv = trace(1, [1, 0, -1], T1, th, ph, thR, phR, np.diag([0.5, 0, 0.5]))
+        want = (T1[0] ** 2 + T1[2] ** 2) * (1 + cb * cb) / 4 + T1[1] ** 2 * sb2 / 2
+        assert np.isclose(tv, want, rtol=1e-12, atol=1e-15)
+        # unpolarised: no angular dependence, sum |T|^2 / (2J + 1)
+        for J in (1, 2):
+            tu = trace(J, [0], [1.0], th, ph, thR, phR, np.eye(2 * J + 1) / (2 * J + 1))
+            assert np.isclose(tu, 1 / (2 * J + 1), rtol=1e-12)
+
+
+def boost_rest(P, p):
+    """p (E, x, y, z) in the rest frame of P"""
+    M = np.sqrt(P[0] ** 2 - P[1:] @ P[1:])
+    b = P[1:] / P[0]
+    g = P[0] / M
+    bp = b @ p[1:]
+    return np.concatenate([[g * (p[0] - bp)], p[1:] + (g * g * bp / (1 + g) - g * p[0]) * b])
+
+
+def cos_to_cs_axis(X, p, E, mb):
+    pz = np.sqrt(E * E - mb * mb)
+    b1 = boost_rest(X, np.array([E, 0, 0, pz]))[1:]
+    b2 = boost_rest(X, np.array([E, 0, 0, -pz]))[1:]
+    z = b1 / np.linalg.norm(b1) - b2 / np.linalg.norm(b2)
+    z /= np.linalg.norm(z)
+    q = boost_rest(X, p)[1:]
+    return q @ z / np.linalg.norm(q)
+
+
+def test_event_spin_factors_in_the_collins_soper_frame():
+    cfg = abi.cfg_default(6)
+    names = [cfg.res[r].pdg for r in range(cfg.n_res)]
+    rf2, rrho, rf0 = names.index(225), names.index(113), names.index(9010221)
+    R = ol.OracleRun(cfg, nthreads=1)
+    lib = L()
+    E, mb = cfg.beam_E1, cfg.m_beam
+    rng = np.random.default_rng(6)
+    n = 0
+    for x in rng.random((400, cfg.dim)):
+        x = np.ascontiguousarray(x)
+        ok, pf, _, _ = R.kinematics(x)
+        if not ok:
+            continue
+        X, p1, p3 = pf[0], pf[1], pf[3]
+        q1 = np.array([E, 0, 0, np.sqrt(E * E - mb * mb)]) - p1
+        cd, cq = cos_to_cs_axis(X, p3, E, mb), cos_to_cs_axis(X, q1, E, mb)
+        out = np.zeros(2)
+        assert lib.ora_res_spin(R.h, abi.dptr(x), rf2, abi.dptr(out)) == 1
+        assert np.isclose(out[1] ** 2, 5 * 3 / 8 * (1 - cd * cd) ** 2, rtol=1e-9, atol=1e-12)
+        assert np.isclose(out[0] ** 2, 5 * 3 / 8 * (1 - cq * cq) ** 2, rtol=1e-9, atol=1e-12)
+        assert lib.ora_res_spin(R.h, abi.dptr(x), rrho, abi.dptr(out)) == 1
+        assert np.isclose(out[1] ** 2, 3 * (1 - cd * cd) / 2, rtol=1e-9, atol=1e-12)
+        T = (-0.697, -0.169, 0.697)
+        want = 3 * ((T[0] ** 2 + T[2] ** 2) * (1 + cq * cq) / 4 + T[1] ** 2 * (1 - cq * cq) / 2)
+        assert np.isclose(out[0] ** 2, want, rtol=1e-9, atol=1e-12)
+        assert lib.ora_res_spin(R.h, abi.dptr(x), rf0, abi.dptr(out)) == 1
+        assert np.allclose(out, 1.0, rtol=1e-14)
+        n += 1
+    assert n > 100
+
+
+def test_decay_coupling_is_the_partial_width():
+    cfg = abi.cfg_default(6)
+    R = ol.OracleRun(cfg, nthreads=1)
+    for r in range(cfg.n_res):
+        q = cfg.res[r]
+        g = L().ora_res_gdecay(R.h, r)
+        pstar = np.sqrt(q.mass ** 2 / 4 - MPI ** 2)
+        assert np.isclose(g * g * pstar / (8 * np.pi * q.mass ** 2), q.BR * q.width, rtol=1e-12)
+
+
+def test_resonance_configuration_is_validated():
+    lib = abi.load()
+    h = C.c_void_p()
+    cfg = abi.cfg_default(6)
+    cfg.amp_precision = abi.GRA_AMP_FP32
+    assert lib.gra_hip_create(C.byref(cfg), C.byref(h)) == abi.GRA_E_ARG
+    assert b"FP64" in lib.gra_hip_last_error(None)
+    cfg = abi.cfg_default(6)
+    cfg.res[0].P = -1  # 0-: no decay to two spinless particles with parity conserved
+    assert lib.gra_hip_create(C.byref(cfg), C.byref(h)) == abi.GRA_E_ARG
+    cfg = abi.cfg_default(6)
+    cfg.res[2].mass = 0.2  # below the pi pi threshold
+    assert lib.gra_hip_create(C.byref(cfg), C.byref(h)) == abi.GRA_E_ARG
