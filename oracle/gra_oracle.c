@@ -31,6 +31,15 @@
  *                          convergence; sigma_tot/sigma_el ballpark (sanity only)
  *   screening loop         node amplitudes equal the table's interpolation; small
  *                          table limit; survival factor in (0.05, 0.5)
+ *   resonances (PP[RES+CON], tests/test_oracle_resonance.py)
+ *     Wigner d, traces     closed-form d^1, d^2 values and orthogonality; the
+ *                          spin-density traces equal sum_j rho_j |d^J_{j lam}(beta)|^2
+ *                          (beta: angle between the two axes); unpolarised rho
+ *                          gives no angular dependence
+ *     Collins-Soper frame  f2 and rho spin factors at generated events equal the
+ *                          closed forms in the CS angle computed from the 4-vectors
+ *     decay coupling       g^2 p* / (8 pi M^2) = BR Gamma
+ *     CBW, resonance form factor, PhotoProp, coherent photon flux: parity unpinned
  */
 #include <math.h>
 #include <pthread.h>
