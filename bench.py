@@ -150,7 +150,11 @@ WORKLOADS = {
     1: "pp->p pi+ pi- p 13 TeV, Regge continuum, unscreened, FP64",
     2: "pp->p pi+ pi- p 13 TeV, Regge continuum, screened (16x31 loop nodes), FP32 amplitude",
     3: "pp->p rho0 rho0 p -> p 2(pi+ pi-) p 13 TeV, tensor pomeron 2xV, screened (16x31 loop "
-       "nodes), FP64 amplitude",
+       "nodes), FP32 amplitude",
+    6: "pp->p pi+ pi- p 13 TeV, Regge continuum + 8 resonances (PP[RES+CON], ALICECUTS), "
+       "unscreened, FP64",
+    7: "pp->p pi+ pi- p 13 TeV, Regge continuum + 8 resonances (PP[RES+CON], ALICECUTS), "
+       "screened (16x31 loop nodes), FP64",
 }
 
 COPY_BYTES = 256 << 20

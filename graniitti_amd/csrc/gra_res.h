@@ -18,9 +18,10 @@ __device__ __forceinline__ double ipow(double x, int n) {
 }
 // Wigner small d^J_{m m'}(theta) as the finite sum over k (spin::WignerSmalld,
 // src/MSpin.cc:927-960), integer J <= 2
-__device__ __forceinline__ double wigner_d(double th, int m, int mp, int J) {
-  const double c = cos(th / 2.0), s = sin(th / 2.0);
+// with c = cos(theta / 2), s = sin(theta / 2)
+__device__ __forceinline__ double wigner_d(double c, double s, int m, int mp, int J) {
   double acc = 0.0;
+#pragma unroll
   for (int k = 0; k <= 2 * J; ++k) {
     if (J - mp - k >= 0 && J + m - k >= 0 && k + mp - m >= 0) {
       const double sg = ((mp - m + k) & 1) ? -1.0 : 1.0;
@@ -64,30 +65,60 @@ __device__ __forceinline__ void rho_rotation(const RunParams& P, const Kin& K, d
     theta_phi(q1b, thR, phR);
   }
 }
+// Wigner d-matrices of the rotation, dR[a][j] = d^J_{M_a M_j}(thR) for J = 1, 2 (rows and
+// columns -J..J), shared by every resonance and by production and decay at one point
+struct RotD {
+  double d1[3][3], d2[5][5];
+  double phR;
+};
+__device__ __forceinline__ void rot_d(double thR, double phR, RotD& D) {
+  const double c = cos(thR / 2.0), s = sin(thR / 2.0);
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) D.d1[a][j] = wigner_d(c, s, a - 1, j - 1, 1);
+#pragma unroll
+  for (int a = 0; a < 5; ++a)
+#pragma unroll
+    for (int j = 0; j < 5; ++j) D.d2[a][j] = wigner_d(c, s, a - 2, j - 2, 2);
+  D.phR = phR;
+}
 // Re Tr[f rho_rot f^dagger] with rho_rot = D^dagger rho D, D_aj = D^J(thR, phR, M_a, M_j),
 // and rows f_rj = T_r D^J(th, ph, lam_r, M_j) (spin::fMatrix, DMatrix, WignerD,
 // src/MSpin.cc:631-705, 968-970), contracted as sum_ab rho_ab g_a conj(g_b) with
 // g_a = sum_j f_rj conj(D_aj)
-__device__ __forceinline__ double spin_trace(const ResDev& R, int nrow, const int* lam,
-                                             const double* T, double th, double ph, double thR,
-                                             double phR) {
-  const int J = R.J, n = 2 * J + 1;
+template <int J, int NROW>
+__device__ __forceinline__ double spin_trace(const ResDev& R, const int* lam, const double* T,
+                                             double th, double ph, const RotD& D) {
+  constexpr int n = 2 * J + 1;
+  const double c = cos(th / 2.0), s = sin(th / 2.0);
+  // e^{i M (ph - phR)} for M = -J..J: the rotation's phase folded into the rows
+  const double dp = ph - D.phR;
+  const cxd e1 = cd(cos(dp), sin(dp));
+  cxd ph_[n];
+  ph_[J] = cd(1.0, 0.0);
+#pragma unroll
+  for (int k = 1; k <= J; ++k) {
+    ph_[J + k] = cmul(ph_[J + k - 1], e1);
+    ph_[J - k] = cd(ph_[J + k].re, -ph_[J + k].im);
+  }
   double tr = 0.0;
-  for (int r = 0; r < nrow; ++r) {
-    cxd g[5];
+#pragma unroll
+  for (int r = 0; r < NROW; ++r) {
+    cxd f[n];
+#pragma unroll
+    for (int j = 0; j < n; ++j) f[j] = cscale(ph_[j], T[r] * wigner_d(c, s, lam[r], j - J, J));
+    cxd g[n];
+#pragma unroll
     for (int a = 0; a < n; ++a) {
       cxd acc = cd(0.0, 0.0);
-      for (int j = 0; j < n; ++j) {
-        const int Mj = j - J;
-        const double fd = T[r] * wigner_d(th, lam[r], Mj, J);
-        const double Dd = wigner_d(thR, a - J, Mj, J);
-        const cxd f = cd(fd * cos(Mj * ph), fd * sin(Mj * ph));
-        const cxd Dc = cd(Dd * cos(Mj * phR), -(Dd * sin(Mj * phR)));
-        acc = cadd(acc, cmul(f, Dc));
-      }
+#pragma unroll
+      for (int j = 0; j < n; ++j) acc = cadd(acc, cscale(f[j], J == 1 ? D.d1[a][j] : D.d2[a][j]));
       g[a] = acc;
     }
+#pragma unroll
     for (int a = 0; a < n; ++a)
+#pragma unroll
       for (int b = 0; b < n; ++b) {
         const cxd rab = cd(R.rho_re[a * n + b], R.rho_im[a * n + b]);
         tr += cmul(rab, cmul(g[a], cd(g[b].re, -g[b].im))).re;
@@ -98,34 +129,33 @@ __device__ __forceinline__ double spin_trace(const ResDev& R, int nrow, const in
 // spin::ProdAmp (src/MSpin.cc:280-403) for J^P = 0+, 1-, 2+: "Pomeron effective
 // spins" s1 = s2 = 0 with T0 = 1 (tensors), s1 = 0, s2 = 1 with the fixed T0 row (vectors)
 __device__ __forceinline__ cxd prod_amp(const RunParams& P, const ResDev& R, double thq,
-                                        double phq, double thR, double phR) {
+                                        double phq, const RotD& D) {
   const cxd A0 = cd(R.g_re, R.g_im);
   if (!P.spin_gen || R.J == 0) return A0;
-  int lam[3] = {0, 0, 0};
-  double T[3] = {1.0, 0.0, 0.0};
-  int nrow = 1;
+  double tr;
   if (R.J == 1) {
-    nrow = 3;
-    lam[0] = 1;
-    lam[1] = 0;
-    lam[2] = -1;
-    T[0] = -0.697;
-    T[1] = -0.169;
-    T[2] = 0.697;
+    // s1 = 0, s2 = 1: lambda = 0 - lambda2 for lambda2 = -1, 0, 1, with the fixed T0 row
+    const int lam[3] = {1, 0, -1};
+    const double T[3] = {-0.697, -0.169, 0.697};
+    tr = spin_trace<1, 3>(R, lam, T, thq, phq, D);
+  } else {
+    const int lam[1] = {0};
+    const double T[1] = {1.0};
+    tr = spin_trace<2, 1>(R, lam, T, thq, phq, D);
   }
-  const double tr = spin_trace(R, nrow, lam, T, thq, phq, thR, phR);
   return cscale(A0, sqrt(2.0 * R.J + 1.0) * msq(tr));
 }
 // spin::DecayAmp (src/MSpin.cc:406-486) for two spinless daughters: f = fX, 1 x (2J+1).
 // |T_00| = 1 leaves the trace unchanged, so only J = 0 carries T_00's phase.
 __device__ __forceinline__ cxd decay_amp(const RunParams& P, const ResDev& R, double thd,
-                                         double phd, double thR, double phR) {
+                                         double phd, const RotD& D) {
   const double A0 = R.gdec;
   if (!P.spin_dec) return cd(A0, 0.0);
   if (R.J == 0) return cd(R.T_re * A0, R.T_im * A0);
   const int lam[1] = {0};
   const double T[1] = {sqrt(R.T_re * R.T_re + R.T_im * R.T_im)};
-  const double tr = spin_trace(R, 1, lam, T, thd, phd, thR, phR);
+  const double tr = R.J == 1 ? spin_trace<1, 1>(R, lam, T, thd, phd, D)
+                             : spin_trace<2, 1>(R, lam, T, thd, phd, D);
   return cd(A0 * (sqrt(2.0 * R.J + 1.0) * msq(tr)), 0.0);
 }
 // form::CBW (src/MForm.cc:743-828)
@@ -188,13 +218,15 @@ __device__ __forceinline__ cxd res_sum(const RunParams& P, const Kin& K) {
   theta_phi(q1b, thq, phq);
   theta_phi(boost_rest(X, M, K.dt[0]), thd, phd);
   rho_rotation(P, K, M, q1b, thR, phR);
+  RotD D;
+  rot_d(thR, phR, D);
   const double mA = mof(K.dt[0]), mB = mof(K.dt[1]);
   cxd A = cd(0.0, 0.0);
   for (int r = 0; r < P.nres; ++r) {
     const ResDev& R = P.res[r];
     const cxd bw = cbw(R, m2, mA, mB);
     const double rff = exp(-((m2 - R.M02) * (m2 - R.M02)) / R.gFF4);
-    const cxd spin = cmul(prod_amp(P, R, thq, phq, thR, phR), decay_amp(P, R, thd, phd, thR, phR));
+    const cxd spin = cmul(prod_amp(P, R, thq, phq, D), decay_amp(P, R, thd, phd, D));
     if (R.photo) {
       const dv4 q2 = v4sub(P.pbeam2, K.pf[2]);
       const double x1 = 1 - K.pf[1].z / P.pbeam1.z, x2 = 1 - K.pf[2].z / P.pbeam2.z;
