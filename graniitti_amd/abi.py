@@ -19,6 +19,7 @@ GRA_VEGAS_BINS = 128
 GRA_PROC_REGGE_CON = 15
 GRA_PROC_TENSOR_CON = 12
 GRA_PROC_RES_CON = 17
+GRA_PROC_RES = 9
 GRA_MAX_RES = 8
 GRA_BW_FIXED, GRA_BW_RUNNING, GRA_BW_BARRIER, GRA_BW_SPIN = 1, 2, 3, 4
 GRA_FRAME_CM, GRA_FRAME_HX, GRA_FRAME_CS, GRA_FRAME_GJ = 0, 1, 2, 3

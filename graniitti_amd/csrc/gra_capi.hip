@@ -154,16 +154,17 @@ int validate(const gra_cfg* c) {
     return set_global_error(GRA_E_ARG, "gra_cfg.abi_size = %d, library expects %d", c->abi_size,
                             (int)sizeof(gra_cfg));
   if (c->process != GRA_PROC_REGGE_CON && c->process != GRA_PROC_TENSOR_CON &&
-      c->process != GRA_PROC_RES_CON)
+      c->process != GRA_PROC_RES_CON && c->process != GRA_PROC_RES)
     return set_global_error(GRA_E_ARG, "unknown process %d", c->process);
-  if (c->process == GRA_PROC_RES_CON) {
-    if (c->phase_space != GRA_PS_C || c->w_central > 0.0 || c->m_daughter > 0.0)
+  if (c->process == GRA_PROC_RES_CON || c->process == GRA_PROC_RES) {
+    if (c->w_central > 0.0 || c->m_daughter > 0.0)
       return set_global_error(GRA_E_ARG,
-                              "PP[RES+CON]: built for the <C> class with stable central particles");
+                              "PP[RES(+CON)]: built for stable central particles");
     if (c->amp_precision != GRA_AMP_FP64)
-      return set_global_error(GRA_E_ARG, "PP[RES+CON]: resonance amplitudes run in FP64 only");
-    if (c->n_res < 0 || c->n_res > GRA_MAX_RES)
-      return set_global_error(GRA_E_ARG, "n_res %d outside 0..%d", c->n_res, GRA_MAX_RES);
+      return set_global_error(GRA_E_ARG, "PP[RES(+CON)]: resonance amplitudes run in FP64 only");
+    if (c->n_res < (c->process == GRA_PROC_RES ? 1 : 0) || c->n_res > GRA_MAX_RES)
+      return set_global_error(GRA_E_ARG, "n_res %d outside %d..%d", c->n_res,
+                              c->process == GRA_PROC_RES ? 1 : 0, GRA_MAX_RES);
     if (c->spin_frame < GRA_FRAME_CM || c->spin_frame > GRA_FRAME_GJ)
       return set_global_error(GRA_E_ARG, "spin_frame %d unknown", c->spin_frame);
     for (int r = 0; r < c->n_res; ++r) {
@@ -188,7 +189,8 @@ int validate(const gra_cfg* c) {
   if (c->phase_space != GRA_PS_C && c->phase_space != GRA_PS_F)
     return set_global_error(GRA_E_ARG, "phase_space %d unknown", c->phase_space);
   if (c->phase_space == GRA_PS_F) {
-    if (c->process != GRA_PROC_REGGE_CON || c->w_central > 0.0 || c->m_daughter > 0.0)
+    if ((c->process != GRA_PROC_REGGE_CON && c->process != GRA_PROC_RES_CON &&
+         c->process != GRA_PROC_RES) || c->w_central > 0.0 || c->m_daughter > 0.0)
       return set_global_error(GRA_E_ARG,
                               "<F> phase space: built for PP[CON] with stable central particles");
     const double Mmax = std::min(c->beam_E1 + c->beam_E2 - 2.0 * c->m_beam, c->gen_M_max);
@@ -407,7 +409,7 @@ void fill_params(const gra_cfg& c, RunParams& P) {
   P.tp_gPvv1 = c.tp_gPvv[1];
   P.tp_mp_ff = c.tp_mp_ff;
   P.res = nullptr;
-  P.nres = c.process == GRA_PROC_RES_CON ? c.n_res : 0;
+  P.nres = (c.process == GRA_PROC_RES_CON || c.process == GRA_PROC_RES) ? c.n_res : 0;
   P.spin_gen = c.spin_gen;
   P.spin_dec = c.spin_dec;
   P.spin_frame = c.spin_frame;

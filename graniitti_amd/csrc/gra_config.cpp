@@ -233,6 +233,19 @@ extern "C" int gra_hip_cfg_default(gra_cfg* cfg, int32_t preset) {
       cfg->screening = preset == 7;
       cfg->amp_precision = GRA_AMP_FP64;
       break;
+    case 8:
+      // tests/run_JW_frames/run.sh: "PP[RES]<F> -> pi+ pi- @RES{f2_1270:1}
+      // @R[f2_1270]{JZ0:0, JZ1:0, JZ2:1} @FRAME:CS"
+      set_pipi(cfg);
+      set_factorized(cfg);
+      set_res_con(cfg);
+      cfg->process = GRA_PROC_RES;
+      cfg->n_res = 0;
+      add_res(cfg, 225, 1.2755, 0.1867, 2, 0.7, -1.57, 0.562, 0.5);
+      cfg->fid_active = 0;
+      cfg->screening = 0;
+      cfg->amp_precision = GRA_AMP_FP64;
+      break;
     default:
       return set_global_error(GRA_E_ARG, "gra_hip_cfg_default: unknown preset %d", preset);
   }

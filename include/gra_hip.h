@@ -56,6 +56,7 @@ typedef struct gra_handle gra_handle; /* opaque: device state of one process */
 #define GRA_PROC_REGGE_CON 15  /* PROC_15 "PP[CON]" 2-body: MRegge::ME4, include/Graniitti/MSubProc.h:361-380 */
 #define GRA_PROC_TENSOR_CON 12 /* PROC_12 "PP[CONTENSOR]" 2xV: MTensorPomeron::ME4, MSubProc.h:288-300 */
 #define GRA_PROC_RES_CON 17    /* PROC_17 "PP[RES+CON]" 2-body: MRegge::ME4 + resonances, MSubProc.h:398-432 */
+#define GRA_PROC_RES 9         /* PROC_9 "PP[RES]": resonances only, MSubProc.h:233-262 */
 
 /* Resonances of PP[RES+CON] */
 #define GRA_MAX_RES 8
@@ -292,6 +293,9 @@ typedef struct {
  *      cuts plus f0(500), rho(770), f0(980), phi(1020), f2(1270), f0(1500), f2'(1525),
  *      f0(1710) with spin correlations in the Collins-Soper frame, FP64 amplitude
  *   7: preset 6 + eikonal screening as preset 2 (FP64 amplitude)
+ *   8: PP[RES]<F> -> pi+ pi- with f2(1270) alone, |J_z| = 2 (rho = diag(1/2, 0, 0, 0, 1/2)),
+ *      Collins-Soper frame, preset 4's generation cuts, no fiducial cuts
+ *      (tests/run_JW_frames/run.sh), no screening, FP64
  * Replaces the steering-card reader MGraniitti::ReadInput (src/MGraniitti.cc:318-329)
  * for the keys these workloads use, with TUNE0 values (modeldata/TUNE0/GENERAL.json). */
 int gra_hip_cfg_default(gra_cfg* cfg, int32_t preset);

@@ -1039,6 +1039,10 @@ static int tensor_me4(const ora_run* R, const kin_t* K, cx* hamp);
 /* Helicity amplitudes of the bare process; returns their count. */
 static int bare_amplitudes(const ora_run* R, const kin_t* K, cx* hamp) {
   if (R->c.process == GRA_PROC_TENSOR_CON) return tensor_me4(R, K, hamp);
+  if (R->c.process == GRA_PROC_RES) { /* PROC_9: resonances only */
+    hamp[0] = res_sum(R, K);
+    return 1;
+  }
   hamp[0] = regge_me4(R, K);
   if (R->c.process == GRA_PROC_RES_CON) hamp[0] = cx_add(hamp[0], res_sum(R, K));
   return 1;

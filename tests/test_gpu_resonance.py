@@ -74,3 +74,71 @@ def test_res_con_events_and_f2_peak():
     side = ((MX > 1.6) & (MX < 1.9)).sum() / 0.3
     say(f"events per GeV: f2 band {band:.0f}, 1.6-1.9 GeV {side:.0f}")
     assert band > 2 * side
+
+
+def test_res_f_pointwise():
+    cfg = abi.cfg_default(8)
+    say("case: PP[RES]<F> f2(1270) FP64, 20000 points, seed 44")
+    u = points(20000, cfg.dim, 44)
+    with MContinuum(cfg) as P:
+        Wg, ag, fg = P.eval(u)
+    Wo, ao, fo = ol.OracleRun(cfg).eval(u)
+    assert ((fo & 7) == 7).sum() > 10000
+    # a lone f2 has |A|^2 ~ sin^4 theta_CS sin^4 theta_q, which vanishes on the axes; there
+    # the spin trace is a small difference of O(1) terms, formed as D^+ rho D literally by
+    # the oracle and contracted as sum rho_ab g_a g_b* on the GPU, so the two roundings
+    # differ by ~eps / trace relative: 1e-9 bound, and the bulk at rounding level
+    rel = compare(Wg, Wo, fg, fo, 1e-9, "RES<F> fp64")
+    assert np.median(rel) < 1e-14
+
+
+def test_res_f_screened_pointwise():
+    cfg = small_tables(abi.cfg_default(8), 1000)
+    cfg.screening = 1
+    say("case: PP[RES]<F> f2(1270) screened FP64, 1500 points, seed 45")
+    u = points(1500, cfg.dim, 45)
+    with MContinuum(cfg) as P:
+        Wg, ag, fg = P.eval(u)
+    Wo, ao, fo = ol.OracleRun(cfg).eval(u)
+    compare(Wg, Wo, fg, fo, 1e-10, "RES<F> screened fp64")
+
+
+def frame_cos(p, frame, E, mb):
+    """cos of the pi+ polar angle in the system rest frame about the frame's axis"""
+    from test_oracle_resonance import boost_rest, cos_to_cs_axis
+    out = np.empty(len(p))
+    for i in range(len(p)):
+        X, pip = p[i, 2] + p[i, 3], p[i, 2]
+        if frame == abi.GRA_FRAME_CS:
+            out[i] = cos_to_cs_axis(X, pip, E, mb)
+            continue
+        q = boost_rest(X, pip)[1:]
+        axis = X[1:] / np.linalg.norm(X[1:]) if frame == abi.GRA_FRAME_HX else np.array([0, 0, 1.0])
+        out[i] = q @ axis / np.linalg.norm(q)
+    return out
+
+
+@pytest.mark.parametrize("frame", [abi.GRA_FRAME_CS, abi.GRA_FRAME_HX, abi.GRA_FRAME_CM])
+def test_jw_frames_decay_angle(frame):
+    # tests/run_JW_frames/run.sh: f2(1270) with |J_z| = 2 in the chosen frame decays as
+    # |d^2_{20}|^2 ~ (1 - cos^2)^2 about that frame's axis; without fiducial cuts the
+    # decay direction is isotropic in phase space, so unweighted events follow it
+    cfg = abi.cfg_default(8)
+    cfg.spin_frame = frame
+    say(f"case: 20000 unweighted PP[RES]<F> f2(1270) events, frame {frame}")
+    with MContinuum(cfg) as P:
+        G = MGraniitti(P)
+        G.Integrate(vegas_cfg(1 << 18, 4, burnin=3, seed=21 + frame))
+        p, f, tot = G.Generate(20000, calls=1 << 20, seed=7 + frame)
+    edges = np.linspace(-1, 1, 21)
+    F = lambda c: (15.0 / 16.0) * (c - 2 * c ** 3 / 3 + c ** 5 / 5)
+    exp_pol = len(p) * np.diff(F(edges))
+    res = {}
+    for fr in (abi.GRA_FRAME_CS, abi.GRA_FRAME_HX, abi.GRA_FRAME_CM):
+        h, _ = np.histogram(frame_cos(p, fr, cfg.beam_E1, cfg.m_beam), edges)
+        res[fr] = ((h - exp_pol) ** 2 / exp_pol).sum()
+    say(f"frame {frame}: chi2/19 against (1 - c^2)^2 about CS {res[abi.GRA_FRAME_CS]:.1f}, "
+        f"HX {res[abi.GRA_FRAME_HX]:.1f}, CM {res[abi.GRA_FRAME_CM]:.1f}")
+    assert res[frame] < 19 + 5 * np.sqrt(2 * 19)
+    # the helicity axis (the system's direction) differs from the near-beam CS and CM axes
+    assert max(v for fr, v in res.items() if fr != frame) > 4 * res[frame]
