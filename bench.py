@@ -160,6 +160,41 @@ WORKLOADS = {
 COPY_BYTES = 256 << 20
 
 
+def res_leg():
+    """PP[RES+CON] -> pi+ pi- (continuum + the eight ALICECUTS resonances, FP64 amplitude):
+    VEGAS throughput unscreened (2^20 points/iteration) and screened (2^18)."""
+    out = {}
+    for preset, n, warm, steps in ((6, 1 << 20, 2, 5), (7, 1 << 18, 1, 3)):
+        cfg = abi.cfg_default(preset)
+        with MContinuum(cfg) as P:
+            grid = abi.GraGrid()
+            abi.check(P.lib.gra_hip_grid_init(C.byref(grid), cfg.dim, 128))
+            it, fs, mx, acc = 0, 0.0, 0.0, [0.0, 0.0]
+            for k in range(warm + steps):
+                if k == warm:
+                    t0 = time.perf_counter()
+                st = P.vegas_iterate(grid, n, 4242, it)
+                it += 1
+                abi.check(P.lib.gra_hip_grid_optimize(C.byref(grid), C.byref(st), 1.5))
+                if k >= warm:
+                    fs += st.fsum
+                    mx = max(mx, st.maxf)
+                    I = st.fsum / n
+                    v = max((st.f2sum / n - I * I) / n, 1e-300)
+                    acc[0] += I / v
+                    acc[1] += 1.0 / v
+            dt = time.perf_counter() - t0
+        pps = n * steps / dt
+        eff = fs / (n * steps) / mx if mx > 0 else 0.0
+        out["screened" if preset == 7 else "unscreened"] = {
+            "preset": preset, "points_per_step": n, "steps": steps, "points_per_s": pps,
+            "unweighting_efficiency": eff, "unweighted_events_per_s": pps * eff,
+            "sigma_barn": acc[0] / acc[1], "sigma_err_barn": acc[1] ** -0.5}
+        log(f"PP[RES+CON] preset {preset}: {pps:.3e} points/s, eff {eff:.4f}, "
+            f"sigma {acc[0] / acc[1]:.5e} barn")
+    return out
+
+
 def calibrate_copy(reps=3):
     """Device-to-device copies of COPY_BYTES (the runtime's blit kernel): the
     reference dispatches that scale the FETCH_SIZE / WRITE_SIZE counters of a
@@ -195,6 +230,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--points", type=int, default=1 << 20)
     ap.add_argument("--preset", type=int, default=2)
+    ap.add_argument("--no-res", action="store_true",
+                    help="skip the PP[RES+CON] resonance throughput leg")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-precision-run", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=1 << 20)
@@ -293,6 +330,12 @@ def main():
                     "seconds_total": res.seconds_total, "calls_total": int(res.calls_total)}
         log(f"precision run: {prec_res}, t(0.1%) = {t_prec}")
     flow = None
+    res = None
+    if not args.no_res and args.preset == 2:
+        try:
+            res = res_leg()
+        except Exception as e:  # reported, never fatal to the headline line
+            res = {"error": str(e)}
     if not args.no_flow:
         flow = flow_leg(P, grid, args, prec_res["sigma"] if prec_res else sigma,
                         prec_res["sigma_err"] if prec_res else sigma_err)
@@ -313,7 +356,7 @@ def main():
         "time_to_0p1pct_s": t_prec, "precision_run": prec_res,
         "sigma_barn": sigma, "sigma_err_barn": sigma_err,
         "kernel_ms": {"tree": ms_tree, "screen": ms_scr, "accumulate": ms_acc},
-        "flow": flow, "roofline": roofline, "cpu_baseline": cpu,
+        "flow": flow, "resonances": res, "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(line))
 
