@@ -142,3 +142,32 @@ def test_jw_frames_decay_angle(frame):
     assert res[frame] < 19 + 5 * np.sqrt(2 * 19)
     # the helicity axis (the system's direction) differs from the near-beam CS and CM axes
     assert max(v for fr, v in res.items() if fr != frame) > 4 * res[frame]
+
+
+@pytest.mark.parametrize("variant", ["bw_fixed_gj", "bw_barrier_hx", "bw_spin_cm", "no_spin_corr"])
+def test_res_con_variants_pointwise(variant):
+    # the Breit-Wigner forms, spin frames and spin switches the ALICECUTS preset leaves unused
+    cfg = abi.cfg_default(6)
+    bw, frame = {"bw_fixed_gj": (abi.GRA_BW_FIXED, abi.GRA_FRAME_GJ),
+                 "bw_barrier_hx": (abi.GRA_BW_BARRIER, abi.GRA_FRAME_HX),
+                 "bw_spin_cm": (abi.GRA_BW_SPIN, abi.GRA_FRAME_CM),
+                 "no_spin_corr": (abi.GRA_BW_RUNNING, abi.GRA_FRAME_CS)}[variant]
+    for r in range(cfg.n_res):
+        cfg.res[r].BW = bw
+    cfg.spin_frame = frame
+    if frame == abi.GRA_FRAME_GJ:
+        # the GJ axis is q1 itself, so an f2 with |J_z| = 2 has a production trace of
+        # exactly 0 and ProdAmp = sqrt(rounding noise), different in every evaluation
+        # order: give the tensors the unpolarised density instead
+        for r in range(cfg.n_res):
+            if cfg.res[r].J == 2:
+                for k in range(25):
+                    cfg.res[r].rho_re[k] = 0.2 if k % 6 == 0 else 0.0
+    if variant == "no_spin_corr":
+        cfg.spin_gen = cfg.spin_dec = 0
+    say(f"case: PP[RES+CON] {variant}, 10000 points, seed 46")
+    u = points(10000, cfg.dim, 46)
+    with MContinuum(cfg) as P:
+        Wg, ag, fg = P.eval(u)
+    Wo, ao, fo = ol.OracleRun(cfg).eval(u)
+    compare(Wg, Wo, fg, fo, 1e-10, f"RES+CON {variant}")
