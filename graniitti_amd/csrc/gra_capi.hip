@@ -414,7 +414,6 @@ void fill_params(const gra_cfg& c, RunParams& P) {
   P.spin_dec = c.spin_dec;
   P.spin_frame = c.spin_frame;
   P.omega = c.regge_omega;
-  P.sgn0 = 1.0;
   P.m_rho = c.m_central;
 }
 

@@ -67,7 +67,7 @@ struct RunParams {
   // resonances (PP[RES+CON]): device table of nres entries
   const ResDev* res;
   int32_t nres, spin_gen, spin_dec, spin_frame;
-  double omega, sgn0;  // PARAM_REGGE::omega; beam pdg product sign for PhotoME3 (+1: pp)
+  double omega;  // PARAM_REGGE::omega
 };
 
 // Screening node constants: one entry per (phi_i, kt_j) node, i-major.

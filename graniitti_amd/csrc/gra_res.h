@@ -235,6 +235,7 @@ __device__ __forceinline__ cxd res_sum(const RunParams& P, const Kin& K) {
       const cxd common = cscale(bw, rff);
       const cxd A1 = cmul(cscale(common, gf2), photo_prop(P, s1, K.t1, R.B0));
       const cxd A2 = cmul(cscale(common, gf1), photo_prop(P, s2, K.t2, R.B0));
+      // identical beams (pp): A_1 + A_2 (src/MRegge.cc:813)
       A = cadd(A, cmul(cadd(A1, A2), spin));
     } else {
       cxd Ap = cscale(pr1, 2.0);
