@@ -26,7 +26,8 @@ GRA_FRAME_CM, GRA_FRAME_HX, GRA_FRAME_CS, GRA_FRAME_GJ = 0, 1, 2, 3
 GRA_AMP_FP64 = 0
 GRA_AMP_FP32 = 1
 GRA_PS_C, GRA_PS_F = 0, 1
-FLAG_KIN_OK, FLAG_FID_OK, FLAG_VETO_OK, FLAG_AMP_OK = 1, 2, 4, 8
+FLAG_KIN_OK, FLAG_FID_OK, FLAG_VETO_OK, FLAG_AMP_OK, FLAG_INF = 1, 2, 4, 8, 16
+GRA_MAX_NONFINITE = 50
 
 D = C.c_double
 I32 = C.c_int32
@@ -61,6 +62,7 @@ class GraCfg(C.Structure):
         ("regge_s0", D), ("b_EXP", D), ("a_OREAR", D), ("b_OREAR", D), ("b_POW", D),
         ("m_prop", D),
         ("tp_delta_P", D), ("tp_ap_P", D), ("tp_gPNN", D), ("tp_gPvv", D * 2), ("tp_mp_ff", D),
+        ("tp_gPpipi", D), ("tp_gPKK", D), ("central_spin2", I32), ("pad_tp", I32),
         ("screening", I32), ("n_loop_phi", I32), ("n_loop_kt", I32), ("n_bt", I32),
         ("n_kt2", I32), ("n_fb", I32),
         ("min_loop_kt", D), ("max_loop_kt", D),
@@ -77,7 +79,8 @@ class GraGrid(C.Structure):
 class GraIterStats(C.Structure):
     _fields_ = [
         ("fsum", D), ("fsum_c", D), ("f2sum", D), ("f2sum_c", D), ("maxW", D), ("maxf", D),
-        ("n_calls", U64), ("n_kin_ok", U64), ("n_fid_ok", U64), ("n_valid", U64), ("n_nan", U64),
+        ("n_calls", U64), ("n_kin_ok", U64), ("n_fid_ok", U64), ("n_valid", U64),
+        ("n_nan", U64), ("n_inf", U64),
         ("fmat", D * (GRA_MAX_DIM * GRA_VEGAS_BINS)), ("f2mat", D * (GRA_MAX_DIM * GRA_VEGAS_BINS)),
     ]
 
@@ -93,7 +96,7 @@ class GraVegasResult(C.Structure):
     _fields_ = [
         ("sigma", D), ("sigma_err", D), ("chi2", D), ("maxW", D), ("maxf", D),
         ("seconds", D), ("seconds_total", D), ("seconds_to_precision", D),
-        ("calls_total", U64), ("iterations", I32), ("pad0", I32), ("kernel_ms", D),
+        ("calls_total", U64), ("iterations", I32), ("restarts", I32), ("kernel_ms", D),
     ]
 
 
@@ -128,6 +131,8 @@ class GraGenStats(C.Structure):
 # Every function the header declares, with its ctypes signature
 P = C.POINTER
 VP = C.c_void_p
+VEGAS_PASS = C.CFUNCTYPE(C.c_int, VP, C.POINTER(GraGrid), U64, U64, C.c_uint32,
+                         C.POINTER(GraIterStats))
 SIGNATURES = {
     "gra_hip_cfg_default": (C.c_int, [P(GraCfg), I32]),
     "gra_hip_create": (C.c_int, [P(GraCfg), P(VP)]),
@@ -144,6 +149,8 @@ SIGNATURES = {
     "gra_hip_grid_optimize": (C.c_int, [P(GraGrid), P(GraIterStats), D]),
     "gra_hip_vegas_iterate": (C.c_int, [VP, P(GraGrid), U64, U64, C.c_uint32, P(GraIterStats)]),
     "gra_hip_vegas_run": (C.c_int, [VP, P(GraVegasCfg), P(GraGrid), P(GraVegasResult)]),
+    "gra_vegas_run_host": (C.c_int, [P(GraVegasCfg), P(GraGrid), I32, VEGAS_PASS, VP,
+                                     P(GraVegasResult)]),
     "gra_hip_vegas_map": (C.c_int, [VP, P(GraGrid), P(D), C.c_size_t, P(D), P(D), P(I32)]),
     "gra_hip_generate": (C.c_int, [VP, P(GraGrid), U64, U64, C.c_uint32, D, P(GraEvent), U64,
                                    P(GraGenStats)]),

@@ -148,6 +148,72 @@ double simpson38_w(int i, int N) {
   return (i % 3 == 0) ? 2.0 : 3.0;
 }
 
+// Spin density of a resonance with J > 0 (spin::Positivity, src/MSpin.cc:988-1110, called
+// from form::ReadResonance, src/MForm.cc:361): |Tr rho| within 1e-2 of 1 as the
+// reference requires; Hermitian and without negative eigenvalue. The reference's
+// eigenvalue test compares against -1e6 (src/MSpin.cc:1012), so it accepts any
+// matrix; here a negative eigenvalue below -1e-9 is rejected. The eigenvalues
+// come from the real symmetric form [[Re, -Im], [Im, Re]] (each eigenvalue of
+// rho twice) by cyclic Jacobi rotations.
+int check_rho(const gra_res& q, int r) {
+  if (q.J == 0) return GRA_OK;
+  const int n = 2 * q.J + 1;
+  double tr_re = 0.0, tr_im = 0.0;
+  for (int a = 0; a < n; ++a) {
+    tr_re += q.rho_re[a * n + a];
+    tr_im += q.rho_im[a * n + a];
+  }
+  const double tr = std::hypot(tr_re, tr_im);
+  if (std::fabs(tr - 1.0) > 1e-2)
+    return set_global_error(GRA_E_ARG,
+                            "resonance %d: spin density not normalised (|Tr rho| = %g, should be 1)",
+                            r, tr);
+  for (int a = 0; a < n; ++a)
+    for (int b = 0; b < n; ++b)
+      if (std::fabs(q.rho_re[a * n + b] - q.rho_re[b * n + a]) > 1e-9 ||
+          std::fabs(q.rho_im[a * n + b] + q.rho_im[b * n + a]) > 1e-9)
+        return set_global_error(GRA_E_ARG, "resonance %d: spin density is not Hermitian", r);
+  const int m = 2 * n;
+  double A[10][10];
+  for (int a = 0; a < n; ++a)
+    for (int b = 0; b < n; ++b) {
+      const double re = q.rho_re[a * n + b], im = q.rho_im[a * n + b];
+      A[a][b] = re;
+      A[n + a][n + b] = re;
+      A[a][n + b] = -im;
+      A[n + a][b] = im;
+    }
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    double off = 0.0;
+    for (int p = 0; p < m; ++p)
+      for (int k = p + 1; k < m; ++k) off += A[p][k] * A[p][k];
+    if (off < 1e-30) break;
+    for (int p = 0; p < m; ++p)
+      for (int k = p + 1; k < m; ++k) {
+        if (A[p][k] == 0.0) continue;
+        const double th = (A[k][k] - A[p][p]) / (2.0 * A[p][k]);
+        const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+        const double cs = 1.0 / std::sqrt(t * t + 1.0), sn = t * cs;
+        for (int i = 0; i < m; ++i) {
+          const double ap = A[i][p], ak = A[i][k];
+          A[i][p] = cs * ap - sn * ak;
+          A[i][k] = sn * ap + cs * ak;
+        }
+        for (int i = 0; i < m; ++i) {
+          const double ap = A[p][i], ak = A[k][i];
+          A[p][i] = cs * ap - sn * ak;
+          A[k][i] = sn * ap + cs * ak;
+        }
+      }
+  }
+  for (int p = 0; p < m; ++p)
+    if (A[p][p] < -1e-9)
+      return set_global_error(GRA_E_ARG,
+                              "resonance %d: spin density has a negative eigenvalue (%g)", r,
+                              A[p][p]);
+  return GRA_OK;
+}
+
 int validate(const gra_cfg* c) {
   if (!c) return set_global_error(GRA_E_ARG, "configuration is NULL");
   if (c->abi_size != (int32_t)sizeof(gra_cfg))
@@ -156,6 +222,30 @@ int validate(const gra_cfg* c) {
   if (c->process != GRA_PROC_REGGE_CON && c->process != GRA_PROC_TENSOR_CON &&
       c->process != GRA_PROC_RES_CON && c->process != GRA_PROC_RES)
     return set_global_error(GRA_E_ARG, "unknown process %d", c->process);
+  if (c->process == GRA_PROC_TENSOR_CON) {
+    // MTensorPomeron::ME4 deduces its spin mode from the central particles' spins
+    // (src/MTensorPomeron.cc:643-654)
+    if (c->central_spin2 == 2) {
+      if (!(c->w_central > 0.0) || !(c->m_daughter > 0.0))
+        return set_global_error(GRA_E_ARG,
+                                "PP[CONTENSOR] 2xV: built for vector mesons with a width and a "
+                                "two-body decay (w_central > 0, m_daughter > 0)");
+    } else if (c->central_spin2 == 0) {
+      if (c->w_central > 0.0 || c->m_daughter > 0.0)
+        return set_global_error(GRA_E_ARG,
+                                "PP[CONTENSOR] 2xS: built for a stable pseudoscalar pair "
+                                "(w_central = 0, m_daughter = 0)");
+      if (c->amp_precision != GRA_AMP_FP64)
+        return set_global_error(GRA_E_ARG, "PP[CONTENSOR] 2xS: the amplitude runs in FP64 only");
+    } else if (c->central_spin2 == 1) {
+      return set_global_error(GRA_E_ARG,
+                              "PP[CONTENSOR] 2xF (fermion pair) is not built");
+    } else {
+      return set_global_error(GRA_E_ARG,
+                              "MTensorPomeron::ME4: invalid daughter spin 2J = %d (J = 0, 1/2, 1 "
+                              "pairs supported)", c->central_spin2);
+    }
+  }
   if (c->process == GRA_PROC_RES_CON || c->process == GRA_PROC_RES) {
     if (c->w_central > 0.0 || c->m_daughter > 0.0)
       return set_global_error(GRA_E_ARG,
@@ -180,6 +270,8 @@ int validate(const gra_cfg* c) {
                                 "resonance %d: needs M > 2 m_central, W > 0, BR > 0, g_FF > 0", r);
       if (q.alpha_re * q.alpha_re + q.alpha_im * q.alpha_im <= 0.0)
         return set_global_error(GRA_E_ARG, "resonance %d: alpha_{J0} = 0 (decay impossible)", r);
+      const int rv = check_rho(q, r);
+      if (rv != GRA_OK) return rv;
     }
   }
   const int dmin = (c->m_daughter > 0.0) ? 14 : (c->w_central > 0.0 ? 10 : 8);
@@ -190,9 +282,11 @@ int validate(const gra_cfg* c) {
     return set_global_error(GRA_E_ARG, "phase_space %d unknown", c->phase_space);
   if (c->phase_space == GRA_PS_F) {
     if ((c->process != GRA_PROC_REGGE_CON && c->process != GRA_PROC_RES_CON &&
-         c->process != GRA_PROC_RES) || c->w_central > 0.0 || c->m_daughter > 0.0)
+         c->process != GRA_PROC_RES &&
+         !(c->process == GRA_PROC_TENSOR_CON && c->central_spin2 == 0)) ||
+        c->w_central > 0.0 || c->m_daughter > 0.0)
       return set_global_error(GRA_E_ARG,
-                              "<F> phase space: built for PP[CON] with stable central particles");
+                              "<F> phase space: built for two stable central particles");
     const double Mmax = std::min(c->beam_E1 + c->beam_E2 - 2.0 * c->m_beam, c->gen_M_max);
     if (!(std::max(2.0 * c->m_central + 1e-4, c->gen_M_min) < Mmax))
       return set_global_error(GRA_E_ARG,
@@ -408,6 +502,9 @@ void fill_params(const gra_cfg& c, RunParams& P) {
   P.tp_gPvv0 = c.tp_gPvv[0];
   P.tp_gPvv1 = c.tp_gPvv[1];
   P.tp_mp_ff = c.tp_mp_ff;
+  P.tp_mode = (c.process == GRA_PROC_TENSOR_CON && c.central_spin2 == 0) ? kTp2xS : kTp2xV;
+  // 2xS: pion or kaon coupling by the central mass (src/MTensorPomeron.cc:730)
+  P.tp_gP = (c.m_central < 0.2) ? c.tp_gPpipi : c.tp_gPKK;
   P.res = nullptr;
   P.nres = (c.process == GRA_PROC_RES_CON || c.process == GRA_PROC_RES) ? c.n_res : 0;
   P.spin_gen = c.spin_gen;
@@ -559,8 +656,8 @@ static bool f32_kernels(const RunParams& P) { return P.amp_mode == GRA_AMP_FP32;
 
 int ensure_point_buffers(gra_handle* h, uint64_t n) {
   const int D = h->P.dim;
-  const size_t recb = f32_kernels(h->P) ? k32::loop_record_bytes(h->P.process)
-                                        : k64::loop_record_bytes(h->P.process);
+  const size_t recb = f32_kernels(h->P) ? k32::loop_record_bytes(h->P)
+                                        : k64::loop_record_bytes(h->P);
   if (h->d_W.ensure(sizeof(double) * n) || h->d_amp2.ensure(sizeof(double) * n) ||
       h->d_flags.ensure(n) || h->d_vw.ensure(sizeof(double) * n) ||
       h->d_bins.ensure((size_t)D * n) || h->d_ncand.ensure(16) ||
@@ -588,28 +685,39 @@ static void time_collect(gra_handle* h, bool tree, bool screen, bool accum) {
   }
 }
 
+// The integrand kernels on the handle's stream: the tree stage appends the
+// points that need screening to rec through the counter ncand (zero on entry),
+// the screening stage completes them. timed: record the kernel durations.
+static int launch_integrand(gra_handle* h, const double* d_grid, unsigned int bins, uint64_t seed,
+                            uint32_t iter, uint64_t first, const double* d_x, uint64_t n,
+                            double* d_W, double* d_amp2, uint8_t* d_flags, double* d_vw,
+                            uint8_t* d_bins, void* rec, uint32_t* ncand, bool timed) {
+  const RunParams& P = h->P;
+  const bool f32 = f32_kernels(P);
+  if (timed) time_begin(h, TS_TREE);
+  GRA_HIP_CHECK(h, f32 ? k32::launch_tree(P, d_grid, bins, seed, iter, first, d_x, n, d_W, d_amp2,
+                                          d_flags, d_vw, d_bins, rec, ncand, h->stream)
+                       : k64::launch_tree(P, d_grid, bins, seed, iter, first, d_x, n, d_W, d_amp2,
+                                          d_flags, d_vw, d_bins, rec, ncand, h->stream));
+  if (timed) time_end(h, TS_TREE);
+  if (P.screening) {
+    if (timed) time_begin(h, TS_SCREEN);
+    GRA_HIP_CHECK(h, f32 ? k32::launch_screen(P, h->d_nodes.p, rec, ncand, n, d_W, d_amp2, d_flags,
+                                              h->stream)
+                         : k64::launch_screen(P, h->d_nodes.p, rec, ncand, n, d_W, d_amp2, d_flags,
+                                              h->stream));
+    if (timed) time_end(h, TS_SCREEN);
+  }
+  return GRA_OK;
+}
+
 int run_integrand(gra_handle* h, const double* d_grid, unsigned int bins, uint64_t seed,
                   uint32_t iter, uint64_t first, const double* d_x, uint64_t n, double* d_W,
                   double* d_amp2, uint8_t* d_flags, double* d_vw, uint8_t* d_bins) {
-  const RunParams& P = h->P;
-  const bool f32 = f32_kernels(P);
   uint32_t* ncand = h->d_ncand.as<uint32_t>();
   GRA_HIP_CHECK(h, hipMemsetAsync(ncand, 0, sizeof(uint32_t), h->stream));
-  time_begin(h, TS_TREE);
-  GRA_HIP_CHECK(h, f32 ? k32::launch_tree(P, d_grid, bins, seed, iter, first, d_x, n, d_W, d_amp2,
-                                          d_flags, d_vw, d_bins, h->d_rec.p, ncand, h->stream)
-                       : k64::launch_tree(P, d_grid, bins, seed, iter, first, d_x, n, d_W, d_amp2,
-                                          d_flags, d_vw, d_bins, h->d_rec.p, ncand, h->stream));
-  time_end(h, TS_TREE);
-  if (P.screening) {
-    time_begin(h, TS_SCREEN);
-    GRA_HIP_CHECK(h, f32 ? k32::launch_screen(P, h->d_nodes.p, h->d_rec.p, ncand, n, d_W, d_amp2,
-                                              d_flags, h->stream)
-                         : k64::launch_screen(P, h->d_nodes.p, h->d_rec.p, ncand, n, d_W, d_amp2,
-                                              d_flags, h->stream));
-    time_end(h, TS_SCREEN);
-  }
-  return GRA_OK;
+  return launch_integrand(h, d_grid, bins, seed, iter, first, d_x, n, d_W, d_amp2, d_flags, d_vw,
+                          d_bins, h->d_rec.p, ncand, true);
 }
 
 }  // namespace gra_hip
@@ -750,10 +858,34 @@ extern "C" int gra_hip_eval_points(gra_handle* h, const double* u, size_t n, dou
   if (amp2)
     GRA_HIP_CHECK(h, hipMemcpyAsync(amp2, h->d_amp2.p, sizeof(double) * n, hipMemcpyDeviceToHost,
                                     h->stream));
-  if (flags)
-    GRA_HIP_CHECK(h, hipMemcpyAsync(flags, h->d_flags.p, n, hipMemcpyDeviceToHost, h->stream));
+  std::vector<uint8_t> own;
+  if (!flags) {
+    own.resize(n);
+    flags = own.data();
+  }
+  GRA_HIP_CHECK(h, hipMemcpyAsync(flags, h->d_flags.p, n, hipMemcpyDeviceToHost, h->stream));
   GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
   time_collect(h, true, h->P.screening != 0, false);
+  uint64_t n_nan = 0, n_inf = 0;
+  for (size_t i = 0; i < n; ++i)
+    if (!(flags[i] & GRA_FLAG_AMP_OK)) ++((flags[i] & GRA_FLAG_INF) ? n_inf : n_nan);
+  return count_nonfinite(h, n_nan, n_inf);
+}
+
+// MProcess::CheckInfNan (include/Graniitti/MProcess.h:317-340): NaN and Inf
+// weights are set to 0 and counted over the life of the process object (here the
+// handle); the run fails once either count exceeds 50.
+int gra_hip::count_nonfinite(gra_handle* h, uint64_t n_nan, uint64_t n_inf) {
+  h->n_nan_seen += n_nan;
+  h->n_inf_seen += n_inf;
+  if (h->n_nan_seen > GRA_MAX_NONFINITE)
+    return fail(h, GRA_E_NUMERIC,
+                "CheckInfNan: too many NaN weights (%llu) - check model parameters and cuts",
+                (unsigned long long)h->n_nan_seen);
+  if (h->n_inf_seen > GRA_MAX_NONFINITE)
+    return fail(h, GRA_E_NUMERIC,
+                "CheckInfNan: too many Inf weights (%llu) - check model parameters and cuts",
+                (unsigned long long)h->n_inf_seen);
   return GRA_OK;
 }
 
@@ -782,9 +914,9 @@ extern "C" int gra_hip_vegas_iterate(gra_handle* h, const gra_grid* grid, uint64
   const int nblk = (int)std::min<uint64_t>(8192, std::max<uint64_t>(1, (n_calls + 511) / 512));
   const size_t H = 2 * (size_t)D * GRA_VEGAS_BINS;
   if (h->d_grid.ensure(sizeof(double) * GRA_MAX_DIM * GRA_VEGAS_BINS) ||
-      h->d_part_hist.ensure(sizeof(double) * (H * (nblk + 32) + 32 * 10)) ||
-      h->d_part_scal.ensure(sizeof(double) * 10 * nblk) ||
-      h->d_stats.ensure(sizeof(double) * (10 + H)))
+      h->d_part_hist.ensure(sizeof(double) * (H * (nblk + 32) + 32 * kIterScalars)) ||
+      h->d_part_scal.ensure(sizeof(double) * kIterScalars * nblk) ||
+      h->d_stats.ensure(sizeof(double) * (kIterScalars + H)))
     return fail(h, GRA_E_NOMEM, "VEGAS buffers");
   GRA_HIP_CHECK(h, hipMemcpyAsync(h->d_grid.p, grid->xmat, sizeof(double) * D * GRA_VEGAS_BINS,
                                   hipMemcpyHostToDevice, h->stream));
@@ -799,8 +931,8 @@ extern "C" int gra_hip_vegas_iterate(gra_handle* h, const gra_grid* grid, uint64
                                      h->d_part_scal.as<double>(), h->d_stats.as<double>(),
                                      h->stream));
   time_end(h, TS_ACCUM);
-  h->h_stats.resize(10 + H);
-  GRA_HIP_CHECK(h, hipMemcpyAsync(h->h_stats.data(), h->d_stats.p, sizeof(double) * (10 + H),
+  h->h_stats.resize(kIterScalars + H);
+  GRA_HIP_CHECK(h, hipMemcpyAsync(h->h_stats.data(), h->d_stats.p, sizeof(double) * (kIterScalars + H),
                                   hipMemcpyDeviceToHost, h->stream));
   GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
   time_collect(h, true, h->P.screening != 0, true);
@@ -817,102 +949,31 @@ extern "C" int gra_hip_vegas_iterate(gra_handle* h, const gra_grid* grid, uint64
   out->n_fid_ok = (uint64_t)s[7];
   out->n_valid = (uint64_t)s[8];
   out->n_nan = (uint64_t)s[9];
+  out->n_inf = (uint64_t)s[10];
   for (int j = 0; j < D; ++j)
     for (int b = 0; b < GRA_VEGAS_BINS; ++b) {
-      out->fmat[j * GRA_VEGAS_BINS + b] = s[10 + j * GRA_VEGAS_BINS + b];
-      out->f2mat[j * GRA_VEGAS_BINS + b] = s[10 + (D + j) * GRA_VEGAS_BINS + b];
+      out->fmat[j * GRA_VEGAS_BINS + b] = s[kIterScalars + j * GRA_VEGAS_BINS + b];
+      out->f2mat[j * GRA_VEGAS_BINS + b] = s[kIterScalars + (D + j) * GRA_VEGAS_BINS + b];
     }
-  return GRA_OK;
+  return count_nonfinite(h, out->n_nan, out->n_inf);
 }
 
-// Iteration combination of MGraniitti::VEGAS, src/MGraniitti.cc:1222-1251
-static void vegas_combine(double fsum, double f2sum, double calls, int iter, double* state,
-                          double* sigma, double* sigma_err, double* chi2) {
-  const double EPS = 1.0e-30;
-  fsum *= 1.0 / calls;
-  f2sum *= (1.0 / calls) * (1.0 / calls);
-  const double I_this = fsum;
-  f2sum = msqrt(f2sum * calls);
-  f2sum = (f2sum - fsum) * (f2sum + fsum);
-  if (f2sum <= 0.0) f2sum = EPS;
-  const double I2_this = f2sum * 1.0 / calls;
-  const double alpha = 1.0 / I2_this;
-  state[0] += alpha * I_this;
-  state[1] += alpha * (I_this * I_this);
-  state[2] += alpha;
-  *sigma = state[0] / state[2];
-  *sigma_err = msqrt(1.0 / state[2]);
-  double c2 = (state[1] - (state[0] * state[0]) / state[2]) / (iter + 1E-5);
-  *chi2 = (c2 < 0) ? 0.0 : c2;
+// One integration pass on the GPU for the host schedule
+static int gpu_pass(void* ctx, const gra_grid* grid, uint64_t n_calls, uint64_t seed, uint32_t iter,
+                    gra_iter_stats* out) {
+  gra_handle* h = (gra_handle*)ctx;
+  const int r = gra_hip_vegas_iterate(h, grid, n_calls, seed, iter, out);
+  if (r) set_global_error(r, "%s", h->err.c_str());
+  return r;
 }
 
 extern "C" int gra_hip_vegas_run(gra_handle* h, const gra_vegas_cfg* vc, gra_grid* grid,
                                  gra_vegas_result* res) {
   if (!h) return set_global_error(GRA_E_ARG, "handle is NULL");
-  if (!vc || !grid || !res) return fail(h, GRA_E_ARG, "vegas_run: NULL argument");
-  if (vc->ncall == 0 || vc->iter < 1 || vc->burnin_iter < 0)
-    return fail(h, GRA_E_ARG, "vegas_run: ncall >= 1, iter >= 1, burnin_iter >= 0 required");
-  std::memset(res, 0, sizeof(*res));
-  res->seconds_to_precision = -1.0;
-  gra_iter_stats* st = new (std::nothrow) gra_iter_stats;
-  if (!st) return fail(h, GRA_E_NOMEM, "vegas_run: host stats");
   const double kt0 = h->time_ms[TS_TREE] + h->time_ms[TS_SCREEN] + h->time_ms[TS_ACCUM];
-  const auto t0 = std::chrono::steady_clock::now();
-  auto since = [&t0]() {
-    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  };
-  int r = GRA_OK;
-  uint32_t counter = 0;
-  double state[3] = {0, 0, 0};
-  double sigma = 0, err = 0, chi2 = 0;
-  const int bins = vc->bins > 0 ? vc->bins : GRA_VEGAS_BINS;
-  // init = 0: fresh uniform grid (VEGASData::Init(0) -> ClearAll)
-  r = grid_init(grid, h->P.dim, bins);
-  if (r) {
-    delete st;
-    return fail(h, r, "%s", global_error());
-  }
-  for (int it = 0; it < vc->burnin_iter && r == GRA_OK; ++it) {
-    r = gra_hip_vegas_iterate(h, grid, vc->ncall, vc->seed, counter++, st);
-    if (r) break;
-    vegas_combine(st->fsum, st->f2sum, (double)vc->ncall, it, state, &sigma, &err, &chi2);
-    res->calls_total += vc->ncall;
-    grid_optimize(grid, st->f2mat, vc->lambda);
-  }
-  const double t_main = since();
-  // init = 1: keep the grid, clear the integral (VEGASData::ClearIntegral)
-  state[0] = state[1] = state[2] = 0;
-  int itermin = vc->iter;
-  const int itermax = std::max(vc->max_iter, vc->iter);
-  int done = 0;
-  for (int it = 0; it < itermin && r == GRA_OK; ++it) {
-    r = gra_hip_vegas_iterate(h, grid, vc->ncall, vc->seed, counter++, st);
-    if (r) break;
-    vegas_combine(st->fsum, st->f2sum, (double)vc->ncall, it, state, &sigma, &err, &chi2);
-    res->calls_total += vc->ncall;
-    res->maxW = std::max(res->maxW, st->maxW);
-    res->maxf = std::max(res->maxf, st->maxf);
-    ++done;
-    if (vc->precision > 0 && res->seconds_to_precision < 0 && sigma > 0 &&
-        err / sigma <= vc->precision)
-      res->seconds_to_precision = since();
-    if (it == itermin - 1 && itermin < itermax &&
-        ((vc->chi2max > 0 && chi2 > vc->chi2max) ||
-         (vc->precision > 0 && err / sigma > vc->precision)))
-      ++itermin;
-    grid_optimize(grid, st->f2mat, vc->lambda);
-  }
-  delete st;
-  if (r) return r;
-  res->sigma = sigma;
-  res->sigma_err = err;
-  res->chi2 = chi2;
-  res->iterations = done;
-  res->seconds_total = since();
-  res->seconds = res->seconds_total - t_main;
-  res->kernel_ms = h->time_ms[TS_TREE] + h->time_ms[TS_SCREEN] + h->time_ms[TS_ACCUM] - kt0;
-  if (!(std::isfinite(sigma)) || sigma < 1e-60)
-    return fail(h, GRA_E_NUMERIC, "VEGAS: integral %g is not finite or < 1e-60", sigma);
+  const int r = vegas_schedule(vc, grid, h->P.dim, gpu_pass, h, res);
+  if (res) res->kernel_ms = h->time_ms[TS_TREE] + h->time_ms[TS_SCREEN] + h->time_ms[TS_ACCUM] - kt0;
+  if (r) return fail(h, r, "%s", global_error());
   return GRA_OK;
 }
 
@@ -966,7 +1027,7 @@ extern "C" int gra_hip_generate(gra_handle* h, const gra_grid* grid, uint64_t n_
   const uint64_t cap = std::max<uint64_t>(1, std::min<uint64_t>(max_events, n_calls));
   DevBuf d_idx, d_cnt, d_ev;
   if (h->d_grid.ensure(sizeof(double) * GRA_MAX_DIM * GRA_VEGAS_BINS) ||
-      d_idx.ensure(sizeof(uint64_t) * n_calls) || d_cnt.ensure(4 * sizeof(unsigned long long)) ||
+      d_idx.ensure(sizeof(uint64_t) * n_calls) || d_cnt.ensure(6 * sizeof(unsigned long long)) ||
       d_ev.ensure(sizeof(gra_event) * cap))
     return fail(h, GRA_E_NOMEM, "generation buffers");
   GRA_HIP_CHECK(h, hipMemcpyAsync(h->d_grid.p, grid->xmat, sizeof(double) * D * GRA_VEGAS_BINS,
@@ -975,14 +1036,16 @@ extern "C" int gra_hip_generate(gra_handle* h, const gra_grid* grid, uint64_t n_
                     n_calls, h->d_W.as<double>(), nullptr, h->d_flags.as<uint8_t>(),
                     h->d_vw.as<double>(), h->d_bins.as<uint8_t>());
   if (r) return r;
-  GRA_HIP_CHECK(h, hipMemsetAsync(d_cnt.p, 0, 4 * sizeof(unsigned long long), h->stream));
+  GRA_HIP_CHECK(h, hipMemsetAsync(d_cnt.p, 0, 6 * sizeof(unsigned long long), h->stream));
   GRA_HIP_CHECK(h, launch_accept(h->d_W.as<double>(), h->d_vw.as<double>(),
                                  h->d_flags.as<uint8_t>(), n_calls, maxf, seed, iter, 0,
                                  d_idx.as<uint64_t>(), n_calls, d_cnt.as<unsigned long long>(),
                                  h->stream));
-  unsigned long long c[4] = {0, 0, 0, 0};
+  unsigned long long c[6] = {0, 0, 0, 0, 0, 0};
   GRA_HIP_CHECK(h, hipMemcpyAsync(c, d_cnt.p, sizeof(c), hipMemcpyDeviceToHost, h->stream));
   GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  r = count_nonfinite(h, c[4], c[5]);
+  if (r) return r;
   const uint64_t stored = std::min<uint64_t>(c[2], max_events);
   std::memset(st, 0, sizeof(*st));
   st->trials = c[0];

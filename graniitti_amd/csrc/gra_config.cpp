@@ -5,8 +5,12 @@
 // and the eikonal numerics (modeldata/NUMERICS.json, include/Graniitti/MEikonal.h:26-57).
 // The grid routines restate VEGASData (include/Graniitti/MVEGAS.h); they run on
 // the host once per iteration over BINS x D = 128 x 8 numbers.
+#include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
+#include <memory>
+#include <new>
 
 #include "gra_internal.h"
 
@@ -83,6 +87,8 @@ void set_common(gra_cfg* c) {
   c->tp_gPvv[0] = 0.70;
   c->tp_gPvv[1] = 6.20;
   c->tp_mp_ff = 0.938272081;
+  c->tp_gPpipi = 1.76;
+  c->tp_gPKK = 1.54;
 
   // Eikonal numerics: NumberBT = NumberKT2 = 10000 (NUMERICS.json),
   // FBIntegralN = 10000, MinLoopKT = 1e-4 (MEikonal.h:37-40)
@@ -103,6 +109,7 @@ void set_pipi(gra_cfg* c) {
   c->m_daughter = 0.0;
   c->S_factor = 1.0;  // pi+ pi-
   c->m_prop = 0.13957061;
+  c->central_spin2 = 0;
   // PARAM_PI couplings and active trajectories
   c->regge_c[0] = 35.00184;
   c->regge_c[1] = 81.63672;
@@ -138,6 +145,7 @@ void set_rhorho(gra_cfg* c) {
   c->m_daughter = 0.13957061;
   c->S_factor = 2.0;  // rho0 rho0
   c->m_prop = 0.775;
+  c->central_spin2 = 2;
   // PARAM_RHO couplings (ansatz), used only if the Regge amplitude is selected
   c->regge_c[0] = 25.0;
   c->regge_on[0] = 1;
@@ -246,6 +254,15 @@ extern "C" int gra_hip_cfg_default(gra_cfg* cfg, int32_t preset) {
       cfg->screening = 0;
       cfg->amp_precision = GRA_AMP_FP64;
       break;
+    case 9:
+    case 10:
+      // "PP[CONTENSOR]<C> -> pi+ pi-": the tensor-pomeron continuum of a pion pair
+      // (2xS, src/MTensorPomeron.cc:726-763) with preset 1's beams and cuts
+      set_pipi(cfg);
+      cfg->process = GRA_PROC_TENSOR_CON;
+      cfg->screening = preset == 10;
+      cfg->amp_precision = GRA_AMP_FP64;
+      break;
     default:
       return set_global_error(GRA_E_ARG, "gra_hip_cfg_default: unknown preset %d", preset);
   }
@@ -336,4 +353,173 @@ extern "C" int gra_hip_grid_init(gra_grid* grid, int32_t dim, int32_t bins) {
 extern "C" int gra_hip_grid_optimize(gra_grid* grid, gra_iter_stats* st, double lambda) {
   if (!grid || !st) return set_global_error(GRA_E_ARG, "gra_hip_grid_optimize: NULL argument");
   return grid_optimize(grid, st->f2mat, lambda);
+}
+
+// ---------------------------------------------------------------------------
+// VEGAS schedule (MGraniitti::SampleVegas + MGraniitti::VEGAS in integration
+// mode, src/MGraniitti.cc:1074-1135, 1146-1338) over a caller-supplied pass.
+// ---------------------------------------------------------------------------
+namespace gra_hip {
+
+// Iteration combination, src/MGraniitti.cc:1222-1251
+void vegas_combine(double fsum, double f2sum, double calls, int iter, double* state,
+                   double* sigma, double* sigma_err, double* chi2) {
+  const double EPS = 1.0e-30;
+  fsum *= 1.0 / calls;
+  f2sum *= (1.0 / calls) * (1.0 / calls);
+  const double I_this = fsum;
+  f2sum = std::sqrt(std::max(0.0, f2sum * calls));
+  f2sum = (f2sum - fsum) * (f2sum + fsum);
+  if (f2sum <= 0.0) f2sum = EPS;
+  const double I2_this = f2sum * 1.0 / calls;
+  const double alpha = 1.0 / I2_this;
+  state[0] += alpha * I_this;
+  state[1] += alpha * (I_this * I_this);
+  state[2] += alpha;
+  *sigma = state[0] / state[2];
+  *sigma_err = std::sqrt(std::max(0.0, 1.0 / state[2]));
+  double c2 = (state[1] - (state[0] * state[0]) / state[2]) / (iter + 1E-5);
+  *chi2 = (c2 < 0) ? 0.0 : c2;
+}
+
+// The run's guards after every iteration but the first of a VEGAS call
+// (src/MGraniitti.cc:1258-1283): 0 = go on, 1 = restart (chi2 > 50),
+// GRA_E_NUMERIC with a message for an integral that is NaN/Inf or < 1e-60.
+static int vegas_guard(int iter, double sigma, double chi2) {
+  if (iter == 0) return 0;
+  if (std::isnan(sigma) || std::isinf(sigma))
+    return set_global_error(GRA_E_NUMERIC,
+                            "VEGAS: integral inf/nan - cuts or parameters need fixing");
+  if (sigma < 1e-60)
+    return set_global_error(GRA_E_NUMERIC,
+                            "VEGAS: integral %g < 1e-60 - check generation and fiducial cuts",
+                            sigma);
+  return chi2 > kVegasChi2Restart ? 1 : 0;
+}
+
+int vegas_schedule(const gra_vegas_cfg* vc, gra_grid* grid, int dim, VegasPass pass, void* ctx,
+                   gra_vegas_result* res) {
+  if (!vc || !grid || !res) return set_global_error(GRA_E_ARG, "vegas_run: NULL argument");
+  if (vc->ncall == 0 || vc->ncall > 0xFFFFFFFFull || vc->iter < 1 || vc->burnin_iter < 0)
+    return set_global_error(GRA_E_ARG,
+                            "vegas_run: 1 <= ncall < 2^32, iter >= 1, burnin_iter >= 0 required");
+  std::memset(res, 0, sizeof(*res));
+  res->seconds_to_precision = -1.0;
+  std::unique_ptr<gra_iter_stats> st(new (std::nothrow) gra_iter_stats);
+  if (!st) return set_global_error(GRA_E_NOMEM, "vegas_run: host stats");
+  const auto t0 = std::chrono::steady_clock::now();
+  auto since = [&t0]() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  };
+  const int bins = vc->bins > 0 ? vc->bins : GRA_VEGAS_BINS;
+  uint64_t ncall = vc->ncall;
+  int burnin = vc->burnin_iter;
+  uint32_t counter = 0;
+  double state[3];
+  double sigma = 0, err = 0, chi2 = 0;
+  double t_main = 0.0;
+  int done = 0;
+  // CheckInfNan over the run (include/Graniitti/MProcess.h:317-340): the passes
+  // report the NaN and Inf weights they set to zero
+  uint64_t n_nan = 0, n_inf = 0;
+  auto run_pass = [&](uint64_t calls) -> int {
+    const int r = pass(ctx, grid, calls, vc->seed, counter++, st.get());
+    if (r) return r;
+    n_nan += st->n_nan;
+    n_inf += st->n_inf;
+    if (n_nan > GRA_MAX_NONFINITE)
+      return set_global_error(GRA_E_NUMERIC,
+                              "CheckInfNan: too many NaN weights (%llu) - check model parameters "
+                              "and cuts", (unsigned long long)n_nan);
+    if (n_inf > GRA_MAX_NONFINITE)
+      return set_global_error(GRA_E_NUMERIC,
+                              "CheckInfNan: too many Inf weights (%llu) - check model parameters "
+                              "and cuts", (unsigned long long)n_inf);
+    return GRA_OK;
+  };
+  for (;;) {
+    // init = 0: burn-in from a fresh uniform grid (VEGASData::Init(0)); repeated
+    // with 10 x the calls while an iteration's chi2 exceeds 50
+    for (bool again = true; again;) {
+      again = false;
+      int r = grid_init(grid, dim, bins);
+      if (r) return r;
+      state[0] = state[1] = state[2] = 0;
+      for (int it = 0; it < burnin; ++it) {
+        r = run_pass(ncall);
+        if (r) return r;
+        vegas_combine(st->fsum, st->f2sum, (double)ncall, it, state, &sigma, &err, &chi2);
+        res->calls_total += ncall;
+        r = vegas_guard(it, sigma, chi2);
+        if (r < 0) return r;
+        if (r == 1) {
+          if (res->restarts >= kVegasMaxRestarts || ncall * 10 > 0xFFFFFFFFull)
+            return set_global_error(GRA_E_NUMERIC,
+                                    "VEGAS: chi2 = %.1f > 50 after %d restarts (%llu calls)", chi2,
+                                    res->restarts, (unsigned long long)ncall);
+          ncall *= 10;
+          ++res->restarts;
+          again = true;
+          break;
+        }
+        grid_optimize(grid, st->f2mat, vc->lambda);
+      }
+    }
+    // init = 1: keep the grid, clear the integral (VEGASData::Init(1))
+    t_main = since();
+    state[0] = state[1] = state[2] = 0;
+    res->maxW = res->maxf = 0.0;
+    res->seconds_to_precision = -1.0;
+    int itermin = vc->iter;
+    const int itermax = std::max(vc->max_iter, vc->iter);
+    done = 0;
+    bool restart = false;
+    for (int it = 0; it < itermin; ++it) {
+      int r = run_pass(ncall);
+      if (r) return r;
+      vegas_combine(st->fsum, st->f2sum, (double)ncall, it, state, &sigma, &err, &chi2);
+      res->calls_total += ncall;
+      res->maxW = std::max(res->maxW, st->maxW);
+      res->maxf = std::max(res->maxf, st->maxf);
+      ++done;
+      r = vegas_guard(it, sigma, chi2);
+      if (r < 0) return r;
+      if (r == 1) {
+        // SampleVegas: twice the burn-in iterations, then integrate again
+        if (res->restarts >= kVegasMaxRestarts)
+          return set_global_error(GRA_E_NUMERIC, "VEGAS: chi2 = %.1f > 50 after %d restarts",
+                                  chi2, res->restarts);
+        burnin = std::max(1, 2 * burnin);
+        ++res->restarts;
+        restart = true;
+        break;
+      }
+      if (vc->precision > 0 && res->seconds_to_precision < 0 && sigma > 0 &&
+          err / sigma <= vc->precision)
+        res->seconds_to_precision = since();
+      if (it == itermin - 1 && itermin < itermax &&
+          ((vc->chi2max > 0 && chi2 > vc->chi2max) ||
+           (vc->precision > 0 && err / sigma > vc->precision)))
+        ++itermin;
+      grid_optimize(grid, st->f2mat, vc->lambda);
+    }
+    if (!restart) break;
+  }
+  res->sigma = sigma;
+  res->sigma_err = err;
+  res->chi2 = chi2;
+  res->iterations = done;
+  res->seconds_total = since();
+  res->seconds = res->seconds_total - t_main;
+  if (!std::isfinite(sigma) || sigma < 1e-60)
+    return set_global_error(GRA_E_NUMERIC, "VEGAS: integral %g is not finite or < 1e-60", sigma);
+  return GRA_OK;
+}
+
+}  // namespace gra_hip
+
+extern "C" int gra_vegas_run_host(const gra_vegas_cfg* vc, gra_grid* grid, int32_t dim,
+                                  gra_vegas_pass_fn pass, void* ctx, gra_vegas_result* res) {
+  if (!pass) return set_global_error(GRA_E_ARG, "gra_vegas_run_host: pass is NULL");
+  return vegas_schedule(vc, grid, dim, pass, ctx, res);
 }

@@ -813,6 +813,60 @@ __device__ __forceinline__ void tensor_me4_fp64(const RunParams& P, const Kin& K
     }
 }
 
+// ---------------------------------------------------------------------------
+// Tensor-pomeron continuum for a pseudoscalar pair (2xS), src/MTensorPomeron.cc:726-763.
+//
+// Each (proton vertex) x (Pomeron propagator) x (meson vertex) chain of the
+// reference's tensor network is a scalar: with the proton vertex
+// -3 i gPNN F1(t) P_mu P_nu, the propagator F (g g + g g - g g / 2) and the meson
+// vertex iG_Ppsps = -2 i gP FM (K_a K_b - g_ab K^2 / 4) (K = k' + k), the index
+// sums give
+//   A = -6 gPNN gP F1 FM F [2 (P.K)^2 - P^2 K^2 / 2],
+// since the propagator's trace part meets the traceless meson vertex. The t- and
+// u-channel amplitudes are M_t = A_t B_t i / (t^ - M^2) ff(t^)^2 and likewise
+// with u^; the four proton-helicity replicas are identical (high-energy vertex),
+// so one amplitude -i (M_t + M_u) carries |M|^2.
+// ---------------------------------------------------------------------------
+// 2 c F d [2 (P.K)^2 - P^2 K^2 / 2] without the couplings: F1 FM F times the
+// bracket, with F = (-i s a')^{alpha(t) - 1} / (4 s) and FM = 1 / (1 - q^2 / 0.5)
+__device__ __forceinline__ cxd tensor_chain_2xs(const RunParams& P, dv4 Pv, dv4 K, double q2,
+                                                double F1, double s, double t) {
+  const double PK = mdot(Pv, K);
+  const double br = 2.0 * (PK * PK) - 0.5 * mdot(Pv, Pv) * mdot(K, K);
+  const double FM = 1.0 / (1.0 - q2 / 0.5);
+  const double am1 = P.tp_delta_P + P.tp_ap_P * t;
+  const double mag = exp(am1 * log(s * P.tp_ap_P)) / (4.0 * s);
+  double sn, cs;
+  sincos(-0.5 * GRA_PI * am1, &sn, &cs);
+  const double r = F1 * FM * mag * br;
+  return cd(r * cs, r * sn);
+}
+__device__ __forceinline__ cxd tensor_me4_2xs_fp64(const RunParams& P, const Kin& K) {
+  const dv4 pa = P.pbeam1, pb = P.pbeam2;
+  const dv4 p1 = K.pf[1], p2 = K.pf[2], p3 = K.dt[0], p4 = K.dt[1];
+  const dv4 P1 = v4add(p1, pa), P2 = v4add(p2, pb);
+  const dv4 pt = v4sub(v4sub(pa, p1), p3);
+  const dv4 pu = v4add(v4sub(p4, pa), p1);
+  const dv4 mp3 = v4scale(p3, -1.0);
+  const double F1a = kelly_f1(K.t1, P.tp_mp_ff), F1b = kelly_f1(K.t2, P.tp_mp_ff);
+  // meson vertices iG_Ppsps(k', k): K = k' + k, FM of (k' - k)^2
+  const cxd At = tensor_chain_2xs(P, P1, v4add(pt, mp3), mdot(v4sub(pt, mp3), v4sub(pt, mp3)),
+                                  F1a, K.ss13, K.t1);
+  const cxd Bt = tensor_chain_2xs(P, P2, v4add(p4, pt), mdot(v4sub(p4, pt), v4sub(p4, pt)), F1b,
+                                  K.ss24, K.t2);
+  const cxd Au = tensor_chain_2xs(P, P1, v4add(p4, pu), mdot(v4sub(p4, pu), v4sub(p4, pu)), F1a,
+                                  K.ss14, K.t1);
+  const cxd Bu = tensor_chain_2xs(P, P2, v4add(pu, mp3), mdot(v4sub(pu, mp3), v4sub(pu, mp3)),
+                                  F1b, K.ss23, K.t2);
+  const double pt2 = mdot(pt, pt), pu2 = mdot(pu, pu);
+  const double fft = meson_ff(P, pt2), ffu = meson_ff(P, pu2);
+  // couplings (-6 gPNN gP)^2, the propagators i / (m^2 - M^2) and the overall -i
+  const double g = 6.0 * P.tp_gPNN * P.tp_gP;
+  const double ct = (g * g) * (fft * fft) / (pt2 - P.m_prop2);
+  const double cu = (g * g) * (ffu * ffu) / (pu2 - P.m_prop2);
+  return cadd(cscale(cmul(At, Bt), ct), cscale(cmul(Au, Bu), cu));
+}
+
 // |z|^2 as gra::math::abs2 (hypot then square)
 __device__ __forceinline__ double abs2d(cxd z) {
   const double h = hypot(z.re, z.im);

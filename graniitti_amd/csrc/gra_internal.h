@@ -20,6 +20,14 @@ const char* global_error();
 int grid_init(gra_grid* g, int dim, int bins);
 int grid_optimize(gra_grid* g, double* f2mat, double lambda);
 
+// VEGAS schedule of MGraniitti::SampleVegas + VEGAS (integration mode) over a
+// pass that fills one iteration's accumulators; errors go to the global message
+typedef gra_vegas_pass_fn VegasPass;
+constexpr double kVegasChi2Restart = 50.0;  // src/MGraniitti.cc:1276
+constexpr int kVegasMaxRestarts = 6;        // the reference retries without limit
+int vegas_schedule(const gra_vegas_cfg* vc, gra_grid* grid, int dim, VegasPass pass, void* ctx,
+                   gra_vegas_result* res);
+
 // Device buffer that grows on demand
 struct DevBuf {
   void* p = nullptr;
@@ -78,6 +86,9 @@ struct gra_handle {
   double time_ms[gra_hip::TS_N] = {};
   uint64_t launches[gra_hip::TS_N] = {};
 
+  // CheckInfNan counters over the handle's life (include/Graniitti/MProcess.h:317-340)
+  uint64_t n_nan_seen = 0, n_inf_seen = 0;
+
   gra_hip::FlowState* flow = nullptr;
 };
 
@@ -88,6 +99,9 @@ int run_integrand(gra_handle* h, const double* d_grid, unsigned int bins, uint64
                   uint32_t iter, uint64_t first, const double* d_x, uint64_t n, double* d_W,
                   double* d_amp2, uint8_t* d_flags, double* d_vw, uint8_t* d_bins);
 int ensure_point_buffers(gra_handle* h, uint64_t n);
+// Adds a pass's NaN and Inf weight counts to the handle's; GRA_E_NUMERIC once
+// either total exceeds GRA_MAX_NONFINITE
+int count_nonfinite(gra_handle* h, uint64_t n_nan, uint64_t n_inf);
 }  // namespace gra_hip
 
 #define GRA_HIP_CHECK(h, call)                                                        \

@@ -11,7 +11,7 @@ namespace gra_hip {
 
 #define GRA_DECLARE_INTEGRAND_LAUNCHERS(NS)                                                        \
   namespace NS {                                                                                   \
-  size_t loop_record_bytes(int process);                                                           \
+  size_t loop_record_bytes(const RunParams& P);                                                    \
   hipError_t launch_tree(const RunParams& P, const double* grid, unsigned int bins, uint64_t seed, \
                          uint32_t iter, uint64_t first, const double* xin, uint64_t n, double* W,  \
                          double* amp2, uint8_t* flags, double* vw, uint8_t* binsout, void* rec,    \
@@ -27,6 +27,11 @@ namespace gra_hip {
 
 GRA_DECLARE_INTEGRAND_LAUNCHERS(k64)
 GRA_DECLARE_INTEGRAND_LAUNCHERS(k32)
+
+// Scalars of one accumulated iteration, ahead of the [2][D][128] histograms:
+// fsum, its compensation, f2sum, its compensation, max W, max f, and the counts of
+// kinematics-valid, fiducial-valid, valid, NaN-weight and Inf-weight points
+constexpr int kIterScalars = 11;
 
 hipError_t launch_accumulate(const double* W, const double* vw, const uint8_t* bins,
                              const uint8_t* flags, uint64_t n, int D, int nblk, double* part_hist,

@@ -25,6 +25,10 @@ struct ResDev {
   int32_t J, P, BW, photo;  // photo: J^P = 1^- through MRegge::PhotoME3
 };
 
+// Tensor-pomeron spin modes of MTensorPomeron::ME4 (src/MTensorPomeron.cc:643-654)
+constexpr int32_t kTp2xV = 0;  // vector-meson pair
+constexpr int32_t kTp2xS = 1;  // pseudoscalar pair
+
 struct RunParams {
   int32_t process, amp_mode, dim, screening;
   // beams and invariants
@@ -62,8 +66,10 @@ struct RunParams {
   int32_t nphi, nkt;  // intervals
   double step_kt, step_phi, min_loop_kt, hfac, nrm;
   double kappa;  // nrm * hfac (FP32 loop sum normalisation)
-  // tensor pomeron
+  // tensor pomeron: spin mode (kTp2xV, kTp2xS) and couplings
+  int32_t tp_mode, pad_tp;
   double tp_delta_P, tp_ap_P, tp_gPNN, tp_gPvv0, tp_gPvv1, tp_mp_ff, m_rho;
+  double tp_gP;  // Pomeron-pseudoscalar-pseudoscalar coupling of the 2xS mode
   // resonances (PP[RES+CON]): device table of nres entries
   const ResDev* res;
   int32_t nres, spin_gen, spin_dec, spin_frame;

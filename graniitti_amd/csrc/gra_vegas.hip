@@ -6,6 +6,8 @@
 
 namespace {
 
+using gra_hip::kIterScalars;
+
 // Deterministic accumulation of one iteration. Each block owns a contiguous
 // range of points and has kAccWaves waves that walk the same points; wave w
 // owns the dimensions w, w + kAccWaves, ... and keeps their histograms of f and
@@ -50,7 +52,7 @@ __global__ __launch_bounds__(kAccWave* kAccWaves) void k_accumulate(
   uint64_t p1 = p0 + chunk;
   if (p1 > n) p1 = n;
   double fs = 0, fc = 0, f2s = 0, f2c = 0, mW = 0, mf = 0;
-  uint64_t c_kin = 0, c_fid = 0, c_val = 0, c_nan = 0;
+  uint64_t c_kin = 0, c_fid = 0, c_val = 0, c_nan = 0, c_inf = 0;
   for (uint64_t base = p0; base < p1; base += kAccWave) {
     const uint64_t i = base + lane;
     double f = 0.0;
@@ -75,7 +77,9 @@ __global__ __launch_bounds__(kAccWave* kAccWaves) void k_accumulate(
         c_kin += (fl & GRA_FLAG_KIN_OK) ? 1 : 0;
         c_fid += (fl & GRA_FLAG_FID_OK) ? 1 : 0;
         c_val += ((fl & GRA_FLAG_KIN_OK) && (fl & GRA_FLAG_FID_OK)) ? 1 : 0;
-        c_nan += (fl & GRA_FLAG_AMP_OK) ? 0 : 1;
+        const bool bad = !(fl & GRA_FLAG_AMP_OK);
+        c_nan += (bad && !(fl & GRA_FLAG_INF)) ? 1 : 0;
+        c_inf += (bad && (fl & GRA_FLAG_INF)) ? 1 : 0;
       }
 #pragma unroll
       for (int q = 0; q < DPW; ++q) {
@@ -141,9 +145,10 @@ __global__ __launch_bounds__(kAccWave* kAccWaves) void k_accumulate(
     c_fid += __shfl_down(c_fid, off);
     c_val += __shfl_down(c_val, off);
     c_nan += __shfl_down(c_nan, off);
+    c_inf += __shfl_down(c_inf, off);
   }
   if (lane == 0) {
-    double* o = part_scal + (uint64_t)blockIdx.x * 10;
+    double* o = part_scal + (uint64_t)blockIdx.x * kIterScalars;
     o[0] = fs;
     o[1] = fc;
     o[2] = f2s;
@@ -154,12 +159,14 @@ __global__ __launch_bounds__(kAccWave* kAccWaves) void k_accumulate(
     o[7] = (double)c_fid;
     o[8] = (double)c_val;
     o[9] = (double)c_nan;
+    o[10] = (double)c_inf;
   }
 }
 
 // Fixed-order two-level reduction of the block partials. Stage 1: group g sums
 // blocks [g nblk / G, (g + 1) nblk / G) in order; stage 2 sums the groups in
-// order into gra_iter_stats layout: out[0..9] scalars, out[10 ..] hist [2][D][128].
+// order into gra_iter_stats layout: out[0 .. kIterScalars - 1] scalars, then the
+// histograms [2][D][128].
 constexpr int kRedGroups = 32;
 
 // Kahan-add of one block's (sum, compensation) pair and the other scalars
@@ -173,7 +180,7 @@ __device__ __forceinline__ void scal_add(double* a, const double* p) {
   a[2] = tt;
   a[4] = p[4] > a[4] ? p[4] : a[4];
   a[5] = p[5] > a[5] ? p[5] : a[5];
-  for (int k = 6; k < 10; ++k) a[k] += p[k];
+  for (int k = 6; k < kIterScalars; ++k) a[k] += p[k];
 }
 
 __global__ void k_reduce1(const double* __restrict__ part_hist, const double* __restrict__ part_scal,
@@ -187,9 +194,9 @@ __global__ void k_reduce1(const double* __restrict__ part_hist, const double* __
     for (int b = b0; b < b1; ++b) s += part_hist[(uint64_t)b * H + t];
     tmp[(uint64_t)g * H + t] = s;
   } else if (t == H) {
-    double a[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int b = b0; b < b1; ++b) scal_add(a, part_scal + (uint64_t)b * 10);
-    for (int k = 0; k < 10; ++k) tmp_scal[g * 10 + k] = a[k];
+    double a[kIterScalars] = {};
+    for (int b = b0; b < b1; ++b) scal_add(a, part_scal + (uint64_t)b * kIterScalars);
+    for (int k = 0; k < kIterScalars; ++k) tmp_scal[g * kIterScalars + k] = a[k];
   }
 }
 
@@ -199,11 +206,11 @@ __global__ void k_reduce2(const double* __restrict__ tmp, const double* __restri
   if (t < H) {
     double s = 0.0;
     for (int g = 0; g < kRedGroups; ++g) s += tmp[(uint64_t)g * H + t];
-    out[10 + t] = s;
+    out[kIterScalars + t] = s;
   } else if (t == H) {
-    double a[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int g = 0; g < kRedGroups; ++g) scal_add(a, tmp_scal + g * 10);
-    for (int k = 0; k < 10; ++k) out[k] = a[k];
+    double a[kIterScalars] = {};
+    for (int g = 0; g < kRedGroups; ++g) scal_add(a, tmp_scal + g * kIterScalars);
+    for (int k = 0; k < kIterScalars; ++k) out[k] = a[k];
   }
 }
 
@@ -251,15 +258,17 @@ __global__ void k_unweight(const double* __restrict__ w, uint64_t n, double wmax
 
 // Hit-or-miss of the generation pass: f = W vw of valid points against maxf;
 // accepted point indices are appended (wave-aggregated) up to max_out.
-// counts: trials, valid, accepted, overflow.
+// counts: trials, valid, accepted, overflow, NaN weights, Inf weights.
 __global__ void k_accept(const double* __restrict__ W, const double* __restrict__ vw,
                          const uint8_t* __restrict__ flags, uint64_t n, double maxf, uint64_t seed,
                          uint32_t iter, uint64_t first, uint64_t* __restrict__ idx, uint64_t max_out,
                          unsigned long long* __restrict__ counts) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  bool valid = false, hit = false, over = false;
+  bool valid = false, hit = false, over = false, is_nan = false, is_inf = false;
   if (i < n) {
     const uint8_t fl = flags[i];
+    is_nan = !(fl & GRA_FLAG_AMP_OK) && !(fl & GRA_FLAG_INF);
+    is_inf = !(fl & GRA_FLAG_AMP_OK) && (fl & GRA_FLAG_INF);
     valid = (fl & (GRA_FLAG_KIN_OK | GRA_FLAG_FID_OK | GRA_FLAG_VETO_OK)) ==
             (GRA_FLAG_KIN_OK | GRA_FLAG_FID_OK | GRA_FLAG_VETO_OK);
     const double f = W[i] * vw[i];
@@ -274,11 +283,14 @@ __global__ void k_accept(const double* __restrict__ W, const double* __restrict_
   }
   const uint64_t mt = __ballot(i < n);
   const uint64_t mv = __ballot(valid), mh = __ballot(hit), mo = __ballot(over);
+  const uint64_t mnan = __ballot(is_nan), minf = __ballot(is_inf);
   const int lane = threadIdx.x & 63;
   if (lane == 0) {
     if (mt) atomicAdd(&counts[0], (unsigned long long)__popcll(mt));
     if (mv) atomicAdd(&counts[1], (unsigned long long)__popcll(mv));
     if (mo) atomicAdd(&counts[3], (unsigned long long)__popcll(mo));
+    if (mnan) atomicAdd(&counts[4], (unsigned long long)__popcll(mnan));
+    if (minf) atomicAdd(&counts[5], (unsigned long long)__popcll(minf));
   }
   if (mh) {
     const int leader = __ffsll((unsigned long long)mh) - 1;
@@ -295,6 +307,7 @@ __global__ void k_accept(const double* __restrict__ W, const double* __restrict_
 }  // namespace
 
 namespace gra_hip {
+
 
 hipError_t launch_accept(const double* W, const double* vw, const uint8_t* flags, uint64_t n,
                          double maxf, uint64_t seed, uint32_t iter, uint64_t first, uint64_t* idx,
@@ -319,7 +332,7 @@ hipError_t launch_accumulate(const double* W, const double* vw, const uint8_t* b
     hipLaunchKernelGGL(k_accumulate<16>, dim3(nblk), dim3(kAccWave * kAccWaves), 0, st, W, vw,
                        bins, flags, n, chunk, D, part_hist, part_scal);
   const int H = 2 * D * GRA_VEGAS_BINS;
-  // stage-1 partial sums use the tail of part_hist's allocation: [G][H] then [G][10]
+  // stage-1 partial sums use the tail of part_hist's allocation: [G][H] then [G][kIterScalars]
   double* tmp = part_hist + (uint64_t)nblk * H;
   double* tmp_scal = tmp + (uint64_t)kRedGroups * H;
   hipLaunchKernelGGL(k_reduce1, dim3((H + 1 + 255) / 256, kRedGroups), dim3(256), 0, st,

@@ -39,7 +39,9 @@ typedef struct gra_handle gra_handle; /* opaque: device state of one process */
 #define GRA_E_HIP (-2)     /* HIP runtime failure */
 #define GRA_E_NOMEM (-3)   /* device allocation failed */
 #define GRA_E_STATE (-4)   /* call not valid in the handle's state */
-#define GRA_E_NUMERIC (-5) /* too many NaN/Inf weights (src/MProcess.h:317-340) */
+#define GRA_E_NUMERIC (-5) /* more than GRA_MAX_NONFINITE NaN (or Inf) weights
+                              (include/Graniitti/MProcess.h:317-340), or a VEGAS integral that is
+                              NaN/Inf or below 1e-60 (src/MGraniitti.cc:1258-1273) */
 #define GRA_E_IO (-6)      /* file missing, unreadable or malformed */
 
 /* Phase-space classes */
@@ -54,7 +56,8 @@ typedef struct gra_handle gra_handle; /* opaque: device state of one process */
 
 /* Processes (sub-process registry, include/Graniitti/MSubProc.h) */
 #define GRA_PROC_REGGE_CON 15  /* PROC_15 "PP[CON]" 2-body: MRegge::ME4, include/Graniitti/MSubProc.h:361-380 */
-#define GRA_PROC_TENSOR_CON 12 /* PROC_12 "PP[CONTENSOR]" 2xV: MTensorPomeron::ME4, MSubProc.h:288-300 */
+#define GRA_PROC_TENSOR_CON 12 /* PROC_12 "PP[CONTENSOR]": MTensorPomeron::ME4, MSubProc.h:288-300;
+                                  2xV (vector-meson pair) or 2xS (pseudoscalar pair) by central_spin2 */
 #define GRA_PROC_RES_CON 17    /* PROC_17 "PP[RES+CON]" 2-body: MRegge::ME4 + resonances, MSubProc.h:398-432 */
 #define GRA_PROC_RES 9         /* PROC_9 "PP[RES]": resonances only, MSubProc.h:233-262 */
 
@@ -80,6 +83,11 @@ typedef struct gra_handle gra_handle; /* opaque: device state of one process */
 #define GRA_FLAG_FID_OK 0x02u /* fidcuts_ok   */
 #define GRA_FLAG_VETO_OK 0x04u /* vetocuts_ok */
 #define GRA_FLAG_AMP_OK 0x08u /* amplitude_ok (finite weight, CheckInfNan) */
+#define GRA_FLAG_INF 0x10u    /* with GRA_FLAG_AMP_OK clear: the weight was +-Inf (else NaN) */
+
+/* CheckInfNan's limit: a run fails once more than this many NaN, or more than
+ * this many Inf, weights have been seen (include/Graniitti/MProcess.h:317-340) */
+#define GRA_MAX_NONFINITE 50
 
 /* ------------------------------------------------------------------------- */
 /* Configuration (plain old data, passed by value through the ABI)           */
@@ -143,7 +151,13 @@ typedef struct gra_cfg {
 
   /* Tensor pomeron (PARAM_TENSORPOM) */
   double tp_delta_P, tp_ap_P, tp_gPNN, tp_gPvv[2];
-  double tp_mp_ff; /* proton mass used by the Kelly/F1 form factor */
+  double tp_mp_ff;  /* proton mass used by the Kelly/F1 form factor */
+  double tp_gPpipi, tp_gPKK; /* Pomeron-pion-pion and Pomeron-kaon-kaon couplings (2xS) */
+  /* 2J of each central particle (lts.decaytree[i].p.spinX2): selects the tensor-pomeron
+   * spin mode, 0 -> "2xS" (pseudoscalar pair), 2 -> "2xV" (vector-meson pair)
+   * (MTensorPomeron::ME4, src/MTensorPomeron.cc:643-654) */
+  int32_t central_spin2;
+  int32_t pad_tp;
 
   /* Eikonal screening (MEikonalNumerics, include/Graniitti/MEikonal.h:26-57) */
   int32_t screening;
@@ -190,7 +204,8 @@ typedef struct gra_grid {
 typedef struct gra_iter_stats {
   double fsum, fsum_c, f2sum, f2sum_c;
   double maxW, maxf;
-  uint64_t n_calls, n_kin_ok, n_fid_ok, n_valid, n_nan;
+  uint64_t n_calls, n_kin_ok, n_fid_ok, n_valid;
+  uint64_t n_nan, n_inf; /* weights that were NaN / +-Inf (set to 0, CheckInfNan) */
   double fmat[GRA_MAX_DIM * GRA_VEGAS_BINS];
   double f2mat[GRA_MAX_DIM * GRA_VEGAS_BINS];
 } gra_iter_stats;
@@ -217,7 +232,8 @@ typedef struct gra_vegas_result {
   double seconds_to_precision;     /* wall time until sigma_err/sigma <= precision, or -1 */
   uint64_t calls_total;
   int32_t iterations;              /* init = 1 iterations done */
-  int32_t pad0;
+  int32_t restarts;                /* chi2 > 50 restarts (burn-in with 10 x calls, or with twice
+                                      the burn-in iterations), src/MGraniitti.cc:1093-1126, 1276-1282 */
   double kernel_ms;                /* sum of integrand-kernel durations (HIP events) */
 } gra_vegas_result;
 
@@ -296,6 +312,8 @@ typedef struct {
  *   8: PP[RES]<F> -> pi+ pi- with f2(1270) alone, |J_z| = 2 (rho = diag(1/2, 0, 0, 0, 1/2)),
  *      Collins-Soper frame, preset 4's generation cuts, no fiducial cuts
  *      (tests/run_JW_frames/run.sh), no screening, FP64
+ *   9: PP[CONTENSOR]<C> -> pi+ pi- (tensor pomeron, 2xS), preset 1's cuts, no screening, FP64
+ *  10: preset 9 + eikonal screening as preset 2, FP64 amplitude
  * Replaces the steering-card reader MGraniitti::ReadInput (src/MGraniitti.cc:318-329)
  * for the keys these workloads use, with TUNE0 values (modeldata/TUNE0/GENERAL.json). */
 int gra_hip_cfg_default(gra_cfg* cfg, int32_t preset);
@@ -363,10 +381,29 @@ int gra_hip_grid_optimize(gra_grid* grid, gra_iter_stats* st, double lambda);
 int gra_hip_vegas_iterate(gra_handle* h, const gra_grid* grid, uint64_t n_calls, uint64_t seed,
                           uint32_t iter, gra_iter_stats* out);
 
-/* Full VEGAS integration with a fixed schedule (MGraniitti::SampleVegas + VEGAS,
- * src/MGraniitti.cc:1074-1338). grid is in/out. */
+/* Full VEGAS integration (MGraniitti::SampleVegas + VEGAS, src/MGraniitti.cc:1074-1338):
+ * the schedule and guards of gra_vegas_run_host over gra_hip_vegas_iterate, with a
+ * fixed clock-independent call count. grid is in/out. */
 int gra_hip_vegas_run(gra_handle* h, const gra_vegas_cfg* vc, gra_grid* grid,
                       gra_vegas_result* res);
+
+/* One VEGAS pass over an integrand: n_calls points of iteration iter drawn through
+ * grid, with their accumulators written to out (the role of
+ * MGraniitti::VEGASMultiThread, src/MGraniitti.cc:1341-1443). Returns GRA_OK or a
+ * negative status, which ends the run with that status. */
+typedef int (*gra_vegas_pass_fn)(void* ctx, const gra_grid* grid, uint64_t n_calls,
+                                 uint64_t seed, uint32_t iter, gra_iter_stats* out);
+
+/* The schedule of gra_hip_vegas_run over a caller's pass, host only (no GPU):
+ * MGraniitti::SampleVegas + VEGAS in integration mode (src/MGraniitti.cc:1074-1338)
+ * with the reference's guards after every iteration but the first of a pass
+ * sequence: an integral that is NaN/Inf or below 1e-60 fails with GRA_E_NUMERIC;
+ * chi2 > 50 restarts the burn-in from a uniform grid, with 10 x the calls when it
+ * happens during burn-in, or with twice the burn-in iterations when it happens
+ * while integrating (at most 6 restarts; the reference retries without limit).
+ * grid is in/out; dim is the integrand's dimension. */
+int gra_vegas_run_host(const gra_vegas_cfg* vc, gra_grid* grid, int32_t dim,
+                       gra_vegas_pass_fn pass, void* ctx, gra_vegas_result* res);
 
 /* Points drawn from a grid (no weights): u -> (x, vegas weight, bin) for tests
  * of the map (src/MGraniitti.cc:1357-1375). Host pointers; x is n x dim, vw n,

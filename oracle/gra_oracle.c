@@ -1361,6 +1361,7 @@ static double event_weight(ora_run* R, const double* x, double* amp2_out, uint8_
     }
   }
   if (isnan(W) || isinf(W)) {
+    if (isinf(W)) fl |= GRA_FLAG_INF;
     W = 0;
   } else {
     fl |= GRA_FLAG_AMP_OK;
@@ -1544,7 +1545,12 @@ static void accumulate_point(gra_iter_stats* st, const gra_grid* g, double W, do
   if (fl & GRA_FLAG_KIN_OK) st->n_kin_ok += 1;
   if (fl & GRA_FLAG_FID_OK) st->n_fid_ok += 1;
   if ((fl & GRA_FLAG_KIN_OK) && (fl & GRA_FLAG_FID_OK)) st->n_valid += 1;
-  if (!(fl & GRA_FLAG_AMP_OK)) st->n_nan += 1;
+  if (!(fl & GRA_FLAG_AMP_OK)) {
+    if (fl & GRA_FLAG_INF)
+      st->n_inf += 1;
+    else
+      st->n_nan += 1;
+  }
 }
 
 static void* vg_worker(void* arg) {
@@ -1611,6 +1617,7 @@ ORA_EXPORT void ora_vegas_iterate(ora_run* R, const gra_grid* g, uint64_t first,
       out->n_fid_ok += L->n_fid_ok;
       out->n_valid += L->n_valid;
       out->n_nan += L->n_nan;
+      out->n_inf += L->n_inf;
     }
   }
   free(jobs);
@@ -1941,7 +1948,76 @@ static int tensor_me4_2xv(const ora_run* R, v4 pa, v4 pb, const kin_t* K, cx* ha
   return 36;
 }
 
+/* iG_Ppsps (src/MTensorPomeron.cc:2203-2215): Pomeron-pseudoscalar-pseudoscalar
+ * vertex -i 2 g FM((k' - k)^2) [(k' + k)_u (k' + k)_v - g_uv (k' + k)^2 / 4] */
+static void tp_iG_Ppsps(v4 prime, v4 p, long double g1, t2_t T) {
+  const v4 ps = v4_add(prime, p);
+  const long double M2 = ld_m2(ps);
+  const long double FM = 1.0L / (1.0L - ld_m2(v4_sub(prime, p)) / 0.5L);
+  for (int u = 0; u < 4; ++u)
+    for (int v = 0; v < 4; ++v)
+      T[u][v] = lc_make(0.0L, -2.0L * g1 * FM *
+                                  (v4_lo(ps, u) * v4_lo(ps, v) - 0.25L * GM[u][v] * M2));
+}
+/* sum_{a,b} X(a,b) Y(a,b) */
+static lcx tp_full(t2_t X, t2_t Y) {
+  lcx s = lc_make(0, 0);
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) s = lc_add(s, lc_mul(X[a][b], Y[a][b]));
+  return s;
+}
+
+/* MTensorPomeron::ME4 for a pseudoscalar pair ("2xS", src/MTensorPomeron.cc:726-763):
+ * the literal index sums of G1a D13 GtA . GtB D24 G2b (t-channel) and of the
+ * u-channel, with iD_MES0 = i / (p^2 - M^2) (2375-2377) and Meson_FF^2; four
+ * identical proton-helicity replicas (ha = h1, hb = h2). Returns 4. */
+static int tensor_me4_2xs(const ora_run* R, v4 pa, v4 pb, const kin_t* K, cx* hamp) {
+  const gra_cfg* c = &R->c;
+  const v4 p1 = K->pf[1], p2 = K->pf[2];
+  const v4 p3 = K->dt[0], p4 = K->dt[1];
+  const long double M_ = c->m_prop;
+  const long double gP = (c->m_central < 0.2) ? c->tp_gPpipi : c->tp_gPKK;
+  const v4 pt = v4_sub(v4_sub(pa, p1), p3);
+  const v4 pu = v4_add(v4_sub(p4, pa), p1);
+  const v4 mp3 = v4_scale(p3, -1.0);
+  t4_t D;
+  t2_t G1a, G2b, Ga, Gb, X;
+  tp_iG_PppHE(c, p1, pa, G1a);
+  tp_iG_PppHE(c, p2, pb, G2b);
+  const long double M2_ = M_ * M_;
+  long double ff[2];
+  const v4 ex[2] = {pt, pu};
+  for (int k = 0; k < 2; ++k)
+    ff[k] = (c->offshellFF == 2) ? 1.0L / (1.0L - (ld_m2(ex[k]) - M2_) / c->b_POW)
+                                 : meson_ff(c, v4_m2(ex[k]), (double)M2_);
+  lcx M[2];
+  for (int ch = 0; ch < 2; ++ch) {
+    /* t: iG_Ppsps(pt, -p3), iG_Ppsps(p4, pt); u: iG_Ppsps(p4, pu), iG_Ppsps(pu, -p3) */
+    if (ch == 0) {
+      tp_iG_Ppsps(pt, mp3, gP, Ga);
+      tp_iG_Ppsps(p4, pt, gP, Gb);
+    } else {
+      tp_iG_Ppsps(p4, pu, gP, Ga);
+      tp_iG_Ppsps(pu, mp3, gP, Gb);
+    }
+    tp_iD_P(c, ch == 0 ? K->ss[1][3] : K->ss[1][4], K->t1, D);
+    tp_vd_first(G1a, D, X);
+    const lcx A = tp_full(X, Ga);
+    tp_iD_P(c, ch == 0 ? K->ss[2][4] : K->ss[2][3], K->t2, D);
+    tp_vd_second(G2b, D, X);
+    const lcx B = tp_full(Gb, X);
+    const long double den = ld_m2(ex[ch]) - M2_;
+    const lcx iD = lc_make(0.0L, 1.0L / den); /* iD_MES0 */
+    M[ch] = lc_scale(lc_mul(lc_mul(A, B), iD), ff[ch] * ff[ch]);
+  }
+  const lcx m = lc_add(M[0], M[1]);
+  for (int rep = 0; rep < 4; ++rep) hamp[rep] = cx_make((double)m.im, (double)-m.re); /* -i m */
+  return 4;
+}
+
+/* Spin mode by the central particles' spins (src/MTensorPomeron.cc:643-654) */
 static int tensor_me4(const ora_run* R, const kin_t* K, cx* hamp) {
+  if (R->c.central_spin2 == 0) return tensor_me4_2xs(R, R->pbeam1, R->pbeam2, K, hamp);
   return tensor_me4_2xv(R, R->pbeam1, R->pbeam2, K, hamp);
 }
 
@@ -1989,6 +2065,18 @@ ORA_EXPORT int ora_tensor_amplitudes(ora_run* R, const double* p, double* hamp_r
     hamp_reim[2 * k + 1] = h[k].im;
   }
   return 9;
+}
+
+/* The 2xS amplitude -i (M_t + M_u) (one of the four identical replicas) at caller
+ * momenta pa, pb, p1, p2, p3, p4 (6 x (E, x, y, z)). Test entry point. */
+ORA_EXPORT void ora_tensor_2xs_amplitude(ora_run* R, const double* p, double* out) {
+  v4 q[6];
+  kin_t K;
+  tensor_kin(p, q, &K);
+  cx h[4];
+  tensor_me4_2xs(R, q[0], q[1], &K, h);
+  out[0] = h[0].re;
+  out[1] = h[0].im;
 }
 
 /* Phase-space part of the weight, BNPhaseSpaceWeight x BNIntegralVolume

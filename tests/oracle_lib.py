@@ -48,6 +48,7 @@ def _load_oracle():
         "ora_bare_amplitudes": (C.c_int, [VP, P(D), P(D)]),
         "ora_tensor_amplitudes": (C.c_int, [VP, P(D), P(D)]),
         "ora_tensor_mtensor": (None, [VP, P(D), P(D)]),
+        "ora_tensor_2xs_amplitude": (None, [VP, P(D), P(D)]),
         "ora_vegas_iterate": (None, [VP, P(abi.GraGrid), U64, U64, U64, C.c_uint32, C.c_int,
                                      C.c_int, P(abi.GraIterStats)]),
         "ora_vegas_combine": (None, [D, D, D, C.c_int, P(D), P(D)]),
@@ -170,6 +171,13 @@ class OracleRun:
         o = np.zeros(32)
         self.L.ora_tensor_mtensor(self.h, abi.dptr(p), abi.dptr(o))
         return (o[0::2] + 1j * o[1::2]).reshape(4, 4)
+
+    def tensor_2xs_amplitude(self, p):
+        """The tensor-pomeron 2xS amplitude at momenta p = (pa, pb, p1, p2, p3, p4)."""
+        p = np.ascontiguousarray(p, dtype=np.float64).reshape(6, 4)
+        o = np.zeros(2)
+        self.L.ora_tensor_2xs_amplitude(self.h, abi.dptr(p), abi.dptr(o))
+        return complex(o[0], o[1])
 
     def vegas_iterate(self, grid, n, seed, it, nthreads=8, first=0, use_mutex=0):
         st = abi.GraIterStats()

@@ -18,7 +18,7 @@ TUNE = json.load(open(os.path.join(ROOT, "tests", "golden", "tune0_params.json")
 
 def declared_symbols():
     txt = open(os.path.join(ROOT, "include", "gra_hip.h")).read()
-    return sorted(set(re.findall(r"\b(gra_(?:hip|hepmc3|eikonal)_\w+|gra_djb2)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(gra_(?:hip|hepmc3|eikonal|vegas)_\w+|gra_djb2)\s*\(", txt)))
 
 
 def test_library_exports_every_declared_symbol():

@@ -153,7 +153,7 @@ def test_vegas_iteration_matches_oracle():
     # deterministic on the GPU
     assert sg.fsum == sg2.fsum and sg.f2sum == sg2.f2sum
     assert np.array_equal(np.ctypeslib.as_array(sg.fmat), np.ctypeslib.as_array(sg2.fmat))
-    for k in ("n_calls", "n_kin_ok", "n_fid_ok", "n_valid", "n_nan"):
+    for k in ("n_calls", "n_kin_ok", "n_fid_ok", "n_valid", "n_nan", "n_inf"):
         assert abs(getattr(sg, k) - getattr(so, k)) <= 1, k
     assert abs(sg.fsum - so.fsum) <= 1e-10 * so.fsum
     assert abs(sg.f2sum - so.f2sum) <= 1e-10 * so.f2sum
