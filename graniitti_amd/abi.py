@@ -145,6 +145,7 @@ SIGNATURES = {
     "gra_hip_last_error": (C.c_char_p, [VP]),
     "gra_hip_eval_points": (C.c_int, [VP, P(D), C.c_size_t, P(D), P(D), P(C.c_uint8)]),
     "gra_hip_eval_points_device": (C.c_int, [VP, VP, C.c_size_t, VP, VP, VP]),
+    "gra_hip_eval_points_noalloc": (C.c_int, [VP, VP, C.c_size_t, VP, VP, VP]),
     "gra_hip_grid_init": (C.c_int, [P(GraGrid), I32, I32]),
     "gra_hip_grid_optimize": (C.c_int, [P(GraGrid), P(GraIterStats), D]),
     "gra_hip_vegas_iterate": (C.c_int, [VP, P(GraGrid), U64, U64, C.c_uint32, P(GraIterStats)]),

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 
 #include "gra_internal.h"
@@ -720,6 +721,58 @@ int run_integrand(gra_handle* h, const double* d_grid, unsigned int bins, uint64
                           d_bins, h->d_rec.p, ncand, true);
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free evaluation: one array per device, in the code
+// object, so that gra_hip_eval_points_noalloc calls no allocator. Its device
+// address comes from hipGetSymbolAddress once per device, at the first handle
+// creation on that device; host code uses that address, never the symbol.
+// Layout: [0, 256) the candidate counter, then the candidate records, the VEGAS
+// weights (unused here: the points come from the caller) and the bin bytes, each
+// 256-byte aligned.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[GRA_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mtx;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Bytes of scratch n points of this process need (counter, records, weights, bins)
+size_t noalloc_bytes(const gra_handle* h, uint64_t n) {
+  const size_t recb = f32_kernels(h->P) ? k32::loop_record_bytes(h->P)
+                                        : k64::loop_record_bytes(h->P);
+  return 256 + (h->P.screening ? align256(recb * n) : 0) + align256(sizeof(double) * n) +
+         align256((size_t)h->P.dim * n);
+}
+// Largest n with noalloc_bytes(h, n) <= the scratch size
+uint64_t noalloc_max_points(const gra_handle* h) {
+  const size_t recb = f32_kernels(h->P) ? k32::loop_record_bytes(h->P)
+                                        : k64::loop_record_bytes(h->P);
+  const size_t per = (h->P.screening ? recb : 0) + sizeof(double) + (size_t)h->P.dim;
+  uint64_t n = (GRA_NOALLOC_SCRATCH_BYTES - 256 - 3 * 256) / per;
+  while (n > 0 && noalloc_bytes(h, n) > (size_t)GRA_NOALLOC_SCRATCH_BYTES) --n;
+  return n;
+}
+}  // namespace
+
+int scratch_setup(gra_handle* h) {
+  if (h->device < 0 || h->device >= kMaxDevices)
+    return set_global_error(GRA_E_HIP, "device %d beyond the scratch table", h->device);
+  std::lock_guard<std::mutex> lk(g_scratch_mtx);
+  if (!g_scratch_ptr[h->device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_global_error(GRA_E_HIP, "hipGetSymbolAddress(scratch): %s",
+                              hipGetErrorString(e));
+    g_scratch_ptr[h->device] = p;
+  }
+  h->scratch = g_scratch_ptr[h->device];
+  return GRA_OK;
+}
+
 }  // namespace gra_hip
 
 using namespace gra_hip;
@@ -747,6 +800,11 @@ int create_handle(const gra_cfg* cfg, const char* eikonal_dir, gra_handle** out)
     return set_global_error(GRA_E_HIP, "gra_hip_create: no HIP device available");
   }
   h->device = dev;
+  if (scratch_setup(h) != GRA_OK) {
+    const std::string m = global_error();
+    delete h;
+    return set_global_error(GRA_E_HIP, "gra_hip_create: %s", m.c_str());
+  }
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     delete h;
     return set_global_error(GRA_E_HIP, "gra_hip_create: stream creation failed");
@@ -835,6 +893,36 @@ extern "C" int gra_hip_eval_points_device(gra_handle* h, const double* d_u, size
   if (r) return r;
   GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
   time_collect(h, true, h->P.screening != 0, false);
+  return GRA_OK;
+}
+
+extern "C" int gra_hip_eval_points_noalloc(gra_handle* h, const double* d_u, size_t n,
+                                           double* d_W, double* d_amp2, uint8_t* d_flags) {
+  if (!h) return set_global_error(GRA_E_ARG, "handle is NULL");
+  if (n == 0) return GRA_OK;
+  if (!d_u || !d_W || !d_flags) return fail(h, GRA_E_ARG, "eval_points_noalloc: NULL buffer");
+  if (!h->scratch) return fail(h, GRA_E_STATE, "eval_points_noalloc: no scratch on this device");
+  const uint64_t nmax = noalloc_max_points(h);
+  if (n > nmax)
+    return fail(h, GRA_E_ARG,
+                "eval_points_noalloc: %zu points exceed the %llu this process fits in the "
+                "%d-byte scratch", n, (unsigned long long)nmax, (int)GRA_NOALLOC_SCRATCH_BYTES);
+  const size_t recb = f32_kernels(h->P) ? k32::loop_record_bytes(h->P)
+                                        : k64::loop_record_bytes(h->P);
+  unsigned char* base = (unsigned char*)h->scratch;
+  uint32_t* ncand = (uint32_t*)base;
+  size_t off = 256;
+  void* rec = base + off;
+  off += h->P.screening ? align256(recb * n) : 0;
+  double* vw = (double*)(base + off);
+  off += align256(sizeof(double) * n);
+  uint8_t* bins = base + off;
+  // the counter is cleared by a kernel (no memset node, so the call can be captured)
+  GRA_HIP_CHECK(h, launch_fill_u32(ncand, 0u, 1, h->stream));
+  const int r = launch_integrand(h, nullptr, 0, 0, 0, 0, d_u, n, d_W, d_amp2, d_flags, vw, bins,
+                                 rec, ncand, false);
+  if (r) return r;
+  GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
   return GRA_OK;
 }
 

@@ -86,6 +86,9 @@ struct gra_handle {
   double time_ms[gra_hip::TS_N] = {};
   uint64_t launches[gra_hip::TS_N] = {};
 
+  // this device's allocation-free scratch (device address of the scratch array)
+  void* scratch = nullptr;
+
   // CheckInfNan counters over the handle's life (include/Graniitti/MProcess.h:317-340)
   uint64_t n_nan_seen = 0, n_inf_seen = 0;
 

@@ -36,6 +36,8 @@ constexpr int kIterScalars = 11;
 hipError_t launch_accumulate(const double* W, const double* vw, const uint8_t* bins,
                              const uint8_t* flags, uint64_t n, int D, int nblk, double* part_hist,
                              double* part_scal, double* out, hipStream_t st);
+// p[0 .. n) = v (a kernel, not a memset node)
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
 hipError_t launch_map(const double* grid, int D, unsigned int bins, const double* u, uint64_t n,
                       double* x, double* vw, int32_t* b, hipStream_t st);
 hipError_t launch_accept(const double* W, const double* vw, const uint8_t* flags, uint64_t n,

@@ -304,10 +304,21 @@ __global__ void k_accept(const double* __restrict__ W, const double* __restrict_
   }
 }
 
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 }  // namespace
 
 namespace gra_hip {
 
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  const unsigned int nb = (unsigned int)((n + 255) / 256);
+  hipLaunchKernelGGL(k_fill_u32, dim3(nb), dim3(256), 0, st, p, v, (uint64_t)n);
+  return hipGetLastError();
+}
 
 hipError_t launch_accept(const double* W, const double* vw, const uint8_t* flags, uint64_t n,
                          double maxf, uint64_t seed, uint32_t iter, uint64_t first, uint64_t* idx,

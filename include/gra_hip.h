@@ -367,6 +367,24 @@ int gra_hip_eval_points(gra_handle* h, const double* u, size_t n, double* W, dou
 int gra_hip_eval_points_device(gra_handle* h, const double* d_u, size_t n, double* d_W,
                                double* d_amp2, uint8_t* d_flags);
 
+/* Size of the per-device scratch array of gra_hip_eval_points_noalloc (64 MiB):
+ * every GPU the library runs on holds it from the first handle creation on that
+ * device until the process ends. */
+#define GRA_NOALLOC_SCRATCH_BYTES (64 << 20)
+
+/* gra_hip_eval_points_device without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). Intermediate buffers (the
+ * candidate counter and records of the screening stage, per-point bin bytes) live
+ * in one static per-device array of GRA_NOALLOC_SCRATCH_BYTES, so n is limited to
+ * what fits there for the process (about 2.8e5 points for the screened FP32
+ * continuum); larger n fails with GRA_E_ARG. Inputs and outputs are the caller's
+ * device buffers, as for gra_hip_eval_points_device; the call runs on the handle's
+ * stream and returns when it is done. Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (several handles or threads) race. */
+int gra_hip_eval_points_noalloc(gra_handle* h, const double* d_u, size_t n, double* d_W,
+                                double* d_amp2, uint8_t* d_flags);
+
 /* Initialize a uniform grid (VEGASData::Init(0) + InitGridDependent,
  * include/Graniitti/MVEGAS.h:38-84). */
 int gra_hip_grid_init(gra_grid* grid, int32_t dim, int32_t bins);
