@@ -2,24 +2,27 @@
 
 Workload (default): pp -> p pi+ pi- p, sqrt(s) = 13 TeV, |eta| < 0.9, pT > 0.15 GeV,
 single-channel eikonal screening (16 phi x 31 kt loop nodes, MaxLoopKT 3 GeV,
-10001-node eikonal tables), FP64 kinematics and Jacobian, FP32 amplitude with
-compensated FP32 loop sums, VEGAS with 2^20 points per iteration.
+10001-row eikonal tables), FP64 kinematics and Jacobian, FP32 amplitude with FP32
+row sums of the loop nodes added in FP64, 2^20 points per pass.
 
-A step is one VEGAS iteration: 2^20 points drawn (Philox), mapped through the
-grid, weighted (tree kernel + screening kernel), accumulated, and the grid
-updated. W untimed warm-up iterations adapt the grid; K iterations are timed.
+The run follows the reference's SampleVegas (src/MGraniitti.cc:1074-1135):
+1. integration: VEGAS with W burn-in iterations (grid adaptation) and 4
+   iterations of 2^20 points that fix max f, the largest f = W vw seen while
+   integrating (the maxf that SaveEvent's hit-or-miss uses);
+2. generation, the timed steps: K passes of 2^20 points through the adapted grid,
+   each drawing the points (Philox), weighting them (tree + screening kernels),
+   accepting U < f / max f (src/MGraniitti.cc:1651-1663) and writing the accepted
+   events' four-momenta to host memory.
+value = unweighted events produced per second by the timed passes. Because max f
+comes from a fixed integration sample, the value does not depend on K.
 
-value = unweighted events/s = points/s x unweighting efficiency <f>/max f of
-the timed iterations (MGraniitti::SaveEvent hit-or-miss, src/MGraniitti.cc:1651-1663).
-Extra fields: points/s, time to 0.1% relative sigma error (a separate VEGAS run
-to that precision, grid adaptation included), sigma.
-
-"flow": the learned sampler on the same process (6 RQS coupling layers, 16 bins,
-2 x 128 hidden in BF16 MFMA), composed with the adapted grid, trained 500 Adam
-steps on batches of 2^18 of its own weighted points and then sampled to 0.1%
-relative sigma error; its time to precision is given with and without training,
-with sigma consistency against VEGAS, variance reduction and unweighting
-efficiency against the grid it starts from.
+Extra fields: points/s of the generation passes, time to 0.1% relative sigma error
+of plain VEGAS (a separate run, grid adaptation included), the flow leg (the
+learned sampler on the same process, with the same max-weight convention: the
+largest weight of a 4 x 2^20 integration sample, then produced events/s from its
+own generation passes, and its time to 0.1% with training included), the flow on
+the screened rho rho tensor-pomeron process, PP[RES+CON] throughput, the k_screen
+roofline and the CPU baseline (the FP64 oracle on the host's cores).
 """
 import argparse
 import ctypes as C
@@ -36,114 +39,175 @@ sys.path.insert(0, ROOT)
 from graniitti_amd import abi  # noqa: E402
 from graniitti_amd.process import MContinuum, MGraniitti, vegas_cfg  # noqa: E402
 
-# Algorithmic work of one screening-loop node of the FP32 amplitude kernel
-# (kinematics as changes from the tree point, invariants, Regge amplitude, row
-# sum), counted on the source: add / mul = 1, fma = 2, each transcendental or
-# reciprocal = 1. A gfx950 SIMD issues up to two FP32 VALU instructions per
-# quad-cycle (the SQ_ACTIVE_INST_VALU2 counter counts the quad-cycles that do),
-# so the bound is the 157.3 TFLOP/s FP32 vector peak, even without packed FP32.
+# Algorithmic work of one screening-loop node of the FP32 amplitude kernel, counted
+# from its gfx950 assembly along the path of an accepted node (TUNE0: offshellFF = 2,
+# series SolvePz step): 52 FMA (2 each), 19 multiplies, 15 adds or subtractions and
+# 9 transcendental or reciprocal instructions (4 log2, 2 exp2, 1 rcp, sin, cos; 1
+# each); comparisons, selects and moves are not counted. A gfx950 SIMD issues up to
+# two FP32 VALU instructions per quad-cycle (SQ_ACTIVE_INST_VALU2 counts the
+# quad-cycles that do), so the bound is the 157.3 TFLOP/s FP32 vector peak, even
+# without packed FP32.
 FLOP_PER_NODE = 147
 VALU_PEAK_TFLOPS = 157.3
+# HBM bytes of one screening launch at the default workload, from rocprofv3
+# FETCH_SIZE / WRITE_SIZE runs of this command (the counters cannot be read inside
+# the run): profiles/k_screen_hbm_traffic_screened_pipi.md
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_screen_hbm_traffic_screened_pipi.md")
+MAXF_SAMPLE = 4  # integration iterations (of the pass size) that fix max f / max w
+EVENT_CAP = 1 << 17
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(cfg, grid, eff, n_sample, threads):
-    """The oracle's multithreaded FP64 restatement on a bounded sample of the
-    same workload (same grid), per-thread accumulators."""
+def cpu_threads():
+    """Host threads of this process's CPU share (the cores it may run on, capped
+    by OMP_NUM_THREADS where the host sets it)."""
+    n = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
+
+
+def cpu_baseline(cfg, grid, eff, n_sample):
+    """The oracle's FP64 restatement (built with the reference's optimisation
+    flags) with the reference's threading model on a bounded sample of the same
+    workload and grid: fork/join threads per iteration with one global mutex per
+    point (src/MGraniitti.cc:1389-1419) at 1.0x and at the reference's default
+    1.5x the hardware threads (include/Graniitti/MGraniitti.h:135-147), and with
+    per-thread accumulators. value = the fastest mode's points/s x the GPU run's
+    unweighting efficiency."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as ol
 
     cfg64 = abi.GraCfg.from_buffer_copy(cfg)
     cfg64.amp_precision = abi.GRA_AMP_FP64
+    hw = cpu_threads()
+    modes = [("mutex", hw, 1), ("mutex", int(round(1.5 * hw)), 1), ("per_thread", hw, 0)]
     t0 = time.time()
-    R = ol.OracleRun(cfg64, threads)
+    R = ol.OracleRun(cfg64, hw)
     t_tables = time.time() - t0
-    t0 = time.time()
-    st = R.vegas_iterate(grid, n_sample, 987654321, 0, nthreads=threads)
-    dt = time.time() - t0
+    runs = []
+    for name, th, mtx in modes:
+        t0 = time.time()
+        st = R.vegas_iterate(grid, n_sample, 987654321, 0, nthreads=th, use_mutex=mtx)
+        dt = time.time() - t0
+        runs.append({"mode": name, "threads": th, "points_per_s": n_sample / dt, "seconds": dt,
+                     "n_valid": int(st.n_valid)})
+        log(f"cpu baseline {name} x{th}: {n_sample / dt:.4g} points/s ({dt:.1f} s)")
     R.close()
-    pts = n_sample / dt
+    best = max(runs, key=lambda r: r["points_per_s"])
     return {
-        "value": pts * eff, "unit": "unweighted events/s", "cores": threads, "kind": "port",
-        "points_per_s": pts,
-        "sample": f"{n_sample} points of the same workload on the same adapted grid, "
-                  f"FP64 oracle, {threads} threads ({dt:.1f} s; eikonal tables {t_tables:.1f} s "
-                  f"not included); value = points/s x the GPU run's unweighting efficiency",
-        "n_valid": int(st.n_valid),
+        "value": best["points_per_s"] * eff, "unit": "unweighted events/s",
+        "cores": best["threads"], "kind": "port", "host_cpus": os.cpu_count(),
+        "cpu_share": hw, "points_per_s": best["points_per_s"], "modes": runs,
+        "sample": f"{n_sample} points of the same workload on the same adapted grid per mode, "
+                  f"FP64 oracle built with the reference's flags (-O3 -ftree-vectorize), "
+                  f"eikonal tables ({t_tables:.1f} s) not included; value = the fastest mode's "
+                  f"points/s x the GPU run's unweighting efficiency",
     }
 
 
-def flow_stats_sums(st):
-    """(sum w, sum w^2, max w, n) of one flow_sample call, from its moments."""
-    n = float(st.n)
-    var = st.sigma_err ** 2 * n
-    return st.sigma * n, (var + st.sigma ** 2) * n, st.maxw, n
+def vegas_generate(P, G, n, seed, first_iter, passes, maxf):
+    """Generation passes on the adapted grid: (events produced, trials, seconds, stats)"""
+    buf = (abi.GraEvent * EVENT_CAP)()
+    tot = {"trials": 0, "valid": 0, "accepted": 0, "overflow": 0, "stored": 0}
+    t0 = time.perf_counter()
+    for k in range(passes):
+        st = abi.GraGenStats()
+        abi.check(P.lib.gra_hip_generate(P.h, C.byref(G.grid), n, seed, first_iter + k, maxf,
+                                         buf, EVENT_CAP, C.byref(st)), P.h)
+        for key in tot:
+            tot[key] += getattr(st, key)
+    return tot, time.perf_counter() - t0
 
 
-def flow_leg(P, grid, args, vegas_sigma, vegas_err):
-    """Learned sampler on the same process: RQS flow (by default 6 coupling
-    layers, 16 bins, 2 x 128 hidden, BF16 MFMA) composed with the adapted VEGAS
-    grid, trained by weight-variance minimisation (Adam, lr 1e-3) on batches of
-    its own weighted points (2^18 by default), then sampled in chunks of 2^20
-    until the relative sigma error reaches 0.1%."""
-    lib = P.lib
+def flow_cfg(dim, layers, bins, hidden, depth, batch, seed=2024):
     fc = abi.GraFlowCfg()
-    fc.dim, fc.layers, fc.bins = P.cfg.dim, args.flow_layers, args.flow_bins
-    fc.hidden, fc.depth = args.flow_hidden, args.flow_depth
-    fc.steps, fc.batch, fc.lr = args.flow_steps, args.flow_batch, 1e-3
+    fc.dim, fc.layers, fc.bins, fc.hidden, fc.depth = dim, layers, bins, hidden, depth
+    fc.steps, fc.batch, fc.lr = 0, batch, 1e-3
     fc.beta1, fc.beta2, fc.eps = 0.9, 0.999, 1e-8
-    fc.seed, fc.ess_fallback = 2024, 0.1
+    fc.seed, fc.ess_fallback = seed, 0.1
+    return fc
+
+
+def flow_leg(P, grid, fc, steps, n_pass, vegas, gen_passes):
+    """The learned sampler on the process of P: an RQS flow composed with the
+    adapted VEGAS grid (identity at initialisation), trained by weight-variance
+    minimisation (Adam, lr 1e-3) on batches of its own weighted points, then
+    - gra_hip_flow_integrate to 0.1% relative sigma error (training included),
+      with the VEGAS fallback below an effective sample size of 10%;
+    - max w of a 4 x n_pass sample (the same convention as VEGAS's max f);
+    - produced unweighted events/s of gen_passes flow generation passes."""
+    lib = P.lib
     abi.check(lib.gra_hip_flow_init(P.h, C.byref(fc)), P.h)
     abi.check(lib.gra_hip_flow_set_grid(P.h, C.byref(grid)), P.h)
-    chunk = 1 << 20
-    # the untrained flow is the identity: it samples exactly the frozen VEGAS grid
     s0 = abi.GraFlowStats()
-    abi.check(lib.gra_hip_flow_sample(P.h, chunk, 11, C.byref(s0)), P.h)
-    st = abi.GraFlowStats()
+    abi.check(lib.gra_hip_flow_sample(P.h, n_pass, 11, None, None, C.byref(s0)), P.h)
+    rc = abi.GraFlowRunCfg()
+    rc.train_steps, rc.n_max, rc.seed, rc.precision, rc.ess_min = steps, 64 * n_pass, 1000, 1e-3, 0.1
+    rc.vegas = vegas_cfg(n_pass, 1, burnin=5, seed=31337, precision=1e-3, max_iter=60)
+    g_fb = abi.GraGrid()
+    res = abi.GraFlowResult()
+    abi.check(lib.gra_hip_flow_integrate(P.h, C.byref(rc), C.byref(g_fb), C.byref(res)), P.h)
+    log(f"flow: trained {steps} steps x {fc.batch} in {res.train_seconds:.3f} s; sampled "
+        f"{res.n} points in {res.sample_seconds:.3f} s; sigma {res.sigma:.6e} +- "
+        f"{res.sigma_err:.2e}, fallback {res.fallback}")
+    sm = abi.GraFlowStats()
+    abi.check(lib.gra_hip_flow_sample(P.h, MAXF_SAMPLE * n_pass, 2000, None, None, C.byref(sm)),
+              P.h)
+    maxw = sm.maxw
+    buf = (abi.GraEvent * EVENT_CAP)()
+    tot = {"trials": 0, "valid": 0, "accepted": 0, "overflow": 0, "stored": 0}
     t0 = time.perf_counter()
-    abi.check(lib.gra_hip_flow_train(P.h, args.flow_steps, C.byref(st)), P.h)
-    t_train = time.perf_counter() - t0
-    log(f"flow: trained {args.flow_steps} steps x {args.flow_batch} in {t_train:.3f} s; "
-        f"last batch cv2 {st.cv2:.3f}")
-    s1 = s2 = mx = n = 0.0
-    t0 = time.perf_counter()
-    calls = 0
-    while True:
-        one = abi.GraFlowStats()
-        abi.check(lib.gra_hip_flow_sample(P.h, chunk, 1000 + calls, C.byref(one)), P.h)
-        calls += 1
-        a, b, m, k = flow_stats_sums(one)
-        s1, s2, mx, n = s1 + a, s2 + b, max(mx, m), n + k
-        mean = s1 / n
-        err = np.sqrt(max(s2 / n - mean * mean, 0.0) / n)
-        if err <= 1e-3 * mean or calls >= 64:
-            break
-    t_sample = time.perf_counter() - t0
-    var = max(s2 / n - mean * mean, 0.0)
-    cv2 = var / mean ** 2
-    eff = mean / mx
-    pts = n / t_sample
-    comb = np.hypot(err, vegas_err)
-    res = {
-        "layers": args.flow_layers, "bins": args.flow_bins,
-        "hidden": f"{args.flow_depth}x{args.flow_hidden}", "mlp": "bf16 mfma, fp32 accumulate",
-        "train_steps": args.flow_steps, "train_batch": args.flow_batch, "train_s": t_train,
-        "sample_s_to_0p1pct": t_sample, "time_to_0p1pct_s": t_train + t_sample,
-        "points_to_0p1pct": int(n), "reached_0p1pct": bool(err <= 1e-3 * mean),
-        "sigma_barn": mean, "sigma_err_barn": err,
-        "sigma_pull_vs_vegas": (mean - vegas_sigma) / comb,
-        "consistent_3sigma": bool(abs(mean - vegas_sigma) < 3 * comb),
-        "cv2": cv2, "cv2_vegas_grid": s0.cv2, "variance_reduction": s0.cv2 / cv2 if cv2 > 0 else None,
-        "ess": 1.0 / (1.0 + cv2), "ess_vegas_grid": s0.ess,
-        "unweighting_efficiency": eff, "unweighting_efficiency_vegas_grid": s0.sigma / s0.maxw,
-        "points_per_s": pts, "unweighted_events_per_s": pts * eff,
-        "vegas_fallback": bool(1.0 / (1.0 + cv2) < fc.ess_fallback),
+    for k in range(gen_passes):
+        st = abi.GraGenStats()
+        abi.check(lib.gra_hip_flow_generate(P.h, n_pass, 3000, k, maxw, buf, EVENT_CAP,
+                                            C.byref(st)), P.h)
+        for key in tot:
+            tot[key] += getattr(st, key)
+    t_gen = time.perf_counter() - t0
+    comb = np.hypot(res.sigma_err, vegas["sigma_err"])
+    out = {
+        "layers": fc.layers, "bins": fc.bins, "hidden": f"{fc.depth}x{fc.hidden}",
+        "mlp": "bf16 mfma, fp32 accumulate", "train_steps": steps, "train_batch": int(fc.batch),
+        "train_s": res.train_seconds, "sample_s_to_0p1pct": res.sample_seconds,
+        "time_to_0p1pct_s": res.seconds_to_precision if res.reached else None,
+        "reached_0p1pct": bool(res.reached), "points_to_0p1pct": int(res.n),
+        "vegas_fallback": bool(res.fallback),
+        "sigma_barn": res.sigma, "sigma_err_barn": res.sigma_err,
+        "sigma_pull_vs_vegas": (res.sigma - vegas["sigma"]) / comb,
+        "consistent_3sigma": bool(abs(res.sigma - vegas["sigma"]) < 3 * comb),
+        "cv2": res.cv2, "cv2_vegas_grid": s0.cv2,
+        "variance_reduction": s0.cv2 / res.cv2 if res.cv2 > 0 else None,
+        "ess": res.ess, "ess_vegas_grid": s0.ess,
+        "maxw_sample_points": MAXF_SAMPLE * n_pass,
+        "generation": {"passes": gen_passes, "points_per_pass": n_pass, **tot,
+                       "seconds": t_gen,
+                       "unweighting_efficiency": tot["accepted"] / tot["trials"],
+                       "points_per_s": tot["trials"] / t_gen,
+                       "unweighted_events_per_s": tot["stored"] / t_gen},
+        "time_ratio_vs_vegas": (res.seconds_to_precision / vegas["time_to_0p1pct_s"]
+                                if res.reached and vegas["time_to_0p1pct_s"] else None),
     }
-    log(f"flow: {res}")
-    return res
+    log(f"flow: {out}")
+    return out
+
+
+def vegas_precision_run(P, n, warmup, seed):
+    """Plain VEGAS to 0.1% relative sigma error, grid adaptation included"""
+    G = MGraniitti(P)
+    vc = vegas_cfg(n, 1, burnin=warmup, seed=seed, precision=1e-3, max_iter=60)
+    res = G.Integrate(vc)
+    t = res.seconds_to_precision if res.seconds_to_precision > 0 else None
+    out = {"sigma": res.sigma, "sigma_err": res.sigma_err, "iterations": res.iterations,
+           "restarts": res.restarts, "seconds_total": res.seconds_total,
+           "calls_total": int(res.calls_total), "time_to_0p1pct_s": t,
+           "integration_points_per_s": res.calls_total / res.seconds_total}
+    log(f"VEGAS precision run: {out}")
+    return out
 
 
 WORKLOADS = {
@@ -162,14 +226,14 @@ COPY_BYTES = 256 << 20
 
 def res_leg():
     """PP[RES+CON] -> pi+ pi- (continuum + the eight ALICECUTS resonances, FP64 amplitude):
-    VEGAS throughput unscreened (2^20 points/iteration) and screened (2^18)."""
+    VEGAS integration throughput unscreened (2^20 points/iteration) and screened (2^18)."""
     out = {}
     for preset, n, warm, steps in ((6, 1 << 20, 2, 5), (7, 1 << 18, 1, 3)):
         cfg = abi.cfg_default(preset)
         with MContinuum(cfg) as P:
             grid = abi.GraGrid()
             abi.check(P.lib.gra_hip_grid_init(C.byref(grid), cfg.dim, 128))
-            it, fs, mx, acc = 0, 0.0, 0.0, [0.0, 0.0]
+            it, acc = 0, [0.0, 0.0]
             for k in range(warm + steps):
                 if k == warm:
                     t0 = time.perf_counter()
@@ -177,22 +241,45 @@ def res_leg():
                 it += 1
                 abi.check(P.lib.gra_hip_grid_optimize(C.byref(grid), C.byref(st), 1.5))
                 if k >= warm:
-                    fs += st.fsum
-                    mx = max(mx, st.maxf)
                     I = st.fsum / n
                     v = max((st.f2sum / n - I * I) / n, 1e-300)
                     acc[0] += I / v
                     acc[1] += 1.0 / v
             dt = time.perf_counter() - t0
         pps = n * steps / dt
-        eff = fs / (n * steps) / mx if mx > 0 else 0.0
         out["screened" if preset == 7 else "unscreened"] = {
             "preset": preset, "points_per_step": n, "steps": steps, "points_per_s": pps,
-            "unweighting_efficiency": eff, "unweighted_events_per_s": pps * eff,
             "sigma_barn": acc[0] / acc[1], "sigma_err_barn": acc[1] ** -0.5}
-        log(f"PP[RES+CON] preset {preset}: {pps:.3e} points/s, eff {eff:.4f}, "
-            f"sigma {acc[0] / acc[1]:.5e} barn")
+        log(f"PP[RES+CON] preset {preset}: {pps:.3e} points/s, sigma {acc[0] / acc[1]:.5e} barn")
     return out
+
+
+def rhorho_leg(args):
+    """The screened rho rho tensor-pomeron process (preset 3, FP32 amplitude):
+    VEGAS to 0.1% at 2^21 points per iteration, and the flow of 10 layers, K = 32,
+    3 x 256 against it."""
+    cfg = abi.cfg_default(3)
+    n = 1 << 21
+    with MContinuum(cfg) as P:
+        v = vegas_precision_run(P, n, 5, 1717)
+        G = MGraniitti(P)
+        G.Integrate(vegas_cfg(n, MAXF_SAMPLE, burnin=5, seed=1818))
+        fc = flow_cfg(cfg.dim, 10, 32, 256, 3, 1 << 18)
+        fl = flow_leg(P, G.grid, fc, args.flow5_steps, 1 << 20, v, 2)
+    return {"preset": 3, "vegas": v, "flow": fl}
+
+
+def config_mode_leg(N, warmup, base):
+    """The workload in its stated loop discretisation (preset 11: 32 Gauss-Legendre
+    kt nodes x 16 azimuths, A_eik on 64 log-spaced kt nodes) against the reference's
+    (Simpson 3/8 on 16 x 31 nodes, 10001-row A_eik table): VEGAS to 0.1%."""
+    with MContinuum(abi.cfg_default(11)) as P:
+        v = vegas_precision_run(P, N, warmup, 424242)
+    comb = np.hypot(v["sigma_err"], base["sigma_err"])
+    return {"preset": 11, "sigma_barn": v["sigma"], "sigma_err_barn": v["sigma_err"],
+            "time_to_0p1pct_s": v["time_to_0p1pct_s"],
+            "delta_sigma_rel": v["sigma"] / base["sigma"] - 1.0,
+            "delta_sigma_pull": (v["sigma"] - base["sigma"]) / comb}
 
 
 def calibrate_copy(reps=3):
@@ -212,12 +299,10 @@ def calibrate_copy(reps=3):
 
 
 def screen_traffic():
-    """HBM bytes per screening launch measured with FETCH_SIZE / WRITE_SIZE
-    (profiles/k_screen_hbm_traffic_screened_pipi.md), or None."""
-    path = os.path.join(ROOT, "profiles", "k_screen_hbm_traffic_screened_pipi.md")
-    if not os.path.exists(path):
+    """HBM bytes per screening launch measured with FETCH_SIZE / WRITE_SIZE, or None."""
+    if not os.path.exists(TRAFFIC_PROFILE):
         return None
-    for line in open(path):
+    for line in open(TRAFFIC_PROFILE):
         if line.startswith("traffic_bytes_per_launch:"):
             return float(line.split(":", 1)[1])
     return None
@@ -226,7 +311,7 @@ def screen_traffic():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--points", type=int, default=1 << 20)
     ap.add_argument("--preset", type=int, default=2)
@@ -234,12 +319,14 @@ def main():
                     help="skip the PP[RES+CON] resonance throughput leg")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-precision-run", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=1 << 20)
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 19)
     ap.add_argument("--no-flow", action="store_true")
+    ap.add_argument("--no-flow5", action="store_true",
+                    help="skip the flow on the screened rho rho process")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run 256 MiB device copies (counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--flow-steps", type=int, default=500)
+    ap.add_argument("--flow5-steps", type=int, default=200)
     ap.add_argument("--flow-layers", type=int, default=6)
     ap.add_argument("--flow-bins", type=int, default=16)
     ap.add_argument("--flow-hidden", type=int, default=128)
@@ -256,57 +343,33 @@ def main():
     cfg = abi.cfg_default(args.preset)
     N = args.points
     P = MContinuum(cfg)
-    lib = P.lib
-    grid = abi.GraGrid()
-    abi.check(lib.gra_hip_grid_init(C.byref(grid), cfg.dim, 128))
+    G = MGraniitti(P)
     seed = 12345678
-    it = 0
-    log(f"workload preset {args.preset}: {N} points/iteration, warmup {args.warmup}, "
-        f"steps {args.steps}")
-    for _ in range(args.warmup):
-        st = P.vegas_iterate(grid, N, seed, it)
-        it += 1
-        abi.check(lib.gra_hip_grid_optimize(C.byref(grid), C.byref(st), 1.5))
+    log(f"workload preset {args.preset}: {N} points/pass, burn-in {args.warmup}, "
+        f"generation passes {args.steps}")
+    # integration: burn-in adapts the grid, MAXF_SAMPLE iterations fix max f
+    integ = G.Integrate(vegas_cfg(N, MAXF_SAMPLE, burnin=args.warmup, seed=seed))
+    maxf = integ.maxf
     for s in range(4):
         P.kernel_time(s, reset=True)
-    fsum = f2sum = 0.0
-    maxf = 0.0
-    n_valid = 0
-    state = [0.0, 0.0, 0.0]
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        st = P.vegas_iterate(grid, N, seed, it)
-        it += 1
-        abi.check(lib.gra_hip_grid_optimize(C.byref(grid), C.byref(st), 1.5))
-        fsum += st.fsum
-        f2sum += st.f2sum
-        maxf = max(maxf, st.maxf)
-        n_valid += st.n_valid
-        # iteration combination (src/MGraniitti.cc:1222-1246) for the sigma report
-        I = st.fsum / N
-        v = max((st.f2sum / N - I * I) / N, 1e-300)
-        state[0] += I / v
-        state[1] += 1.0 / v
-    t1 = time.perf_counter()
-    dt = t1 - t0
+    # timed steps: generation passes
+    tot, dt = vegas_generate(P, G, N, seed, 1000, args.steps, maxf)
     ms_tree, n_tree = P.kernel_time(0)
     ms_scr, n_scr = P.kernel_time(1)
-    ms_acc, n_acc = P.kernel_time(2)
-    pts_per_s = N * args.steps / dt
-    mean_f = fsum / (N * args.steps)
-    eff = mean_f / maxf if maxf > 0 else 0.0
-    value = pts_per_s * eff
-    sigma = state[0] / state[1]
-    sigma_err = 1.0 / np.sqrt(state[1])
-    log(f"timed: {dt:.3f} s, {pts_per_s:.3e} points/s, eff {eff:.4f}, sigma {sigma:.5e} +- "
-        f"{sigma_err:.2e} barn")
+    pts_per_s = tot["trials"] / dt
+    eff = tot["accepted"] / tot["trials"]
+    value = tot["stored"] / dt
+    if tot["stored"] != tot["accepted"]:
+        log(f"warning: {tot['accepted'] - tot['stored']} accepted events beyond the buffer")
+    log(f"timed: {dt:.3f} s, {pts_per_s:.3e} points/s, efficiency {eff:.5f}, "
+        f"{value:.4e} events/s; integration sigma {integ.sigma:.5e} +- {integ.sigma_err:.2e} b")
 
     roofline = None
     # per-node operation counts are those of the FP32 Regge continuum node
     if (cfg.screening and n_scr > 0 and cfg.process == abi.GRA_PROC_REGGE_CON
             and cfg.amp_precision == abi.GRA_AMP_FP32):
         nodes = (cfg.n_loop_phi + 1) * (cfg.n_loop_kt + 1)
-        cand_per_launch = n_valid / n_scr
+        cand_per_launch = tot["valid"] / n_scr
         avg_ms = ms_scr / n_scr
         flop = FLOP_PER_NODE * nodes * cand_per_launch
         achieved = flop / (avg_ms * 1e-3) / 1e12
@@ -314,34 +377,40 @@ def main():
             "bound": "valu", "kernel": "k_screen (screening loop)",
             "achieved": achieved, "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / VALU_PEAK_TFLOPS, "traffic": screen_traffic(),
+            "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT),
             "avg_launch_ms": avg_ms, "launches": n_scr,
             "flop_per_node": FLOP_PER_NODE, "node_dtype": "f32",
             "nodes_per_point": nodes, "screened_points_per_launch": cand_per_launch,
         }
-    # time to 0.1% relative sigma error with plain VEGAS (GPU)
-    t_prec = None
+    vegas = {"sigma": integ.sigma, "sigma_err": integ.sigma_err, "time_to_0p1pct_s": None}
     prec_res = None
     if not args.no_precision_run:
-        G = MGraniitti(P)
-        vc = vegas_cfg(N, 1, burnin=args.warmup, seed=424242, precision=1e-3, max_iter=60)
-        res = G.Integrate(vc)
-        t_prec = res.seconds_to_precision if res.seconds_to_precision > 0 else None
-        prec_res = {"sigma": res.sigma, "sigma_err": res.sigma_err, "iterations": res.iterations,
-                    "seconds_total": res.seconds_total, "calls_total": int(res.calls_total)}
-        log(f"precision run: {prec_res}, t(0.1%) = {t_prec}")
-    flow = None
+        prec_res = vegas_precision_run(P, N, args.warmup, 424242)
+        vegas = prec_res
+    cmode = None
+    if prec_res is not None and args.preset == 2:
+        cmode = config_mode_leg(N, args.warmup, prec_res)
+        log(f"config mode: {cmode}")
     res = None
     if not args.no_res and args.preset == 2:
         try:
             res = res_leg()
         except Exception as e:  # reported, never fatal to the headline line
             res = {"error": str(e)}
+    flow = None
     if not args.no_flow:
-        flow = flow_leg(P, grid, args, prec_res["sigma"] if prec_res else sigma,
-                        prec_res["sigma_err"] if prec_res else sigma_err)
+        fc = flow_cfg(cfg.dim, args.flow_layers, args.flow_bins, args.flow_hidden,
+                      args.flow_depth, args.flow_batch)
+        flow = flow_leg(P, G.grid, fc, args.flow_steps, N, vegas, 4)
+    flow5 = None
+    if not args.no_flow and not args.no_flow5 and args.preset == 2:
+        try:
+            flow5 = rhorho_leg(args)
+        except Exception as e:
+            flow5 = {"error": str(e)}
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(cfg, grid, eff, args.cpu_sample, args.cpu_threads)
+        cpu = cpu_baseline(cfg, G.grid, eff, args.cpu_sample)
         log(f"cpu baseline: {cpu}")
     P.close()
     line = {
@@ -350,13 +419,18 @@ def main():
         "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
         "config": {"workload": WORKLOADS.get(args.preset, f"preset {args.preset}") +
-                   f", VEGAS {N} points/iteration",
+                   f", VEGAS generation passes of {N} points",
                    "preset": args.preset, "points_per_step": N, "integrator": "VEGAS"},
         "points_per_s": pts_per_s, "unweighting_efficiency": eff,
-        "time_to_0p1pct_s": t_prec, "precision_run": prec_res,
-        "sigma_barn": sigma, "sigma_err_barn": sigma_err,
-        "kernel_ms": {"tree": ms_tree, "screen": ms_scr, "accumulate": ms_acc},
-        "flow": flow, "resonances": res, "roofline": roofline, "cpu_baseline": cpu,
+        "generation": {**tot, "maxf": maxf,
+                       "maxf_sample": f"{MAXF_SAMPLE} x {N} integration points after "
+                                      f"{args.warmup} burn-in iterations"},
+        "time_to_0p1pct_s": vegas.get("time_to_0p1pct_s"), "precision_run": prec_res,
+        "sigma_barn": integ.sigma, "sigma_err_barn": integ.sigma_err,
+        "kernel_ms": {"tree": ms_tree, "screen": ms_scr},
+        "config_mode": cmode, "flow": flow, "flow_rhorho": flow5, "resonances": res,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(line))
 

@@ -26,6 +26,7 @@ GRA_FRAME_CM, GRA_FRAME_HX, GRA_FRAME_CS, GRA_FRAME_GJ = 0, 1, 2, 3
 GRA_AMP_FP64 = 0
 GRA_AMP_FP32 = 1
 GRA_PS_C, GRA_PS_F = 0, 1
+GRA_LOOP_SIMPSON38, GRA_LOOP_GAUSS = 0, 1
 FLAG_KIN_OK, FLAG_FID_OK, FLAG_VETO_OK, FLAG_AMP_OK, FLAG_INF = 1, 2, 4, 8, 16
 GRA_MAX_NONFINITE = 50
 
@@ -65,7 +66,7 @@ class GraCfg(C.Structure):
         ("tp_gPpipi", D), ("tp_gPKK", D), ("central_spin2", I32), ("pad_tp", I32),
         ("screening", I32), ("n_loop_phi", I32), ("n_loop_kt", I32), ("n_bt", I32),
         ("n_kt2", I32), ("n_fb", I32),
-        ("min_loop_kt", D), ("max_loop_kt", D),
+        ("min_loop_kt", D), ("max_loop_kt", D), ("loop_rule", I32), ("eik_log_nodes", I32),
         ("phase_space", I32), ("pad_ps", I32), ("gen_M_min", D), ("gen_M_max", D),
         ("n_res", I32), ("spin_gen", I32), ("spin_dec", I32), ("spin_frame", I32),
         ("regge_omega", D), ("res", GraRes * GRA_MAX_RES),
@@ -115,6 +116,21 @@ class GraFlowStats(C.Structure):
     ]
 
 
+class GraFlowRunCfg(C.Structure):
+    _fields_ = [
+        ("train_steps", I32), ("pad0", I32), ("n_max", U64), ("seed", U64), ("precision", D),
+        ("ess_min", D), ("vegas", GraVegasCfg),
+    ]
+
+
+class GraFlowResult(C.Structure):
+    _fields_ = [
+        ("sigma", D), ("sigma_err", D), ("cv2", D), ("ess", D), ("maxw", D),
+        ("train_seconds", D), ("sample_seconds", D), ("seconds_to_precision", D),
+        ("n", U64), ("fallback", I32), ("reached", I32), ("vegas", GraVegasResult),
+    ]
+
+
 class GraEvent(C.Structure):
     _fields_ = [("p", D * 4 * 8), ("f", D), ("index", U64), ("n_p", I32), ("pad0", I32)]
 
@@ -160,7 +176,10 @@ SIGNATURES = {
     "gra_hip_flow_init": (C.c_int, [VP, P(GraFlowCfg)]),
     "gra_hip_flow_set_grid": (C.c_int, [VP, P(GraGrid)]),
     "gra_hip_flow_train": (C.c_int, [VP, I32, P(GraFlowStats)]),
-    "gra_hip_flow_sample": (C.c_int, [VP, U64, U64, P(GraFlowStats)]),
+    "gra_hip_flow_sample": (C.c_int, [VP, U64, U64, P(D), P(D), P(GraFlowStats)]),
+    "gra_hip_flow_integrate": (C.c_int, [VP, P(GraFlowRunCfg), P(GraGrid), P(GraFlowResult)]),
+    "gra_hip_flow_generate": (C.c_int, [VP, U64, U64, C.c_uint32, D, P(GraEvent), U64,
+                                        P(GraGenStats)]),
     "gra_hip_flow_param_count": (C.c_int, [VP, P(U64)]),
     "gra_hip_flow_get_params": (C.c_int, [VP, P(C.c_float)]),
     "gra_hip_flow_set_params": (C.c_int, [VP, P(C.c_float)]),

@@ -144,6 +144,31 @@ double gev2barn() {
   return GeV2m2 / 1E-28;
 }
 
+// Gauss-Legendre nodes and weights on [-1, 1] (Newton iteration on P_n from the
+// Chebyshev-like first guesses, as in Numerical Recipes' gauleg)
+void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
+  x.assign(n, 0.0);
+  w.assign(n, 0.0);
+  for (int i = 0; i < (n + 1) / 2; ++i) {
+    double z = std::cos(GRA_PI * (i + 0.75) / (n + 0.5)), pp = 0.0;
+    for (int it = 0; it < 100; ++it) {
+      double p1 = 1.0, p2 = 0.0;
+      for (int k = 0; k < n; ++k) {
+        const double p3 = p2;
+        p2 = p1;
+        p1 = ((2.0 * k + 1.0) * z * p2 - k * p3) / (k + 1.0);
+      }
+      pp = n * (z * p1 - p2) / (z * z - 1.0);
+      const double z1 = z;
+      z = z1 - p1 / pp;
+      if (std::fabs(z - z1) < 1e-15) break;
+    }
+    x[i] = -z;
+    x[n - 1 - i] = z;
+    w[i] = w[n - 1 - i] = 2.0 / ((1.0 - z * z) * pp * pp);
+  }
+}
+
 double simpson38_w(int i, int N) {
   if (i == 0 || i == N) return 1.0;
   return (i % 3 == 0) ? 2.0 : 3.0;
@@ -302,10 +327,22 @@ int validate(const gra_cfg* c) {
     return set_global_error(GRA_E_ARG, "empty generation-cut interval");
   if (c->S_factor <= 0.0) return set_global_error(GRA_E_ARG, "S_factor must be > 0");
   if (c->screening) {
-    if (c->n_loop_phi < 3 || c->n_loop_phi % 3 || c->n_loop_kt < 3 || c->n_loop_kt % 3)
+    if (c->loop_rule == GRA_LOOP_SIMPSON38) {
+      if (c->n_loop_phi < 3 || c->n_loop_phi % 3 || c->n_loop_kt < 3 || c->n_loop_kt % 3)
+        return set_global_error(GRA_E_ARG,
+                                "loop intervals must be multiples of 3 (Simpson 3/8): phi %d, kt %d",
+                                c->n_loop_phi, c->n_loop_kt);
+    } else if (c->loop_rule == GRA_LOOP_GAUSS) {
+      if (c->n_loop_phi < 0 || c->n_loop_kt < 0)
+        return set_global_error(GRA_E_ARG, "loop node counts must be >= 1");
+    } else {
+      return set_global_error(GRA_E_ARG, "loop_rule %d unknown", c->loop_rule);
+    }
+    if (c->eik_log_nodes != 0 &&
+        (c->eik_log_nodes < 2 || c->eik_log_nodes > 4096 || !(c->min_loop_kt > 0.0)))
       return set_global_error(GRA_E_ARG,
-                              "loop intervals must be multiples of 3 (Simpson 3/8): phi %d, kt %d",
-                              c->n_loop_phi, c->n_loop_kt);
+                              "eik_log_nodes %d: 0, or 2..4096 log-spaced nodes with MinLoopKT > 0",
+                              c->eik_log_nodes);
     if ((c->n_loop_phi + 1) * (c->n_loop_kt + 1) > 1024)
       return set_global_error(GRA_E_ARG, "too many loop nodes (max 1024, LDS node table)");
     if (c->n_fb < 2 || c->n_fb % 2)
@@ -493,7 +530,9 @@ void fill_params(const gra_cfg& c, RunParams& P) {
   P.step_kt = (c.max_loop_kt - c.min_loop_kt) / c.n_loop_kt;
   P.step_phi = (2.0 * GRA_PI - 0.0) / c.n_loop_phi;
   P.min_loop_kt = c.min_loop_kt;
-  P.hfac = 9 * (P.step_phi * P.step_kt) / 64;
+  // Simpson 3/8 in both variables (src/MProcess.cc:254-261); the Gauss rule's
+  // node weights carry their step sizes themselves
+  P.hfac = (c.loop_rule == GRA_LOOP_GAUSS) ? 1.0 : 9 * (P.step_phi * P.step_kt) / 64;
   P.nrm = 1.0 / (8.0 * (GRA_PI * GRA_PI) * P.s);
   P.kappa = P.nrm * P.hfac;
   // tensor pomeron
@@ -544,7 +583,7 @@ int build_eikonal(gra_handle* h, const char* cache_dir) {
   const int NF = c.n_fb;
   std::vector<double> pre(2 * (size_t)(NF + 1));
   DevBuf d_pre, d_out;
-  const int nrow_max = std::max(c.n_bt, c.n_kt2) + 1;
+  const int nrow_max = std::max(std::max(c.n_bt, c.n_kt2) + 1, c.eik_log_nodes);
   if (d_pre.ensure(sizeof(double) * pre.size()) || d_out.ensure(sizeof(double) * 2 * nrow_max))
     return fail(h, GRA_E_NOMEM, "eikonal table buffers");
   std::vector<double> out(2 * (size_t)nrow_max);
@@ -562,7 +601,7 @@ int build_eikonal(gra_handle* h, const char* cache_dir) {
                                     hipMemcpyHostToDevice, h->stream));
     GRA_HIP_CHECK(h, launch_fb_table(0, d_pre.as<double>(), NF, kmin, kstep, bmin, bstep,
                                      c.n_bt + 1, 1 / (2.0 * GRA_PI * P.s), d_out.as<double>(),
-                                     h->stream));
+                                     h->stream, nullptr));
     GRA_HIP_CHECK(h, hipMemcpyAsync(out.data(), d_out.p, sizeof(double) * 2 * (c.n_bt + 1),
                                     hipMemcpyDeviceToHost, h->stream));
     GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
@@ -592,7 +631,7 @@ int build_eikonal(gra_handle* h, const char* cache_dir) {
                                     hipMemcpyHostToDevice, h->stream));
     GRA_HIP_CHECK(h, launch_fb_table(1, d_pre.as<double>(), NF, bmin, STEP, amin, astep,
                                      c.n_kt2 + 1, (2.0 * P.s) * (2.0 * GRA_PI),
-                                     d_out.as<double>(), h->stream));
+                                     d_out.as<double>(), h->stream, nullptr));
     GRA_HIP_CHECK(h, hipMemcpyAsync(out.data(), d_out.p, sizeof(double) * 2 * (c.n_kt2 + 1),
                                     hipMemcpyDeviceToHost, h->stream));
     GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
@@ -605,21 +644,68 @@ int build_eikonal(gra_handle* h, const char* cache_dir) {
     return GRA_OK;
   });
   if (r != GRA_OK) return r;
+  // A_eik at log-spaced kt nodes: the same Fourier-Bessel integral as the MSA
+  // table's rows, evaluated at those kt^2 exactly
+  const int NL = c.eik_log_nodes;
+  std::vector<double> lkt, lre, lim;
+  if (NL > 0) {
+    const double l0 = std::log(c.min_loop_kt), l1 = std::log(c.max_loop_kt);
+    std::vector<double> xs((size_t)NL);
+    lkt.resize(NL);
+    for (int m = 0; m < NL; ++m) {
+      lkt[m] = l0 + (l1 - l0) * m / (NL - 1);
+      const double kt = std::exp(lkt[m]);
+      xs[m] = kt * kt;
+    }
+    DevBuf d_xs;
+    if (d_xs.ensure(sizeof(double) * NL)) return fail(h, GRA_E_NOMEM, "log-node buffer");
+    GRA_HIP_CHECK(h, hipMemcpyAsync(d_pre.p, pre.data(), sizeof(double) * pre.size(),
+                                    hipMemcpyHostToDevice, h->stream));
+    GRA_HIP_CHECK(h, hipMemcpyAsync(d_xs.p, xs.data(), sizeof(double) * NL, hipMemcpyHostToDevice,
+                                    h->stream));
+    GRA_HIP_CHECK(h, launch_fb_table(1, d_pre.as<double>(), NF, bmin, STEP, 0.0, 0.0, NL,
+                                     (2.0 * P.s) * (2.0 * GRA_PI), d_out.as<double>(), h->stream,
+                                     d_xs.as<double>()));
+    std::vector<double> o(2 * (size_t)NL);
+    GRA_HIP_CHECK(h, hipMemcpyAsync(o.data(), d_out.p, sizeof(double) * 2 * NL,
+                                    hipMemcpyDeviceToHost, h->stream));
+    GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    lre.resize(NL);
+    lim.resize(NL);
+    for (int m = 0; m < NL; ++m) {
+      lre[m] = o[2 * m];
+      lim[m] = o[2 * m + 1];
+    }
+  }
+  auto aeik_at = [&](double kt) -> hcx {
+    if (NL == 0) return interp_rows(h->msa, c.n_kt2, amin, astep, kt * kt);
+    const double lk = std::log(kt);
+    const double step = (lkt[NL - 1] - lkt[0]) / (NL - 1);
+    int m = (int)std::floor((lk - lkt[0]) / step);
+    m = std::max(0, std::min(NL - 2, m));
+    const double f = (lk - lkt[m]) / (lkt[m + 1] - lkt[m]);
+    return {lre[m] + f * (lre[m + 1] - lre[m]), lim[m] + f * (lim[m + 1] - lim[m])};
+  };
   d_pre.release();
   d_out.release();
   // Loop nodes (phi_i, kt_j), i-major as the reference's (i, j) loops
   const int NP = c.n_loop_phi, NK = c.n_loop_kt;
+  std::vector<double> gx, gw;
+  if (c.loop_rule == GRA_LOOP_GAUSS) gauss_legendre(NK + 1, gx, gw);
   h->nodes.resize((size_t)(NP + 1) * (NK + 1));
   for (int i = 0; i <= NP; ++i) {
-    const double phi = 0.0 + i * P.step_phi;
+    const bool gauss = c.loop_rule == GRA_LOOP_GAUSS;
+    const double phi = gauss ? 2.0 * GRA_PI * i / (NP + 1) : 0.0 + i * P.step_phi;
     for (int j = 0; j <= NK; ++j) {
-      const double kt = c.min_loop_kt + j * P.step_kt;
-      const hcx A = interp_rows(h->msa, c.n_kt2, amin, astep, kt * kt);
+      const double half = 0.5 * (c.max_loop_kt - c.min_loop_kt);
+      const double kt = gauss ? c.min_loop_kt + half * (1.0 + gx[j]) : c.min_loop_kt + j * P.step_kt;
+      const hcx A = aeik_at(kt);
       NodeConst& nd = h->nodes[(size_t)i * (NK + 1) + j];
       nd.kx = kt * std::cos(phi);
       nd.ky = kt * std::sin(phi);
       nd.kt = kt;
-      nd.w = simpson38_w(i, NP) * simpson38_w(j, NK);
+      nd.w = gauss ? (2.0 * GRA_PI / (NP + 1)) * (half * gw[j])
+                   : simpson38_w(i, NP) * simpson38_w(j, NK);
       nd.aeik_re = A.re;
       nd.aeik_im = A.im;
       nd.cf_re = (float)(nd.w * kt * A.re);
@@ -1132,6 +1218,7 @@ extern "C" int gra_hip_generate(gra_handle* h, const gra_grid* grid, uint64_t n_
   unsigned long long c[6] = {0, 0, 0, 0, 0, 0};
   GRA_HIP_CHECK(h, hipMemcpyAsync(c, d_cnt.p, sizeof(c), hipMemcpyDeviceToHost, h->stream));
   GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  time_collect(h, true, h->P.screening != 0, false);
   r = count_nonfinite(h, c[4], c[5]);
   if (r) return r;
   const uint64_t stored = std::min<uint64_t>(c[2], max_events);
@@ -1151,7 +1238,7 @@ extern "C" int gra_hip_generate(gra_handle* h, const gra_grid* grid, uint64_t n_
   GRA_HIP_CHECK(h, hipMemcpy(d_idx.p, idx.data(), sizeof(uint64_t) * stored,
                              hipMemcpyHostToDevice));
   GRA_HIP_CHECK(h, k64::launch_event_record(h->P, h->d_grid.as<double>(),
-                                            (unsigned int)grid->bins, seed, iter, 0,
+                                            (unsigned int)grid->bins, seed, iter, 0, nullptr,
                                             d_idx.as<uint64_t>(), h->d_W.as<double>(),
                                             h->d_vw.as<double>(), stored, d_ev.p, h->stream));
   GRA_HIP_CHECK(h, hipMemcpyAsync(events, d_ev.p, sizeof(gra_event) * stored,

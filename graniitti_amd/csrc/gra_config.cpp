@@ -263,6 +263,17 @@ extern "C" int gra_hip_cfg_default(gra_cfg* cfg, int32_t preset) {
       cfg->screening = preset == 10;
       cfg->amp_precision = GRA_AMP_FP64;
       break;
+    case 11:
+      // the screened pi+ pi- workload in the loop discretisation stated for it: 32
+      // radial x 16 azimuthal loop nodes, a 64-node log-spaced kt table of A_eik
+      set_pipi(cfg);
+      cfg->screening = 1;
+      cfg->amp_precision = GRA_AMP_FP32;
+      cfg->loop_rule = GRA_LOOP_GAUSS;
+      cfg->n_loop_phi = 15;
+      cfg->n_loop_kt = 31;
+      cfg->eik_log_nodes = 64;
+      break;
     default:
       return set_global_error(GRA_E_ARG, "gra_hip_cfg_default: unknown preset %d", preset);
   }

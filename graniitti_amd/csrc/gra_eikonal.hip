@@ -39,10 +39,11 @@ __device__ __forceinline__ double bessj0(double X) {
 __global__ __launch_bounds__(256) void k_fb_table(int which, const double* __restrict__ pre, int nf,
                                                   double qmin, double qstep, double xmin,
                                                   double xstep, int nrows, double factor,
+                                                  const double* __restrict__ xs,
                                                   double* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nrows) return;
-  const double xv = xmin + i * xstep;
+  const double xv = xs ? xs[i] : xmin + i * xstep;
   const double m = (which == 0) ? xv : msq(xv);
   double f0r = 0, f0i = 0, fNr = 0, fNi = 0, er = 0, ei = 0, odr = 0, odi = 0;
   for (int q = 0; q <= nf; ++q) {
@@ -76,9 +77,9 @@ namespace gra_hip {
 
 hipError_t launch_fb_table(int which, const double* pre, int nf, double qmin, double qstep,
                            double xmin, double xstep, int nrows, double factor, double* out,
-                           hipStream_t st) {
+                           hipStream_t st, const double* xs) {
   hipLaunchKernelGGL(k_fb_table, dim3((nrows + 255) / 256), dim3(256), 0, st, which, pre, nf, qmin,
-                     qstep, xmin, xstep, nrows, factor, out);
+                     qstep, xmin, xstep, nrows, factor, xs, out);
   return hipGetLastError();
 }
 

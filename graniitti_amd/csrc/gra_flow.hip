@@ -1879,32 +1879,57 @@ extern "C" int gra_hip_flow_train(gra_handle* h, int32_t steps, gra_flow_stats* 
   return GRA_OK;
 }
 
-extern "C" int gra_hip_flow_sample(gra_handle* h, uint64_t n_total, uint64_t seed,
-                                   gra_flow_stats* out) {
+namespace gra_hip {
+// Sampling chunk size: a multiple of 256 points (the MLP products tile rows by
+// 64), the training batch by default, at most 2^20
+static int sample_chunk(const FlowState* F) {
+  uint64_t c0 = F->fc.batch > 0 ? F->fc.batch : (1u << 18);
+  return (int)std::min<uint64_t>(std::max<uint64_t>(256, (c0 / 256) * 256), 1u << 20);
+}
+
+// One sampling chunk of pass `pass`: n points (padded to a multiple of 256 in
+// the flow, the padding never weighted) with their flow-stream index starting at
+// `first`, the integrand and the weights w = f vw / q in F->wts, x in F->xx
+static int flow_chunk(gra_handle* h, FlowState* F, int n, uint64_t seed, uint32_t pass,
+                      uint32_t call, uint64_t first, double st[3]) {
+  const int npad = ((n + 255) / 256) * 256;
+  const uint32_t stream = 0x40000000u | ((pass & 0x3FFFu) << 16) | (call & 0xFFFFu);
+  int r = flow_sample_pass(h, F, npad, seed, stream, first, nullptr, false);
+  if (r) return r;
+  return flow_weights(h, F, n, st);
+}
+}  // namespace gra_hip
+
+extern "C" int gra_hip_flow_sample(gra_handle* h, uint64_t n_total, uint64_t seed, double* w,
+                                   double* x, gra_flow_stats* out) {
   FlowState* F = flow_of(h);
   if (!F) return h ? fail(h, GRA_E_STATE, "flow_sample: flow not initialised")
                    : set_global_error(GRA_E_ARG, "handle is NULL");
   if (!out) return fail(h, GRA_E_ARG, "flow_sample: out is NULL");
   if (n_total == 0) return fail(h, GRA_E_ARG, "flow_sample: n = 0");
-  // chunks are multiples of 256 points (the MLP products tile rows by 64); the
-  // last chunk is padded with extra flow points that do not enter the sums
-  uint64_t c0 = F->fc.batch > 0 ? F->fc.batch : (1u << 18);
-  c0 = std::min<uint64_t>(std::max<uint64_t>(256, (c0 / 256) * 256), 1u << 20);
-  const int chunk = (int)c0;
+  const int chunk = sample_chunk(F);
+  if ((n_total + chunk - 1) / chunk > 0xFFFF)
+    return fail(h, GRA_E_ARG, "flow_sample: n = %llu needs more than 65535 chunks of %d",
+                (unsigned long long)n_total, chunk);
   int r = flow_alloc_batch(h, F, chunk);
   if (r) return r;
   const auto t0 = std::chrono::steady_clock::now();
   double acc[3] = {0, 0, 0};
   uint64_t done = 0;
   uint32_t call = 0;
+  const int D = F->D;
   while (done < n_total) {
     const int n = (int)std::min<uint64_t>(chunk, n_total - done);
-    const int npad = ((n + 255) / 256) * 256;
-    r = flow_sample_pass(h, F, npad, seed, 0x40000000u | call, done, nullptr, false);
-    if (r) return r;
     double st[3];
-    r = flow_weights(h, F, n, st);
+    r = flow_chunk(h, F, n, seed, 0, call, done, st);
     if (r) return r;
+    if (w)
+      GRA_HIP_CHECK(h, hipMemcpyAsync(w + done, F->wts.p, sizeof(double) * n,
+                                      hipMemcpyDeviceToHost, h->stream));
+    if (x)
+      GRA_HIP_CHECK(h, hipMemcpyAsync(x + done * D, F->xx.p, sizeof(double) * n * D,
+                                      hipMemcpyDeviceToHost, h->stream));
+    if (w || x) GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
     acc[0] += st[0];
     acc[1] += st[1];
     acc[2] = std::max(acc[2], st[2]);
@@ -1915,6 +1940,153 @@ extern "C" int gra_hip_flow_sample(gra_handle* h, uint64_t n_total, uint64_t see
   fill_stats(out, acc, n_total,
              std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
   out->fallback = (out->ess < F->fc.ess_fallback) ? 1 : 0;
+  return GRA_OK;
+}
+
+extern "C" int gra_hip_flow_integrate(gra_handle* h, const gra_flow_run_cfg* rc, gra_grid* grid,
+                                      gra_flow_result* res) {
+  FlowState* F = flow_of(h);
+  if (!F) return h ? fail(h, GRA_E_STATE, "flow_integrate: flow not initialised")
+                   : set_global_error(GRA_E_ARG, "handle is NULL");
+  if (!rc || !res) return fail(h, GRA_E_ARG, "flow_integrate: NULL argument");
+  if (rc->n_max == 0 || rc->train_steps < 0)
+    return fail(h, GRA_E_ARG, "flow_integrate: n_max >= 1 and train_steps >= 0 required");
+  std::memset(res, 0, sizeof(*res));
+  res->seconds_to_precision = -1.0;
+  const auto t0 = std::chrono::steady_clock::now();
+  auto since = [&t0]() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  };
+  int r = GRA_OK;
+  if (rc->train_steps > 0) {
+    gra_flow_stats st;
+    r = gra_hip_flow_train(h, rc->train_steps, &st);
+    if (r) return r;
+  }
+  res->train_seconds = since();
+  const int chunk = sample_chunk(F);
+  if ((rc->n_max + chunk - 1) / chunk > 0xFFFF)
+    return fail(h, GRA_E_ARG, "flow_integrate: n_max needs more than 65535 chunks");
+  r = flow_alloc_batch(h, F, chunk);
+  if (r) return r;
+  // SampleNeuro's integration (src/MGraniitti.cc:1586-1622): plain mean of
+  // w = f vw / q over the flow's points until the relative error reaches the
+  // requested precision or the budget ends
+  double acc[3] = {0, 0, 0};
+  uint64_t done = 0;
+  uint32_t call = 0;
+  gra_flow_stats fs;
+  std::memset(&fs, 0, sizeof(fs));
+  bool low_ess = false;
+  while (done < rc->n_max) {
+    const int n = (int)std::min<uint64_t>(chunk, rc->n_max - done);
+    double st[3];
+    r = flow_chunk(h, F, n, rc->seed, 0, call++, done, st);
+    if (r) return r;
+    acc[0] += st[0];
+    acc[1] += st[1];
+    acc[2] = std::max(acc[2], st[2]);
+    done += n;
+    fill_stats(&fs, acc, done, 0.0);
+    if (fs.ess < rc->ess_min) {
+      low_ess = true;
+      break;
+    }
+    if (rc->precision > 0 && fs.sigma > 0 && fs.sigma_err / fs.sigma <= rc->precision) {
+      res->seconds_to_precision = since();
+      res->reached = 1;
+      break;
+    }
+  }
+  res->sample_seconds = since() - res->train_seconds;
+  res->sigma = fs.sigma;
+  res->sigma_err = fs.sigma_err;
+  res->cv2 = fs.cv2;
+  res->ess = fs.ess;
+  res->maxw = fs.maxw;
+  res->n = done;
+  if (!low_ess) return GRA_OK;
+  // effective sample size below the threshold: plain VEGAS takes over and its
+  // result is the one reported
+  if (!grid) return fail(h, GRA_E_ARG, "flow_integrate: ESS %.3g < %.3g and no grid for VEGAS",
+                         fs.ess, rc->ess_min);
+  res->fallback = 1;
+  r = gra_hip_vegas_run(h, &rc->vegas, grid, &res->vegas);
+  if (r) return r;
+  res->sigma = res->vegas.sigma;
+  res->sigma_err = res->vegas.sigma_err;
+  res->reached = (rc->precision > 0 && res->vegas.seconds_to_precision >= 0) ? 1 : 0;
+  res->seconds_to_precision =
+      res->reached ? res->train_seconds + res->sample_seconds + res->vegas.seconds_to_precision
+                   : -1.0;
+  return GRA_OK;
+}
+
+extern "C" int gra_hip_flow_generate(gra_handle* h, uint64_t n_calls, uint64_t seed,
+                                     uint32_t pass, double maxw, gra_event* events,
+                                     uint64_t max_events, gra_gen_stats* st) {
+  FlowState* F = flow_of(h);
+  if (!F) return h ? fail(h, GRA_E_STATE, "flow_generate: flow not initialised")
+                   : set_global_error(GRA_E_ARG, "handle is NULL");
+  if (!st) return fail(h, GRA_E_ARG, "flow_generate: st is NULL");
+  if (max_events > 0 && !events) return fail(h, GRA_E_ARG, "flow_generate: events is NULL");
+  if (!(maxw > 0.0)) return fail(h, GRA_E_ARG, "flow_generate: maxw must be > 0");
+  if (n_calls == 0) return fail(h, GRA_E_ARG, "flow_generate: n_calls = 0");
+  if (pass > 0x3FFFu) return fail(h, GRA_E_ARG, "flow_generate: pass %u > 16383", pass);
+  const int chunk = sample_chunk(F);
+  if ((n_calls + chunk - 1) / chunk > 0xFFFF)
+    return fail(h, GRA_E_ARG, "flow_generate: n_calls needs more than 65535 chunks");
+  int r = flow_alloc_batch(h, F, chunk);
+  if (r) return r;
+  DevBuf d_idx, d_cnt, d_ev;
+  if (d_idx.ensure(sizeof(uint64_t) * chunk) || d_cnt.ensure(6 * sizeof(unsigned long long)) ||
+      d_ev.ensure(sizeof(gra_event) * (size_t)chunk))
+    return fail(h, GRA_E_NOMEM, "flow generation buffers");
+  std::memset(st, 0, sizeof(*st));
+  std::vector<uint64_t> idx;
+  uint64_t done = 0;
+  uint32_t call = 0;
+  while (done < n_calls) {
+    const int n = (int)std::min<uint64_t>(chunk, n_calls - done);
+    double s3[3];
+    r = flow_chunk(h, F, n, seed, pass, call++, done, s3);
+    if (r) return r;
+    // SaveEvent's hit-or-miss on w = f vw / q: U < w / maxw, never when w > maxw
+    GRA_HIP_CHECK(h, hipMemsetAsync(d_cnt.p, 0, 6 * sizeof(unsigned long long), h->stream));
+    GRA_HIP_CHECK(h, launch_accept(F->wts.as<double>(), nullptr, h->d_flags.as<uint8_t>(),
+                                   (uint64_t)n, maxw, seed, 0x10000000u | pass, done,
+                                   d_idx.as<uint64_t>(), (uint64_t)n,
+                                   d_cnt.as<unsigned long long>(), h->stream));
+    unsigned long long c[6];
+    GRA_HIP_CHECK(h, hipMemcpyAsync(c, d_cnt.p, sizeof(c), hipMemcpyDeviceToHost, h->stream));
+    GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    r = count_nonfinite(h, c[4], c[5]);
+    if (r) return r;
+    st->trials += c[0];
+    st->valid += c[1];
+    st->accepted += c[2];
+    st->overflow += c[3];
+    const uint64_t room = max_events > st->stored ? max_events - st->stored : 0;
+    const uint64_t take = std::min<uint64_t>(c[2], room);
+    if (take > 0) {
+      // the append order depends on the schedule: sort, keep the smallest indices
+      idx.resize(c[2]);
+      GRA_HIP_CHECK(h, hipMemcpy(idx.data(), d_idx.p, sizeof(uint64_t) * c[2],
+                                 hipMemcpyDeviceToHost));
+      std::sort(idx.begin(), idx.end());
+      GRA_HIP_CHECK(h, hipMemcpy(d_idx.p, idx.data(), sizeof(uint64_t) * take,
+                                 hipMemcpyHostToDevice));
+      GRA_HIP_CHECK(h, k64::launch_event_record(h->P, nullptr, 0, 0, 0, 0, F->xx.as<double>(),
+                                                d_idx.as<uint64_t>(), F->wts.as<double>(), nullptr,
+                                                take, d_ev.p, h->stream));
+      GRA_HIP_CHECK(h, hipMemcpyAsync(events + st->stored, d_ev.p, sizeof(gra_event) * take,
+                                      hipMemcpyDeviceToHost, h->stream));
+      GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
+      for (uint64_t e = 0; e < take; ++e) events[st->stored + e].index += done;
+      st->stored += take;
+    }
+    done += n;
+  }
   return GRA_OK;
 }
 

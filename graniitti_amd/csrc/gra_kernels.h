@@ -20,9 +20,9 @@ namespace gra_hip {
                            const uint32_t* n_cand, uint64_t n_max, double* W, double* amp2,        \
                            uint8_t* flags, hipStream_t st);                                        \
   hipError_t launch_event_record(const RunParams& P, const double* grid, unsigned int bins,         \
-                                 uint64_t seed, uint32_t iter, uint64_t first, const uint64_t* idx, \
-                                 const double* W, const double* vw, uint64_t n_ev, void* out,       \
-                                 hipStream_t st);                                                  \
+                                 uint64_t seed, uint32_t iter, uint64_t first, const double* xin,   \
+                                 const uint64_t* idx, const double* W, const double* vw,            \
+                                 uint64_t n_ev, void* out, hipStream_t st);                         \
   }
 
 GRA_DECLARE_INTEGRAND_LAUNCHERS(k64)
@@ -40,17 +40,22 @@ hipError_t launch_accumulate(const double* W, const double* vw, const uint8_t* b
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
 hipError_t launch_map(const double* grid, int D, unsigned int bins, const double* u, uint64_t n,
                       double* x, double* vw, int32_t* b, hipStream_t st);
+// Hit-or-miss of a generation pass: f = W vw (W alone if vw is NULL) of valid
+// points against maxf, U from Philox (seed, 0xC0000000 | iter, first + i); the
+// accepted indices (pass-local) go to idx; counts: trials, valid, accepted,
+// overflow, NaN weights, Inf weights
 hipError_t launch_accept(const double* W, const double* vw, const uint8_t* flags, uint64_t n,
                          double maxf, uint64_t seed, uint32_t iter, uint64_t first, uint64_t* idx,
                          uint64_t max_out, unsigned long long* counts, hipStream_t st);
 hipError_t launch_unweight(const double* w, uint64_t n, double wmax, uint64_t seed, uint8_t* acc,
                            unsigned long long* counts, hipStream_t st);
 
-// Eikonal Fourier-Bessel tables: rows i = 0..nrows-1 at x_i = xmin + i xstep,
-// out_i = factor * CSIntegral_q[pre_q J0(m(x_i) q_v) q_v], q_v = qmin + q qstep,
-// m(x) = x (density, which = 0) or sqrt(max(x, 0)) (screening, which = 1).
+// Eikonal Fourier-Bessel tables: rows i = 0..nrows-1 at x_i = xmin + i xstep (or
+// x_i = xs[i] when xs is given), out_i = factor * CSIntegral_q[pre_q J0(m(x_i) q_v) q_v],
+// q_v = qmin + q qstep, m(x) = x (density, which = 0) or sqrt(max(x, 0)) (screening,
+// which = 1).
 hipError_t launch_fb_table(int which, const double* pre, int nf, double qmin, double qstep,
                            double xmin, double xstep, int nrows, double factor, double* out,
-                           hipStream_t st);
+                           hipStream_t st, const double* xs);
 
 }  // namespace gra_hip

@@ -271,7 +271,7 @@ __global__ void k_accept(const double* __restrict__ W, const double* __restrict_
     is_inf = !(fl & GRA_FLAG_AMP_OK) && (fl & GRA_FLAG_INF);
     valid = (fl & (GRA_FLAG_KIN_OK | GRA_FLAG_FID_OK | GRA_FLAG_VETO_OK)) ==
             (GRA_FLAG_KIN_OK | GRA_FLAG_FID_OK | GRA_FLAG_VETO_OK);
-    const double f = W[i] * vw[i];
+    const double f = vw ? W[i] * vw[i] : W[i];
     double u;
     point_uniforms(seed, 0xC0000000u | (iter & 0x3FFFFFFFu), first + i, 1, &u);
     hit = u < (f / maxf);

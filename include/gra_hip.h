@@ -74,6 +74,10 @@ typedef struct gra_handle gra_handle; /* opaque: device state of one process */
 #define GRA_FRAME_CS 2 /* Collins-Soper */
 #define GRA_FRAME_GJ 3 /* Gottfried-Jackson */
 
+/* Screening-loop quadratures */
+#define GRA_LOOP_SIMPSON38 0
+#define GRA_LOOP_GAUSS 1
+
 /* Amplitude arithmetic */
 #define GRA_AMP_FP64 0 /* everything FP64, operation order of the reference */
 #define GRA_AMP_FP32 1 /* FP64 kinematics/invariants/Jacobian, FP32 complex amplitude + compensated sums */
@@ -167,6 +171,16 @@ typedef struct gra_cfg {
   int32_t n_kt2;      /* NumberKT2: screening table intervals */
   int32_t n_fb;       /* FBIntegralN: Fourier-Bessel quadrature intervals (even) */
   double min_loop_kt, max_loop_kt;
+  /* Loop quadrature: GRA_LOOP_SIMPSON38, the reference's (n_loop_phi + 1) x
+   * (n_loop_kt + 1) Simpson 3/8 grid (S3ScreenedAmp2, src/MProcess.cc:187-261), or
+   * GRA_LOOP_GAUSS: n_loop_kt + 1 Gauss-Legendre nodes in kt on [MinLoopKT,
+   * MaxLoopKT] times n_loop_phi + 1 equally spaced azimuths (the periodic trapezoid
+   * rule), any node counts. eik_log_nodes = 0 takes A_eik(kt^2) from the MSA table
+   * by linear interpolation (IArray1D::Interpolate1D, src/MEikonal.cc:489-512);
+   * eik_log_nodes = M > 1 evaluates A_eik's Fourier-Bessel integral at M
+   * log-spaced kt on [MinLoopKT, MaxLoopKT] and interpolates linearly in log kt. */
+  int32_t loop_rule;
+  int32_t eik_log_nodes;
   /* Phase-space class, the <C> / <F> suffix of the process string (MProcess
    * classes MContinuum / MFactorized). With GRA_PS_F, rap_min/rap_max bound the
    * central system's rapidity and gen_M_min/gen_M_max its mass (GENCUTS "<F>",
@@ -314,6 +328,9 @@ typedef struct {
  *      (tests/run_JW_frames/run.sh), no screening, FP64
  *   9: PP[CONTENSOR]<C> -> pi+ pi- (tensor pomeron, 2xS), preset 1's cuts, no screening, FP64
  *  10: preset 9 + eikonal screening as preset 2, FP64 amplitude
+ *  11: preset 2 in the stated loop discretisation: 32 Gauss-Legendre kt nodes on
+ *      [1e-4, 3] GeV x 16 azimuths, A_eik at 64 log-spaced kt nodes (the azimuthally
+ *      symmetric content of a 64 x 64 (kt, phi) table), FP32 amplitude
  * Replaces the steering-card reader MGraniitti::ReadInput (src/MGraniitti.cc:318-329)
  * for the keys these workloads use, with TUNE0 values (modeldata/TUNE0/GENERAL.json). */
 int gra_hip_cfg_default(gra_cfg* cfg, int32_t preset);
@@ -488,8 +505,54 @@ int gra_hip_flow_init(gra_handle* h, const gra_flow_cfg* fc);
 int gra_hip_flow_set_grid(gra_handle* h, const gra_grid* grid);
 int gra_hip_flow_train(gra_handle* h, int32_t steps, gra_flow_stats* last);
 /* Sample n points from the flow, evaluate the integrand and report sigma, CV^2,
- * ESS; W (nullable, device size n) receives w = f/q. */
-int gra_hip_flow_sample(gra_handle* h, uint64_t n, uint64_t seed, gra_flow_stats* st);
+ * ESS (the per-point loop of SampleNeuro, src/MGraniitti.cc:1586-1622). Host
+ * arrays, nullable: w (n) receives each point's weight w_i = f(x_i) vw_i / q_i in
+ * barn, x (n x dim, row-major) the point x_i = grid(u_i) the integrand was
+ * evaluated at. st->fallback is set when the ESS is below the flow's ess_fallback. */
+int gra_hip_flow_sample(gra_handle* h, uint64_t n, uint64_t seed, double* w, double* x,
+                        gra_flow_stats* st);
+
+/* The NEURO integrator run (MGraniitti::SampleNeuro integration, src/MGraniitti.cc:
+ * 1527-1648) with plain VEGAS as fallback and judge. */
+typedef struct gra_flow_run_cfg {
+  int32_t train_steps; /* Adam steps before sampling (0: sample the flow as it is) */
+  int32_t pad0;
+  uint64_t n_max;      /* sampling budget in points */
+  uint64_t seed;       /* sampling stream */
+  double precision;    /* stop when sigma_err / sigma <= precision (0: use the budget) */
+  double ess_min;      /* VEGAS takes over when the running ESS falls below this */
+  gra_vegas_cfg vegas; /* the fallback VEGAS run */
+} gra_flow_run_cfg;
+
+typedef struct gra_flow_result {
+  double sigma, sigma_err;     /* barn: the flow's plain mean, or VEGAS's after a fallback */
+  double cv2, ess, maxw;       /* of the flow's sample (maxw: the largest w) */
+  double train_seconds;        /* training part of the call */
+  double sample_seconds;       /* flow sampling part of the call */
+  double seconds_to_precision; /* from the call's start, training included; -1 if not reached */
+  uint64_t n;                  /* flow points sampled */
+  int32_t fallback;            /* 1: ESS < ess_min, sigma and sigma_err come from VEGAS */
+  int32_t reached;             /* 1: sigma_err / sigma <= precision */
+  gra_vegas_result vegas;      /* the fallback run (zero without one) */
+} gra_flow_result;
+
+/* Train (rc->train_steps), then sample in chunks until the precision or the
+ * budget is reached; if the running effective sample size (sum w)^2 / (N sum w^2)
+ * drops below rc->ess_min, run gra_hip_vegas_run(rc->vegas) on grid (in/out) and
+ * report its result with fallback = 1. */
+int gra_hip_flow_integrate(gra_handle* h, const gra_flow_run_cfg* rc, gra_grid* grid,
+                           gra_flow_result* res);
+
+/* Unweighted events from the flow (the generation mode of SampleNeuro with
+ * MGraniitti::SaveEvent, src/MGraniitti.cc:1624-1632, 1651-1726): n_calls flow
+ * points of generation pass `pass` (pass 0 draws the points gra_hip_flow_sample
+ * draws with the same seed), each valid point accepted when U < w / maxw (U from
+ * Philox, its own stream) and never when w > maxw (overflow). The first
+ * max_events accepted points, in increasing point index, are written to events
+ * (host memory) with f = w and index = the point index. */
+int gra_hip_flow_generate(gra_handle* h, uint64_t n_calls, uint64_t seed, uint32_t pass,
+                          double maxw, gra_event* events, uint64_t max_events,
+                          gra_gen_stats* st);
 
 /* Parameter access for tests: host copy in/out of all flow parameters (FP32,
  * packed per layer: W1,b1,...,Wout,bout) and of their gradients. */

@@ -343,7 +343,35 @@ typedef struct {
   int tables_ready;
   /* <F>: system mass range, src/MFactorized.cc:259-285 */
   double f_Mmin, f_Mmax;
+  /* Gauss loop rule: radial nodes and weights on [MinLoopKT, MaxLoopKT]; the
+   * screening amplitude at log-spaced kt nodes (eik_log_nodes > 0) */
+  double *g_kt, *g_w;
+  double* l_lkt; /* log kt of the log-spaced nodes */
+  cx* l_amp;
 } ora_run;
+
+/* Gauss-Legendre nodes x and weights w on [-1, 1]: roots of P_n by Newton's
+ * iteration from cos(pi (i + 3/4) / (n + 1/2)), w = 2 / ((1 - x^2) P_n'(x)^2) */
+ORA_EXPORT void ora_gauss_legendre(int n, double* x, double* w) {
+  for (int i = 0; i < (n + 1) / 2; ++i) {
+    double z = cos(PI * (i + 0.75) / (n + 0.5)), dp = 0.0;
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, pm = 0.0;
+      for (int k = 1; k <= n; ++k) {
+        const double pk = ((2.0 * k - 1.0) * z * p0 - (k - 1.0) * pm) / k;
+        pm = p0;
+        p0 = pk;
+      }
+      dp = n * (z * p0 - pm) / (z * z - 1.0);
+      const double dz = p0 / dp;
+      z -= dz;
+      if (fabs(dz) < 1e-15) break;
+    }
+    x[i] = -z;
+    x[n - 1 - i] = z;
+    w[i] = w[n - 1 - i] = 2.0 / ((1.0 - z * z) * dp * dp);
+  }
+}
 
 /* MParticle masses and units, include/Graniitti/MPDG.h:54-78 */
 static double gev2barn(void) {
@@ -1240,8 +1268,37 @@ static void build_tables(ora_run* R, int nthreads) {
     R->msa_x[i] = round15(R->msa_min + i * R->msa_step);
     R->msa_y[i] = cx_make(round15(R->msa_y[i].re), round15(R->msa_y[i].im));
   }
+  /* A_eik at log-spaced kt: the S3Screening integral at those kt^2 */
+  const int NL = R->c.eik_log_nodes;
+  if (NL > 0) {
+    const double STEP = (R->mbt_max - R->mbt_min) / NF;
+    const double l0 = log(R->c.min_loop_kt), l1 = log(R->c.max_loop_kt);
+    cx* f = (cx*)malloc(sizeof(cx) * (NF + 1));
+    for (int m = 0; m < NL; ++m) {
+      R->l_lkt[m] = l0 + (l1 - l0) * m / (NL - 1);
+      const double e = exp(R->l_lkt[m]);
+      const double kt = msqrt(e * e); /* as S3Screening takes sqrt(kt^2) */
+      for (int q = 0; q <= NF; ++q) {
+        const double bt = R->mbt_min + q * STEP;
+        f[q] = cx_scale(cx_scale(pre[q], ora_bessj0(bt * kt)), bt);
+      }
+      R->l_amp[m] = cx_scale(cs_integral(f, NF, STEP), (2.0 * R->s) * (2.0 * PI));
+    }
+    free(f);
+  }
   free(pre);
   R->tables_ready = 1;
+}
+
+/* A_eik(kt) interpolated linearly in log kt between the log-spaced nodes */
+static cx log_node_amp(const ora_run* R, double kt) {
+  const int NL = R->c.eik_log_nodes;
+  const double lk = log(kt), step = (R->l_lkt[NL - 1] - R->l_lkt[0]) / (NL - 1);
+  int m = (int)floor((lk - R->l_lkt[0]) / step);
+  if (m < 0) m = 0;
+  if (m > NL - 2) m = NL - 2;
+  const double f = (lk - R->l_lkt[m]) / (R->l_lkt[m + 1] - R->l_lkt[m]);
+  return cx_add(R->l_amp[m], cx_scale(cx_sub(R->l_amp[m + 1], R->l_amp[m]), f));
 }
 
 /* ------------------------------------------------------------------------- */
@@ -1267,21 +1324,26 @@ static double screened_amp2(ora_run* R, kin_t* K) {
   for (int i = 0; i < 5; ++i) K->pf_orig[i] = K->pf[i];
   const int NP = R->nphi, NK = R->nkt;
   const double StepKT = R->step_kt, StepPhi = R->step_phi;
+  const int gauss = R->c.loop_rule == GRA_LOOP_GAUSS;
   cx I[MAX_HAMP];
   for (int q = 0; q < nh; ++q) I[q] = cx_make(0, 0);
   /* The reference stores f[h][i][j] and sums W[i][j] * f[i][j] in (i, j) order;
-   * accumulating in the same order gives the same sums. */
+   * accumulating in the same order gives the same sums. The Gauss rule: radial
+   * Gauss-Legendre nodes, equally spaced azimuths with weight 2 pi / (NP + 1). */
   for (int i = 0; i <= NP; ++i) {
-    const double phi = 0.0 + i * StepPhi;
+    const double phi = gauss ? 2.0 * PI * i / (NP + 1) : 0.0 + i * StepPhi;
     for (int j = 0; j <= NK; ++j) {
-      const double kt = R->c.min_loop_kt + j * StepKT;
+      const double kt = gauss ? R->g_kt[j] : R->c.min_loop_kt + j * StepKT;
       const double kt2 = kt * kt;
       const cx A_eik =
-          interp1d(R->msa_x, R->msa_y, R->n_kt2, R->msa_min, R->msa_max, R->msa_step, kt2);
+          R->c.eik_log_nodes > 0
+              ? log_node_amp(R, kt)
+              : interp1d(R->msa_x, R->msa_y, R->n_kt2, R->msa_min, R->msa_max, R->msa_step, kt2);
       const double ktx = kt * cos(phi), kty = kt * sin(phi);
       const double p1p[2] = {p1T[0] - ktx, p1T[1] - kty};
       const double p2p[2] = {p2T[0] + ktx, p2T[1] + kty};
-      const double w = simpson38_w(i, NP) * simpson38_w(j, NK);
+      const double w = gauss ? (2.0 * PI / (NP + 1)) * R->g_w[j]
+                             : simpson38_w(i, NP) * simpson38_w(j, NK);
       if (!loop_kinematics(R, K, p1p, p2p)) {
         continue; /* f[i][j] = 0 still enters the sum as W * 0 */
       }
@@ -1293,7 +1355,7 @@ static double screened_amp2(ora_run* R, kin_t* K) {
     }
   }
   const double nrm = 1.0 / (8.0 * (PI * PI) * R->s);
-  const double hfac = 9 * (StepPhi * StepKT) / 64;
+  const double hfac = gauss ? 1.0 : 9 * (StepPhi * StepKT) / 64;
   loop_kinematics(R, K, p1T, p2T); /* back to tree level */
   double amp2 = 0.0;
   for (int q = 0; q < nh; ++q) {
@@ -1409,6 +1471,21 @@ ORA_EXPORT ora_run* ora_create(const gra_cfg* cfg, int nthreads) {
     R->msa_x = (double*)malloc(sizeof(double) * (R->n_kt2 + 1));
     R->mbt_y = (cx*)malloc(sizeof(cx) * (R->n_bt + 1));
     R->msa_y = (cx*)malloc(sizeof(cx) * (R->n_kt2 + 1));
+    if (cfg->eik_log_nodes > 0) {
+      R->l_lkt = (double*)malloc(sizeof(double) * cfg->eik_log_nodes);
+      R->l_amp = (cx*)malloc(sizeof(cx) * cfg->eik_log_nodes);
+    }
+    if (cfg->loop_rule == GRA_LOOP_GAUSS) {
+      const int n = cfg->n_loop_kt + 1;
+      const double half = 0.5 * (cfg->max_loop_kt - cfg->min_loop_kt);
+      R->g_kt = (double*)malloc(sizeof(double) * n);
+      R->g_w = (double*)malloc(sizeof(double) * n);
+      ora_gauss_legendre(n, R->g_kt, R->g_w);
+      for (int j = 0; j < n; ++j) {
+        R->g_kt[j] = cfg->min_loop_kt + half * (1.0 + R->g_kt[j]);
+        R->g_w[j] *= half;
+      }
+    }
     build_tables(R, nthreads);
   }
   return R;
@@ -1420,7 +1497,18 @@ ORA_EXPORT void ora_destroy(ora_run* R) {
   free(R->msa_x);
   free(R->mbt_y);
   free(R->msa_y);
+  free(R->g_kt);
+  free(R->g_w);
+  free(R->l_lkt);
+  free(R->l_amp);
   free(R);
+}
+
+/* A_eik at kt from the log-spaced nodes (test access; eik_log_nodes > 0) */
+ORA_EXPORT void ora_log_node_amp(const ora_run* R, double kt, double* out) {
+  const cx a = log_node_amp(R, kt);
+  out[0] = a.re;
+  out[1] = a.im;
 }
 
 /* Table access: rows of (x, re, im) */
@@ -2065,6 +2153,16 @@ ORA_EXPORT int ora_tensor_amplitudes(ora_run* R, const double* p, double* hamp_r
     hamp_reim[2 * k + 1] = h[k].im;
   }
   return 9;
+}
+
+/* form::F1 with the Kelly fits at Q2 (test access) */
+ORA_EXPORT double ora_kelly_f1(double Q2, double mp) { return (double)kelly_f1(Q2, mp); }
+
+/* form::CBW of resonance q at m2 with decay masses mA, mB: out = (re, im) */
+ORA_EXPORT void ora_cbw(const gra_res* q, double m2, double mA, double mB, double* out) {
+  const cx z = o_cbw(q, m2, mA, mB);
+  out[0] = z.re;
+  out[1] = z.im;
 }
 
 /* The 2xS amplitude -i (M_t + M_u) (one of the four identical replicas) at caller

@@ -49,9 +49,19 @@ def layout(D, L, K, H, depth):
     return lay, off, ldp
 
 
+def bf16(x):
+    """Round to bfloat16 (round to nearest even) and back, as the BF16 MLP path
+    stores its operands."""
+    return x.to(torch.bfloat16).to(x.dtype)
+
+
 class FlowRef:
-    def __init__(self, theta, D, L, K, H, depth):
+    """bf16=True rounds what the BF16 MFMA path rounds: the hidden and output
+    weights, and every hidden activation after its ReLU; sums stay in float64."""
+
+    def __init__(self, theta, D, L, K, H, depth, bf16=False):
         self.D, self.L, self.K, self.H, self.depth = D, L, K, H, depth
+        self.bf = bf16
         self.lay, self.n, self.ldp = layout(D, L, K, H, depth)
         assert len(theta) == self.n
         self.theta = torch.tensor(np.asarray(theta, dtype=np.float64), requires_grad=True)
@@ -63,10 +73,11 @@ class FlowRef:
 
     def mlp(self, l, uA):
         d = self.lay[l]
-        h = torch.relu(uA @ self._blk(d["W1"]) + self._blk(d["b1"]))
+        r = bf16 if self.bf else (lambda t: t)
+        h = r(torch.relu(uA @ self._blk(d["W1"]) + self._blk(d["b1"])))
         for W, b in d["Wh"]:
-            h = torch.relu(h @ self._blk(W) + self._blk(b))
-        return h @ self._blk(d["Wo"]) + self._blk(d["bo"])
+            h = r(torch.relu(h @ r(self._blk(W)) + self._blk(b)))
+        return h @ r(self._blk(d["Wo"])) + self._blk(d["bo"])
 
     def _knots(self, p):
         K = self.K
