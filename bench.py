@@ -111,17 +111,19 @@ def cpu_baseline(cfg, grid, eff, n_sample):
 
 
 def vegas_generate(P, G, n, seed, first_iter, passes, maxf):
-    """Generation passes on the adapted grid: (events produced, trials, seconds, stats)"""
-    buf = (abi.GraEvent * EVENT_CAP)()
+    """Generation passes on the adapted grid, events into pinned host memory:
+    (counters summed over the passes, seconds)"""
     tot = {"trials": 0, "valid": 0, "accepted": 0, "overflow": 0, "stored": 0}
-    t0 = time.perf_counter()
-    for k in range(passes):
-        st = abi.GraGenStats()
-        abi.check(P.lib.gra_hip_generate(P.h, C.byref(G.grid), n, seed, first_iter + k, maxf,
-                                         buf, EVENT_CAP, C.byref(st)), P.h)
-        for key in tot:
-            tot[key] += getattr(st, key)
-    return tot, time.perf_counter() - t0
+    with abi.PinnedEvents(EVENT_CAP) as ev:
+        t0 = time.perf_counter()
+        for k in range(passes):
+            st = abi.GraGenStats()
+            abi.check(P.lib.gra_hip_generate(P.h, C.byref(G.grid), n, seed, first_iter + k, maxf,
+                                             ev.buf, EVENT_CAP, C.byref(st)), P.h)
+            for key in tot:
+                tot[key] += getattr(st, key)
+        dt = time.perf_counter() - t0
+    return tot, dt
 
 
 def flow_cfg(dim, layers, bins, hidden, depth, batch, seed=2024):
@@ -159,16 +161,16 @@ def flow_leg(P, grid, fc, steps, n_pass, vegas, gen_passes):
     abi.check(lib.gra_hip_flow_sample(P.h, MAXF_SAMPLE * n_pass, 2000, None, None, C.byref(sm)),
               P.h)
     maxw = sm.maxw
-    buf = (abi.GraEvent * EVENT_CAP)()
     tot = {"trials": 0, "valid": 0, "accepted": 0, "overflow": 0, "stored": 0}
-    t0 = time.perf_counter()
-    for k in range(gen_passes):
-        st = abi.GraGenStats()
-        abi.check(lib.gra_hip_flow_generate(P.h, n_pass, 3000, k, maxw, buf, EVENT_CAP,
-                                            C.byref(st)), P.h)
-        for key in tot:
-            tot[key] += getattr(st, key)
-    t_gen = time.perf_counter() - t0
+    with abi.PinnedEvents(EVENT_CAP) as ev:
+        t0 = time.perf_counter()
+        for k in range(gen_passes):
+            st = abi.GraGenStats()
+            abi.check(lib.gra_hip_flow_generate(P.h, n_pass, 3000, k, maxw, ev.buf, EVENT_CAP,
+                                                C.byref(st)), P.h)
+            for key in tot:
+                tot[key] += getattr(st, key)
+        t_gen = time.perf_counter() - t0
     comb = np.hypot(res.sigma_err, vegas["sigma_err"])
     out = {
         "layers": fc.layers, "bins": fc.bins, "hidden": f"{fc.depth}x{fc.hidden}",

@@ -172,6 +172,8 @@ SIGNATURES = {
     "gra_hip_generate": (C.c_int, [VP, P(GraGrid), U64, U64, C.c_uint32, D, P(GraEvent), U64,
                                    P(GraGenStats)]),
     "gra_hip_unweight": (C.c_int, [VP, P(D), C.c_size_t, D, U64, P(C.c_uint8), P(U64), P(U64)]),
+    "gra_hip_host_alloc": (C.c_int, [C.c_size_t, P(VP)]),
+    "gra_hip_host_free": (None, [VP]),
     "gra_hip_eikonal_tables": (C.c_int, [VP, P(D), P(D), P(D)]),
     "gra_hip_flow_init": (C.c_int, [VP, P(GraFlowCfg)]),
     "gra_hip_flow_set_grid": (C.c_int, [VP, P(GraGrid)]),
@@ -246,3 +248,31 @@ def grid_xmat(grid):
     """(dim, bins) view of a GraGrid's edges."""
     a = np.ctypeslib.as_array(grid.xmat).reshape(GRA_MAX_DIM, GRA_VEGAS_BINS)
     return a[: grid.dim, : grid.bins]
+
+
+class PinnedEvents:
+    """An array of n gra_event records in pinned host memory (gra_hip_host_alloc),
+    the buffer the generation calls copy into at DMA rate. Use as a context
+    manager or call close(); .buf is the ctypes array."""
+
+    def __init__(self, n):
+        lib = load()
+        p = C.c_void_p()
+        check(lib.gra_hip_host_alloc(C.sizeof(GraEvent) * n, C.byref(p)))
+        self._lib, self._p, self.n = lib, p, n
+        self.buf = (GraEvent * n).from_address(p.value)
+
+    def close(self):
+        if self._p:
+            self.buf = None
+            self._lib.gra_hip_host_free(self._p)
+            self._p = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        self.close()

@@ -753,6 +753,49 @@ int ensure_point_buffers(gra_handle* h, uint64_t n) {
   return GRA_OK;
 }
 
+int ensure_gen_buffers(gra_handle* h, uint64_t n, uint64_t cap) {
+  if (h->d_gen_idx.ensure(sizeof(uint64_t) * std::max<uint64_t>(cap, 1)) ||
+      h->d_gen_mask.ensure(sizeof(uint64_t) * accept_mask_words(n)) ||
+      h->d_gen_tile.ensure(sizeof(uint32_t) * accept_tile_words(n)) ||
+      h->d_gen_cnt.ensure(6 * sizeof(unsigned long long)) ||
+      h->d_gen_ev.ensure(sizeof(gra_event) * std::max<uint64_t>(cap, 1)))
+    return fail(h, GRA_E_NOMEM, "generation buffers for %llu points", (unsigned long long)n);
+  if (!h->h_gen_cnt &&
+      hipHostMalloc((void**)&h->h_gen_cnt, 6 * sizeof(unsigned long long), hipHostMallocDefault) !=
+          hipSuccess) {
+    h->h_gen_cnt = nullptr;
+    return fail(h, GRA_E_NOMEM, "pinned generation counters");
+  }
+  return GRA_OK;
+}
+
+int events_to_host(gra_handle* h, gra_event* dst, const void* d_src, uint64_t n) {
+  const size_t bytes = sizeof(gra_event) * n;
+  if (bytes == 0) return GRA_OK;
+  hipPointerAttribute_t a;
+  const bool pinned = hipPointerGetAttributes(&a, dst) == hipSuccess && a.type == hipMemoryTypeHost;
+  (void)hipGetLastError();  // an unregistered pointer leaves an error behind
+  if (pinned) {
+    GRA_HIP_CHECK(h, hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, h->stream));
+    GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    return GRA_OK;
+  }
+  if (bytes > h->h_ev_stage_bytes) {
+    if (h->h_ev_stage) (void)hipHostFree(h->h_ev_stage);
+    h->h_ev_stage = nullptr;
+    h->h_ev_stage_bytes = 0;
+    if (hipHostMalloc(&h->h_ev_stage, bytes, hipHostMallocDefault) != hipSuccess) {
+      h->h_ev_stage = nullptr;
+      return fail(h, GRA_E_NOMEM, "pinned event staging (%zu bytes)", bytes);
+    }
+    h->h_ev_stage_bytes = bytes;
+  }
+  GRA_HIP_CHECK(h, hipMemcpyAsync(h->h_ev_stage, d_src, bytes, hipMemcpyDeviceToHost, h->stream));
+  GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  std::memcpy(dst, h->h_ev_stage, bytes);
+  return GRA_OK;
+}
+
 static void time_begin(gra_handle* h, int slot) {
   (void)hipEventRecord(h->ev[2 * slot], h->stream);
 }
@@ -956,10 +999,14 @@ extern "C" void gra_hip_destroy(gra_handle* h) {
   if (!h) return;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   if (h->flow) flow_release(h->flow);
-  DevBuf* bufs[] = {&h->d_nodes, &h->d_res, &h->d_x,    &h->d_W,         &h->d_amp2,      &h->d_flags,
-                    &h->d_vw,    &h->d_bins, &h->d_rec,       &h->d_ncand,     &h->d_grid,
-                    &h->d_part_hist, &h->d_part_scal, &h->d_stats, &h->d_aux};
+  DevBuf* bufs[] = {&h->d_nodes,     &h->d_res,       &h->d_x,         &h->d_W,
+                    &h->d_amp2,      &h->d_flags,     &h->d_vw,        &h->d_bins,
+                    &h->d_rec,       &h->d_ncand,     &h->d_grid,      &h->d_part_hist,
+                    &h->d_part_scal, &h->d_stats,     &h->d_aux,       &h->d_gen_idx,
+                    &h->d_gen_mask,  &h->d_gen_tile,  &h->d_gen_cnt,   &h->d_gen_ev};
   for (DevBuf* b : bufs) b->release();
+  if (h->h_gen_cnt) (void)hipHostFree(h->h_gen_cnt);
+  if (h->h_ev_stage) (void)hipHostFree(h->h_ev_stage);
   for (int k = 0; k < 2 * TS_N; ++k)
     if (h->ev[k]) (void)hipEventDestroy(h->ev[k]);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -1196,13 +1243,12 @@ extern "C" int gra_hip_generate(gra_handle* h, const gra_grid* grid, uint64_t n_
   const int D = h->P.dim;
   r = ensure_point_buffers(h, n_calls);
   if (r) return r;
-  // every accepted index is kept (at most n_calls), so that the events written
-  // are always the accepted points of smallest index, independent of the schedule
-  const uint64_t cap = std::max<uint64_t>(1, std::min<uint64_t>(max_events, n_calls));
-  DevBuf d_idx, d_cnt, d_ev;
-  if (h->d_grid.ensure(sizeof(double) * GRA_MAX_DIM * GRA_VEGAS_BINS) ||
-      d_idx.ensure(sizeof(uint64_t) * n_calls) || d_cnt.ensure(6 * sizeof(unsigned long long)) ||
-      d_ev.ensure(sizeof(gra_event) * cap))
+  // the accepted indices come out in ascending order (ordered compaction), so
+  // the events written are the accepted points of smallest index
+  const uint64_t cap = std::min<uint64_t>(max_events, n_calls);
+  r = ensure_gen_buffers(h, n_calls, cap);
+  if (r) return r;
+  if (h->d_grid.ensure(sizeof(double) * GRA_MAX_DIM * GRA_VEGAS_BINS))
     return fail(h, GRA_E_NOMEM, "generation buffers");
   GRA_HIP_CHECK(h, hipMemcpyAsync(h->d_grid.p, grid->xmat, sizeof(double) * D * GRA_VEGAS_BINS,
                                   hipMemcpyHostToDevice, h->stream));
@@ -1210,18 +1256,19 @@ extern "C" int gra_hip_generate(gra_handle* h, const gra_grid* grid, uint64_t n_
                     n_calls, h->d_W.as<double>(), nullptr, h->d_flags.as<uint8_t>(),
                     h->d_vw.as<double>(), h->d_bins.as<uint8_t>());
   if (r) return r;
-  GRA_HIP_CHECK(h, hipMemsetAsync(d_cnt.p, 0, 6 * sizeof(unsigned long long), h->stream));
+  unsigned long long* d_cnt = h->d_gen_cnt.as<unsigned long long>();
   GRA_HIP_CHECK(h, launch_accept(h->d_W.as<double>(), h->d_vw.as<double>(),
                                  h->d_flags.as<uint8_t>(), n_calls, maxf, seed, iter, 0,
-                                 d_idx.as<uint64_t>(), n_calls, d_cnt.as<unsigned long long>(),
-                                 h->stream));
-  unsigned long long c[6] = {0, 0, 0, 0, 0, 0};
-  GRA_HIP_CHECK(h, hipMemcpyAsync(c, d_cnt.p, sizeof(c), hipMemcpyDeviceToHost, h->stream));
+                                 h->d_gen_mask.as<uint64_t>(), h->d_gen_tile.as<uint32_t>(),
+                                 h->d_gen_idx.as<uint64_t>(), cap, d_cnt, h->stream));
+  unsigned long long* c = h->h_gen_cnt;
+  GRA_HIP_CHECK(h, hipMemcpyAsync(c, d_cnt, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                  h->stream));
   GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
   time_collect(h, true, h->P.screening != 0, false);
   r = count_nonfinite(h, c[4], c[5]);
   if (r) return r;
-  const uint64_t stored = std::min<uint64_t>(c[2], max_events);
+  const uint64_t stored = std::min<uint64_t>(c[2], cap);
   std::memset(st, 0, sizeof(*st));
   st->trials = c[0];
   st->valid = c[1];
@@ -1229,22 +1276,27 @@ extern "C" int gra_hip_generate(gra_handle* h, const gra_grid* grid, uint64_t n_
   st->overflow = c[3];
   st->stored = stored;
   if (stored == 0) return GRA_OK;
-  // the append order depends on the schedule: sort the accepted indices and keep
-  // the first max_events
-  std::vector<uint64_t> idx(c[2]);
-  GRA_HIP_CHECK(h, hipMemcpy(idx.data(), d_idx.p, sizeof(uint64_t) * c[2],
-                             hipMemcpyDeviceToHost));
-  std::sort(idx.begin(), idx.end());
-  GRA_HIP_CHECK(h, hipMemcpy(d_idx.p, idx.data(), sizeof(uint64_t) * stored,
-                             hipMemcpyHostToDevice));
   GRA_HIP_CHECK(h, k64::launch_event_record(h->P, h->d_grid.as<double>(),
                                             (unsigned int)grid->bins, seed, iter, 0, nullptr,
-                                            d_idx.as<uint64_t>(), h->d_W.as<double>(),
-                                            h->d_vw.as<double>(), stored, d_ev.p, h->stream));
-  GRA_HIP_CHECK(h, hipMemcpyAsync(events, d_ev.p, sizeof(gra_event) * stored,
-                                  hipMemcpyDeviceToHost, h->stream));
-  GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
+                                            h->d_gen_idx.as<uint64_t>(), h->d_W.as<double>(),
+                                            h->d_vw.as<double>(), stored, h->d_gen_ev.p,
+                                            h->stream));
+  return events_to_host(h, events, h->d_gen_ev.p, stored);
+}
+
+extern "C" int gra_hip_host_alloc(size_t bytes, void** p) {
+  if (!p) return set_global_error(GRA_E_ARG, "host_alloc: p is NULL");
+  *p = nullptr;
+  if (bytes == 0) return set_global_error(GRA_E_ARG, "host_alloc: 0 bytes");
+  if (hipHostMalloc(p, bytes, hipHostMallocDefault) != hipSuccess) {
+    *p = nullptr;
+    return set_global_error(GRA_E_NOMEM, "host_alloc: %zu bytes of pinned memory", bytes);
+  }
   return GRA_OK;
+}
+
+extern "C" void gra_hip_host_free(void* p) {
+  if (p) (void)hipHostFree(p);
 }
 
 extern "C" int gra_hip_unweight(gra_handle* h, const double* w, size_t n, double wmax,

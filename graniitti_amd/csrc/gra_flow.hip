@@ -2038,12 +2038,11 @@ extern "C" int gra_hip_flow_generate(gra_handle* h, uint64_t n_calls, uint64_t s
     return fail(h, GRA_E_ARG, "flow_generate: n_calls needs more than 65535 chunks");
   int r = flow_alloc_batch(h, F, chunk);
   if (r) return r;
-  DevBuf d_idx, d_cnt, d_ev;
-  if (d_idx.ensure(sizeof(uint64_t) * chunk) || d_cnt.ensure(6 * sizeof(unsigned long long)) ||
-      d_ev.ensure(sizeof(gra_event) * (size_t)chunk))
-    return fail(h, GRA_E_NOMEM, "flow generation buffers");
+  r = ensure_gen_buffers(h, (uint64_t)chunk, (uint64_t)chunk);
+  if (r) return r;
+  unsigned long long* d_cnt = h->d_gen_cnt.as<unsigned long long>();
+  unsigned long long* c = h->h_gen_cnt;
   std::memset(st, 0, sizeof(*st));
-  std::vector<uint64_t> idx;
   uint64_t done = 0;
   uint32_t call = 0;
   while (done < n_calls) {
@@ -2051,14 +2050,16 @@ extern "C" int gra_hip_flow_generate(gra_handle* h, uint64_t n_calls, uint64_t s
     double s3[3];
     r = flow_chunk(h, F, n, seed, pass, call++, done, s3);
     if (r) return r;
-    // SaveEvent's hit-or-miss on w = f vw / q: U < w / maxw, never when w > maxw
-    GRA_HIP_CHECK(h, hipMemsetAsync(d_cnt.p, 0, 6 * sizeof(unsigned long long), h->stream));
-    GRA_HIP_CHECK(h, launch_accept(F->wts.as<double>(), nullptr, h->d_flags.as<uint8_t>(),
+    // SaveEvent's hit-or-miss on w = f vw / q: U < w / maxw, never when w > maxw;
+    // the accepted indices come out in ascending order
+    const uint64_t room = max_events > st->stored ? max_events - st->stored : 0;
+      GRA_HIP_CHECK(h, launch_accept(F->wts.as<double>(), nullptr, h->d_flags.as<uint8_t>(),
                                    (uint64_t)n, maxw, seed, 0x10000000u | pass, done,
-                                   d_idx.as<uint64_t>(), (uint64_t)n,
-                                   d_cnt.as<unsigned long long>(), h->stream));
-    unsigned long long c[6];
-    GRA_HIP_CHECK(h, hipMemcpyAsync(c, d_cnt.p, sizeof(c), hipMemcpyDeviceToHost, h->stream));
+                                   h->d_gen_mask.as<uint64_t>(), h->d_gen_tile.as<uint32_t>(),
+                                   h->d_gen_idx.as<uint64_t>(), std::min<uint64_t>(room, n),
+                                   d_cnt, h->stream));
+    GRA_HIP_CHECK(h, hipMemcpyAsync(c, d_cnt, 6 * sizeof(unsigned long long),
+                                    hipMemcpyDeviceToHost, h->stream));
     GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
     r = count_nonfinite(h, c[4], c[5]);
     if (r) return r;
@@ -2066,22 +2067,13 @@ extern "C" int gra_hip_flow_generate(gra_handle* h, uint64_t n_calls, uint64_t s
     st->valid += c[1];
     st->accepted += c[2];
     st->overflow += c[3];
-    const uint64_t room = max_events > st->stored ? max_events - st->stored : 0;
     const uint64_t take = std::min<uint64_t>(c[2], room);
     if (take > 0) {
-      // the append order depends on the schedule: sort, keep the smallest indices
-      idx.resize(c[2]);
-      GRA_HIP_CHECK(h, hipMemcpy(idx.data(), d_idx.p, sizeof(uint64_t) * c[2],
-                                 hipMemcpyDeviceToHost));
-      std::sort(idx.begin(), idx.end());
-      GRA_HIP_CHECK(h, hipMemcpy(d_idx.p, idx.data(), sizeof(uint64_t) * take,
-                                 hipMemcpyHostToDevice));
       GRA_HIP_CHECK(h, k64::launch_event_record(h->P, nullptr, 0, 0, 0, 0, F->xx.as<double>(),
-                                                d_idx.as<uint64_t>(), F->wts.as<double>(), nullptr,
-                                                take, d_ev.p, h->stream));
-      GRA_HIP_CHECK(h, hipMemcpyAsync(events + st->stored, d_ev.p, sizeof(gra_event) * take,
-                                      hipMemcpyDeviceToHost, h->stream));
-      GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
+                                                h->d_gen_idx.as<uint64_t>(), F->wts.as<double>(),
+                                                nullptr, take, h->d_gen_ev.p, h->stream));
+      r = events_to_host(h, events + st->stored, h->d_gen_ev.p, take);
+      if (r) return r;
       for (uint64_t e = 0; e < take; ++e) events[st->stored + e].index += done;
       st->stored += take;
     }

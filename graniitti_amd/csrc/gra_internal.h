@@ -79,6 +79,13 @@ struct gra_handle {
   // per-point buffers
   gra_hip::DevBuf d_x, d_W, d_amp2, d_flags, d_vw, d_bins, d_rec, d_ncand, d_grid;
   gra_hip::DevBuf d_part_hist, d_part_scal, d_stats, d_aux;
+  // generation passes: accepted indices, hit masks, tile counts/offsets, event
+  // records (grow-only), and the pass counters in pinned host memory
+  gra_hip::DevBuf d_gen_idx, d_gen_mask, d_gen_tile, d_gen_cnt, d_gen_ev;
+  unsigned long long* h_gen_cnt = nullptr;
+  // pinned staging of event records bound for pageable caller memory (grow-only)
+  void* h_ev_stage = nullptr;
+  size_t h_ev_stage_bytes = 0;
   std::vector<double> h_stats;
 
   // timing
@@ -102,6 +109,12 @@ int run_integrand(gra_handle* h, const double* d_grid, unsigned int bins, uint64
                   uint32_t iter, uint64_t first, const double* d_x, uint64_t n, double* d_W,
                   double* d_amp2, uint8_t* d_flags, double* d_vw, uint8_t* d_bins);
 int ensure_point_buffers(gra_handle* h, uint64_t n);
+// Buffers of a generation pass of n points storing up to cap events
+int ensure_gen_buffers(gra_handle* h, uint64_t n, uint64_t cap);
+// Copies n event records from device memory to the caller's host buffer and
+// synchronizes the stream: directly into pinned memory, else through the
+// handle's pinned staging buffer
+int events_to_host(gra_handle* h, gra_event* dst, const void* d_src, uint64_t n);
 // Adds a pass's NaN and Inf weight counts to the handle's; GRA_E_NUMERIC once
 // either total exceeds GRA_MAX_NONFINITE
 int count_nonfinite(gra_handle* h, uint64_t n_nan, uint64_t n_inf);

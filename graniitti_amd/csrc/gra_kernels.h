@@ -42,11 +42,16 @@ hipError_t launch_map(const double* grid, int D, unsigned int bins, const double
                       double* x, double* vw, int32_t* b, hipStream_t st);
 // Hit-or-miss of a generation pass: f = W vw (W alone if vw is NULL) of valid
 // points against maxf, U from Philox (seed, 0xC0000000 | iter, first + i); the
-// accepted indices (pass-local) go to idx; counts: trials, valid, accepted,
-// overflow, NaN weights, Inf weights
+// first max_out accepted indices (pass-local, ascending) go to idx; counts:
+// trials, valid, accepted, overflow, NaN weights, Inf weights (all six
+// written, no atomics). masks holds accept_mask_words(n) words, tiles
+// accept_tile_words(n) entries; 0 < n < 2^32.
+size_t accept_mask_words(uint64_t n);
+size_t accept_tile_words(uint64_t n);
 hipError_t launch_accept(const double* W, const double* vw, const uint8_t* flags, uint64_t n,
-                         double maxf, uint64_t seed, uint32_t iter, uint64_t first, uint64_t* idx,
-                         uint64_t max_out, unsigned long long* counts, hipStream_t st);
+                         double maxf, uint64_t seed, uint32_t iter, uint64_t first, uint64_t* masks,
+                         uint32_t* tiles, uint64_t* idx, uint64_t max_out,
+                         unsigned long long* counts, hipStream_t st);
 hipError_t launch_unweight(const double* w, uint64_t n, double wmax, uint64_t seed, uint8_t* acc,
                            unsigned long long* counts, hipStream_t st);
 

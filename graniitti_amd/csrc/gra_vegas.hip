@@ -256,13 +256,16 @@ __global__ void k_unweight(const double* __restrict__ w, uint64_t n, double wmax
   }
 }
 
-// Hit-or-miss of the generation pass: f = W vw of valid points against maxf;
-// accepted point indices are appended (wave-aggregated) up to max_out.
-// counts: trials, valid, accepted, overflow, NaN weights, Inf weights.
+// Hit-or-miss of the generation pass: f = W vw of valid points against maxf.
+// Each wave stores five ballots of its 64 points (word 5 g + m of masks for
+// group g): hits, valid points, overflows (f > maxf), NaN weights and Inf
+// weights. k_tile_count, k_tile_scan and k_tile_write then count them and
+// compact the hit indices in index order, with no atomics, so the result does
+// not depend on the schedule.
+constexpr int kAccMasks = 5;
 __global__ void k_accept(const double* __restrict__ W, const double* __restrict__ vw,
                          const uint8_t* __restrict__ flags, uint64_t n, double maxf, uint64_t seed,
-                         uint32_t iter, uint64_t first, uint64_t* __restrict__ idx, uint64_t max_out,
-                         unsigned long long* __restrict__ counts) {
+                         uint32_t iter, uint64_t first, uint64_t* __restrict__ masks) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool valid = false, hit = false, over = false, is_nan = false, is_inf = false;
   if (i < n) {
@@ -281,25 +284,144 @@ __global__ void k_accept(const double* __restrict__ W, const double* __restrict_
     }
     hit = hit && valid;
   }
-  const uint64_t mt = __ballot(i < n);
-  const uint64_t mv = __ballot(valid), mh = __ballot(hit), mo = __ballot(over);
-  const uint64_t mnan = __ballot(is_nan), minf = __ballot(is_inf);
+  const uint64_t m[kAccMasks] = {__ballot(hit), __ballot(valid), __ballot(over), __ballot(is_nan),
+                                 __ballot(is_inf)};
   const int lane = threadIdx.x & 63;
-  if (lane == 0) {
-    if (mt) atomicAdd(&counts[0], (unsigned long long)__popcll(mt));
-    if (mv) atomicAdd(&counts[1], (unsigned long long)__popcll(mv));
-    if (mo) atomicAdd(&counts[3], (unsigned long long)__popcll(mo));
-    if (mnan) atomicAdd(&counts[4], (unsigned long long)__popcll(mnan));
-    if (minf) atomicAdd(&counts[5], (unsigned long long)__popcll(minf));
+  // lane 0 holds the wave's smallest index: the wave has points iff lane 0 does
+  if (lane < kAccMasks && (i - lane) < n) {
+    uint64_t v = m[0];
+#pragma unroll
+    for (int k = 1; k < kAccMasks; ++k) v = (lane == k) ? m[k] : v;
+    masks[kAccMasks * ((i - lane) >> 6) + lane] = v;
   }
-  if (mh) {
-    const int leader = __ffsll((unsigned long long)mh) - 1;
-    unsigned long long base = 0;
-    if (lane == leader) base = atomicAdd(&counts[2], (unsigned long long)__popcll(mh));
-    base = __shfl(base, leader);
-    if (hit) {
-      const uint64_t pos = base + (uint64_t)__popcll(mh & ((1ull << lane) - 1ull));
-      if (pos < max_out) idx[pos] = i;
+}
+
+// Ordered compaction of the hit masks: a tile is kAccTileWords groups of 64
+// points (one block of 256 threads, kAccWordsPerThread consecutive groups per
+// thread)
+constexpr int kAccWordsPerThread = 16;
+constexpr int kAccTileWords = 256 * kAccWordsPerThread;
+constexpr int kAccMaxTiles = 1024 * 16;  // k_tile_scan: 1024 threads x 16 tiles
+
+// Exclusive prefix sum of v over the 256 threads of a block; total in *tot.
+// sh holds 4 entries; a second call in the same kernel must follow a barrier.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32_t* tot) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(x, d);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) sh[wave] = x;
+  __syncthreads();
+  uint32_t base = 0, all = 0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+    if (w < wave) base += sh[w];
+    all += sh[w];
+  }
+  *tot = all;
+  return base + x - v;
+}
+
+// Per tile: the five mask counts (tile_cnt[5 t + m])
+__global__ __launch_bounds__(256) void k_tile_count(const uint64_t* __restrict__ masks,
+                                                    uint64_t ngroups, uint32_t* __restrict__ tile_cnt) {
+  __shared__ uint32_t sh[kAccMasks][4];
+  const uint64_t g0 = (uint64_t)blockIdx.x * kAccTileWords + threadIdx.x * kAccWordsPerThread;
+  uint32_t c[kAccMasks] = {};
+  for (int j = 0; j < kAccWordsPerThread; ++j) {
+    if (g0 + j >= ngroups) break;
+#pragma unroll
+    for (int k = 0; k < kAccMasks; ++k) c[k] += (uint32_t)__popcll(masks[kAccMasks * (g0 + j) + k]);
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < kAccMasks; ++k) {
+    uint32_t x = c[k];
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d);
+    if (lane == 0) sh[k][wave] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < kAccMasks)
+    tile_cnt[kAccMasks * blockIdx.x + threadIdx.x] =
+        sh[threadIdx.x][0] + sh[threadIdx.x][1] + sh[threadIdx.x][2] + sh[threadIdx.x][3];
+}
+
+// One block: exclusive offsets of the tiles' hits, and the pass counters
+// (trials, valid, accepted, overflow, NaN, Inf)
+__global__ __launch_bounds__(1024) void k_tile_scan(const uint32_t* __restrict__ tile_cnt,
+                                                    int ntiles, uint64_t n,
+                                                    uint32_t* __restrict__ tile_off,
+                                                    unsigned long long* __restrict__ counts) {
+  __shared__ uint32_t sh[16];
+  __shared__ unsigned long long tot[kAccMasks];
+  if (threadIdx.x < kAccMasks) tot[threadIdx.x] = 0;
+  const int t0 = threadIdx.x * 16;
+  uint32_t c = 0;
+  unsigned long long o[kAccMasks - 1] = {};
+  for (int j = 0; j < 16; ++j)
+    if (t0 + j < ntiles) {
+      c += tile_cnt[kAccMasks * (t0 + j)];
+#pragma unroll
+      for (int k = 1; k < kAccMasks; ++k) o[k - 1] += tile_cnt[kAccMasks * (t0 + j) + k];
+    }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t x = c;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(x, d);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) sh[wave] = x;
+  __syncthreads();
+#pragma unroll
+  for (int k = 1; k < kAccMasks; ++k)
+    if (o[k - 1]) atomicAdd(&tot[k], o[k - 1]);
+  uint32_t base = 0, all = 0;
+  for (int w = 0; w < 16; ++w) {
+    if (w < wave) base += sh[w];
+    all += sh[w];
+  }
+  uint32_t run = base + x - c;
+  for (int j = 0; j < 16; ++j)
+    if (t0 + j < ntiles) {
+      tile_off[t0 + j] = run;
+      run += tile_cnt[kAccMasks * (t0 + j)];
+    }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    counts[0] = n;
+    counts[1] = tot[1];
+    counts[2] = all;
+    counts[3] = tot[2];
+    counts[4] = tot[3];
+    counts[5] = tot[4];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_tile_write(const uint64_t* __restrict__ masks,
+                                                    uint64_t ngroups,
+                                                    const uint32_t* __restrict__ tile_off,
+                                                    uint64_t* __restrict__ idx, uint64_t max_out) {
+  __shared__ uint32_t sh[4];
+  const uint64_t g0 = (uint64_t)blockIdx.x * kAccTileWords + threadIdx.x * kAccWordsPerThread;
+  uint32_t c = 0;
+  for (int j = 0; j < kAccWordsPerThread; ++j) {
+    if (g0 + j >= ngroups) break;
+    c += (uint32_t)__popcll(masks[kAccMasks * (g0 + j)]);
+  }
+  uint32_t tot;
+  uint64_t pos = (uint64_t)tile_off[blockIdx.x] + block_exclusive_scan(c, sh, &tot);
+  for (int j = 0; j < kAccWordsPerThread; ++j) {
+    if (g0 + j >= ngroups || pos >= max_out) break;
+    uint64_t m = masks[kAccMasks * (g0 + j)];
+    while (m) {
+      const int b = __ffsll((unsigned long long)m) - 1;
+      if (pos < max_out) idx[pos] = ((g0 + j) << 6) + (uint64_t)b;
+      ++pos;
+      m &= m - 1;
     }
   }
 }
@@ -320,12 +442,28 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st) {
   return hipGetLastError();
 }
 
+size_t accept_mask_words(uint64_t n) { return kAccMasks * (size_t)((n + 63) / 64); }
+static size_t accept_tiles(uint64_t n) {
+  return (size_t)(((n + 63) / 64 + kAccTileWords - 1) / kAccTileWords);
+}
+size_t accept_tile_words(uint64_t n) { return (kAccMasks + 1) * accept_tiles(n); }
+
 hipError_t launch_accept(const double* W, const double* vw, const uint8_t* flags, uint64_t n,
-                         double maxf, uint64_t seed, uint32_t iter, uint64_t first, uint64_t* idx,
-                         uint64_t max_out, unsigned long long* counts, hipStream_t st) {
+                         double maxf, uint64_t seed, uint32_t iter, uint64_t first, uint64_t* masks,
+                         uint32_t* tiles, uint64_t* idx, uint64_t max_out,
+                         unsigned long long* counts, hipStream_t st) {
+  if (n == 0 || n > 0xFFFFFFFFull) return hipErrorInvalidValue;
+  const uint64_t ng = (n + 63) / 64;
+  const size_t nt = accept_tiles(n);
+  if (nt > (size_t)kAccMaxTiles) return hipErrorInvalidValue;
   const unsigned int nb = (unsigned int)((n + 255) / 256);
   hipLaunchKernelGGL(k_accept, dim3(nb), dim3(256), 0, st, W, vw, flags, n, maxf, seed, iter, first,
-                     idx, max_out, counts);
+                     masks);
+  hipLaunchKernelGGL(k_tile_count, dim3((unsigned int)nt), dim3(256), 0, st, masks, ng, tiles);
+  uint32_t* off = tiles + kAccMasks * nt;
+  hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, st, tiles, (int)nt, n, off, counts);
+  hipLaunchKernelGGL(k_tile_write, dim3((unsigned int)nt), dim3(256), 0, st, masks, ng, off, idx,
+                     max_out);
   return hipGetLastError();
 }
 

@@ -54,6 +54,37 @@ def test_hit_or_miss_matches_host_restatement():
     np.testing.assert_allclose(ev["f"], f[want], rtol=1e-13)
 
 
+def test_generation_into_pinned_and_pageable_buffers_agree():
+    # the same pass into pageable memory (through the handle's pinned staging)
+    # and into pinned memory from gra_hip_host_alloc (direct copy), with a
+    # max_events cut that keeps the accepted points of smallest index; a pass
+    # of 2^20 + 37 points covers a partial last group of 64 and several tiles
+    cfg = abi.cfg_default(1)
+    n, seed, it = (1 << 20) + 37, 99, 5
+    say(f"case: generate {n} calls, preset 1, pinned vs pageable event buffers")
+    with MContinuum(cfg) as P:
+        G = MGraniitti(P)
+        G.Integrate(vegas_cfg(1 << 17, 3, burnin=3, seed=7))
+        maxf = G.result.maxf
+        out = []
+        for cap in (1 << 16, 500):
+            page = (abi.GraEvent * cap)()
+            with abi.PinnedEvents(cap) as pin:
+                for buf in (page, pin.buf):
+                    st = abi.GraGenStats()
+                    P._chk(P.lib.gra_hip_generate(P.h, C.byref(G.grid), n, seed, it, maxf, buf,
+                                                  cap, C.byref(st)))
+                    out.append((cap, st.trials, st.accepted, st.stored,
+                                np.ctypeslib.as_array(buf)[: st.stored].copy()))
+    say("accepted " + ", ".join(f"{a} (stored {s}, cap {c})" for c, _, a, s, _ in out))
+    for cap, trials, acc, stored, ev in out:
+        assert trials == n and stored == min(acc, cap) and acc > 500
+        assert np.all(np.diff(ev["index"].astype(np.int64)) > 0)
+    np.testing.assert_array_equal(out[0][4], out[1][4])
+    np.testing.assert_array_equal(out[2][4], out[3][4])
+    np.testing.assert_array_equal(out[2][4], out[0][4][:500])
+
+
 def check_records(p, cfg):
     E = cfg.beam_E1
     pz = np.sqrt(E * E - cfg.m_beam ** 2)
