@@ -39,15 +39,14 @@ sys.path.insert(0, ROOT)
 from graniitti_amd import abi  # noqa: E402
 from graniitti_amd.process import MContinuum, MGraniitti, vegas_cfg  # noqa: E402
 
-# Algorithmic work of one screening-loop node of the FP32 amplitude kernel, counted
-# from its gfx950 assembly along the path of an accepted node (TUNE0: offshellFF = 2,
-# series SolvePz step): 52 FMA (2 each), 19 multiplies, 15 adds or subtractions and
-# 9 transcendental or reciprocal instructions (4 log2, 2 exp2, 1 rcp, sin, cos; 1
-# each); comparisons, selects and moves are not counted. A gfx950 SIMD issues up to
-# two FP32 VALU instructions per quad-cycle (SQ_ACTIVE_INST_VALU2 counts the
-# quad-cycles that do), so the bound is the 157.3 TFLOP/s FP32 vector peak, even
-# without packed FP32.
-FLOP_PER_NODE = 147
+# Algorithmic work of the screening loop: counted in the run by the op-counting
+# instance of the FP32 node (gra_hip_screen_opcount) over OPCOUNT_POINTS screened
+# points of the last timed pass: every operation the node formulation executes,
+# rejected nodes included (fma = 2 FLOP; add, sub, mul, log2, exp2, rcp, sqrt,
+# sin, cos = 1; compares 0). A gfx950 SIMD issues up to two FP32 VALU
+# instructions per quad-cycle (SQ_ACTIVE_INST_VALU2 counts the quad-cycles that
+# do), so the bound is the 157.3 TFLOP/s FP32 vector peak, even without packed FP32.
+OPCOUNT_POINTS = 65536
 VALU_PEAK_TFLOPS = 157.3
 # HBM bytes of one screening launch at the default workload, from rocprofv3
 # FETCH_SIZE / WRITE_SIZE runs of this command (the counters cannot be read inside
@@ -373,17 +372,28 @@ def main():
         nodes = (cfg.n_loop_phi + 1) * (cfg.n_loop_kt + 1)
         cand_per_launch = tot["valid"] / n_scr
         avg_ms = ms_scr / n_scr
-        flop = FLOP_PER_NODE * nodes * cand_per_launch
-        achieved = flop / (avg_ms * 1e-3) / 1e12
+        oc = (C.c_double * 12)()
+        abi.check(P.lib.gra_hip_screen_opcount(P.h, OPCOUNT_POINTS, oc), P.h)
+        flop_pt = (oc[3] + oc[4] + 2 * oc[5] + oc[6]) / oc[0]
+        achieved = flop_pt * cand_per_launch / (avg_ms * 1e-3) / 1e12
         roofline = {
             "bound": "valu", "kernel": "k_screen (screening loop)",
             "achieved": achieved, "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / VALU_PEAK_TFLOPS, "traffic": screen_traffic(),
             "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT),
             "avg_launch_ms": avg_ms, "launches": n_scr,
-            "flop_per_node": FLOP_PER_NODE, "node_dtype": "f32",
-            "nodes_per_point": nodes, "screened_points_per_launch": cand_per_launch,
+            "flop_per_point": flop_pt, "flop_per_node": flop_pt / nodes,
+            "flop_source": f"op-counting run of the node code over {int(oc[0])} screened "
+                           f"points of the last pass (gra_hip_screen_opcount)",
+            "accepted_node_fraction": oc[2] / oc[1],
+            "flop_per_accepted_node": oc[8] + oc[9] + 2 * oc[10] + oc[11],
+            "accepted_node_ops": {"add": oc[8], "mul": oc[9], "fma": oc[10], "trans": oc[11]},
+            "node_dtype": "f32", "nodes_per_point": nodes,
+            "screened_points_per_launch": cand_per_launch,
         }
+        log(f"op count: {int(oc[0])} points, accepted nodes {oc[2] / oc[1]:.4f}, "
+            f"{flop_pt / nodes:.2f} FLOP per node, one accepted node {oc[8]:.0f} add "
+            f"{oc[9]:.0f} mul {oc[10]:.0f} fma {oc[11]:.0f} trans")
     vegas = {"sigma": integ.sigma, "sigma_err": integ.sigma_err, "time_to_0p1pct_s": None}
     prec_res = None
     if not args.no_precision_run:

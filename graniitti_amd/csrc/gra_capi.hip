@@ -1341,6 +1341,32 @@ extern "C" int gra_hip_eikonal_tables(gra_handle* h, double* node_amp, double* m
   return GRA_OK;
 }
 
+extern "C" int gra_hip_screen_opcount(gra_handle* h, uint64_t max_points, double* out) {
+  if (!h) return set_global_error(GRA_E_ARG, "handle is NULL");
+  if (!out) return fail(h, GRA_E_ARG, "screen_opcount: out is NULL");
+  if (max_points == 0 || max_points > 65536)
+    return fail(h, GRA_E_ARG, "screen_opcount: max_points %llu outside [1, 65536]",
+                (unsigned long long)max_points);
+  if (!h->P.screening || h->P.process != GRA_PROC_REGGE_CON || !f32_kernels(h->P))
+    return fail(h, GRA_E_STATE,
+                "screen_opcount: counts the FP32 Regge screening node (screened PP[CON], "
+                "FP32 amplitude)");
+  if (!h->d_rec.p || !h->d_ncand.p)
+    return fail(h, GRA_E_STATE, "screen_opcount: no screened evaluation has run on this handle");
+  DevBuf d_out;
+  if (d_out.ensure(12 * sizeof(unsigned long long)))
+    return fail(h, GRA_E_NOMEM, "screen_opcount buffer");
+  GRA_HIP_CHECK(h, launch_fill_u32(d_out.as<uint32_t>(), 0u, 24, h->stream));
+  GRA_HIP_CHECK(h, k32::launch_screen_opcount(h->P, h->d_nodes.p, h->d_rec.p,
+                                              h->d_ncand.as<uint32_t>(), (uint32_t)max_points,
+                                              d_out.as<unsigned long long>(), h->stream));
+  unsigned long long c[12];
+  GRA_HIP_CHECK(h, hipMemcpyAsync(c, d_out.p, sizeof(c), hipMemcpyDeviceToHost, h->stream));
+  GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  for (int k = 0; k < 12; ++k) out[k] = (double)c[k];
+  return GRA_OK;
+}
+
 extern "C" int gra_hip_kernel_time(gra_handle* h, int32_t which, double* total_ms,
                                    uint64_t* launches, int32_t reset) {
   if (!h) return set_global_error(GRA_E_ARG, "handle is NULL");

@@ -28,6 +28,16 @@ namespace gra_hip {
 GRA_DECLARE_INTEGRAND_LAUNCHERS(k64)
 GRA_DECLARE_INTEGRAND_LAUNCHERS(k32)
 
+namespace k32 {
+// Op-counting run of the FP32 Regge screening node over the first
+// min(*n_cand, max_pts) records (max_pts <= 65536); out[12] (zeroed by the
+// caller): points, nodes, accepted nodes, add, mul, fma, transcendental, compare
+// counts, then add, mul, fma, transcendental of one accepted node
+hipError_t launch_screen_opcount(const RunParams& P, const void* nodes, const void* rec,
+                                 const uint32_t* n_cand, uint32_t max_pts, unsigned long long* out,
+                                 hipStream_t st);
+}  // namespace k32
+
 // Scalars of one accumulated iteration, ahead of the [2][D][128] histograms:
 // fsum, its compensation, f2sum, its compensation, max W, max f, and the counts of
 // kinematics-valid, fiducial-valid, valid, NaN-weight and Inf-weight points

@@ -577,6 +577,21 @@ int gra_hip_flow_forward(gra_handle* h, const float* z, size_t n, float* x, floa
 int gra_hip_flow_grad(gra_handle* h, const float* x, const float* c, size_t n, float* grad,
                       float* logq);
 
+/* Op count of the FP32 screening node of the Regge continuum (screened PP[CON]
+ * in the FP32-amplitude mode). Measurement only: no counterpart in the
+ * reference. An op-counting instance of the node code re-runs the loop of the
+ * first min(max_points, screened points) records of the handle's last integrand
+ * evaluation through its own buffers (any entry point but
+ * gra_hip_eval_points_noalloc; max_points <= 65536) and tallies every
+ * operation it executes; a
+ * rejected node counts what ran before its rejection. out[12]: points, nodes,
+ * accepted nodes, add/sub, mul, fma, transcendental (log2, exp2, rcp, sqrt, sin,
+ * cos, and the FP64 exp of each point's epilogue), compare; then add, mul, fma,
+ * transcendental of one accepted node. FLOP = add + mul + 2 fma + transcendental.
+ * The counting launch uses a static device tally: calls that overlap on one GPU
+ * race. */
+int gra_hip_screen_opcount(gra_handle* h, uint64_t max_points, double* out);
+
 /* Accumulated duration of the integrand kernels (HIP events on the handle's
  * stream) since creation or the last reset, and the number of launches. */
 int gra_hip_kernel_time(gra_handle* h, int32_t which, double* total_ms, uint64_t* launches,
