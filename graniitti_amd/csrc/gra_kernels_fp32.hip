@@ -1,5 +1,5 @@
-// Integrand kernels with FP64 kinematics and an FP32 amplitude / FP32
-// compensated screening sum (contraction on).
+// Integrand kernels with FP64 kinematics and an FP32 amplitude in the screening
+// loop (contraction on).
 #define GRA_KERNEL_FP32 1
 #define GRA_KNS k32
 #include "gra_kernels.inc"
