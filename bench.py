@@ -391,6 +391,15 @@ def main():
             "node_dtype": "f32", "nodes_per_point": nodes,
             "screened_points_per_launch": cand_per_launch,
         }
+        # the FP32 rate this GPU sustains without packed FP32 (which this build does not
+        # use): independent FMA chains, measured now on the same device
+        ceil = C.c_double()
+        abi.check(P.lib.gra_hip_fp32_fma_rate(C.byref(ceil)))
+        roofline["attainable_fp32_no_packed_tflops"] = ceil.value
+        roofline["frac_of_attainable"] = achieved / ceil.value
+        log(f"k_screen {achieved:.1f} TFLOP/s: {achieved / VALU_PEAK_TFLOPS:.3f} of the published "
+            f"peak, {achieved / ceil.value:.3f} of the {ceil.value:.1f} TFLOP/s FMA-chain rate "
+            f"without packed FP32")
         log(f"op count: {int(oc[0])} points, accepted nodes {oc[2] / oc[1]:.4f}, "
             f"{flop_pt / nodes:.2f} FLOP per node, one accepted node {oc[8]:.0f} add "
             f"{oc[9]:.0f} mul {oc[10]:.0f} fma {oc[11]:.0f} trans")

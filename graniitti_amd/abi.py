@@ -190,6 +190,7 @@ SIGNATURES = {
                                     P(C.c_float)]),
     "gra_hip_kernel_time": (C.c_int, [VP, I32, P(D), P(U64), I32]),
     "gra_hip_screen_opcount": (C.c_int, [VP, U64, P(D)]),
+    "gra_hip_fp32_fma_rate": (C.c_int, [P(D)]),
     "gra_hepmc3_open": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, P(VP)]),
     "gra_hepmc3_write": (C.c_int, [VP, P(GraCfg), P(GraRecordPdg), P(GraEvent), U64, D, D, I32]),
     "gra_hepmc3_close": (C.c_int, [VP]),

@@ -30,6 +30,7 @@ UNITS = {
     "gra_vegas.hip": ["-ffp-contract=off"],
     "gra_capi.hip": ["-ffp-contract=off"],
     "gra_flow.hip": ["-ffp-contract=fast"],
+    "gra_measure.hip": ["-ffp-contract=fast"],
     "gra_config.cpp": ["-ffp-contract=off", "-x", "c++"],
     "gra_hepmc3.cpp": ["-ffp-contract=off", "-x", "c++"],
     "gra_eikcache.cpp": ["-ffp-contract=off", "-x", "c++"],

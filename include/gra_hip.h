@@ -592,6 +592,13 @@ int gra_hip_flow_grad(gra_handle* h, const float* x, const float* c, size_t n, f
  * race. */
 int gra_hip_screen_opcount(gra_handle* h, uint64_t max_points, double* out);
 
+/* The FP32 FMA rate (TFLOP/s, FMA = 2) the current device sustains on
+ * independent FMA chains without packed FP32 instructions, which this library
+ * does not use: the attainable ceiling of its FP32 vector kernels, measured with
+ * HIP events on a private stream (about 10 ms). Measurement only: no
+ * counterpart in the reference. */
+int gra_hip_fp32_fma_rate(double* tflops);
+
 /* Accumulated duration of the integrand kernels (HIP events on the handle's
  * stream) since creation or the last reset, and the number of launches. */
 int gra_hip_kernel_time(gra_handle* h, int32_t which, double* total_ms, uint64_t* launches,
