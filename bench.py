@@ -134,7 +134,7 @@ def flow_cfg(dim, layers, bins, hidden, depth, batch, seed=2024):
     return fc
 
 
-def flow_leg(P, grid, fc, steps, n_pass, vegas, gen_passes):
+def flow_leg(P, grid, fc, steps, n_pass, vegas, gen_passes, plateau_steps=0):
     """The learned sampler on the process of P: an RQS flow composed with the
     adapted VEGAS grid (identity at initialisation), trained by weight-variance
     minimisation (Adam, lr 1e-3) on batches of its own weighted points, then
@@ -193,6 +193,23 @@ def flow_leg(P, grid, fc, steps, n_pass, vegas, gen_passes):
         "time_ratio_vs_vegas": (res.seconds_to_precision / vegas["time_to_0p1pct_s"]
                                 if res.reached and vegas["time_to_0p1pct_s"] else None),
     }
+    if plateau_steps:
+        # the same flow from a fresh initialisation, its training stopped by the CV^2
+        # plateau test (windows of 50 steps, less than 5% improvement per window)
+        abi.check(lib.gra_hip_flow_init(P.h, C.byref(fc)), P.h)
+        abi.check(lib.gra_hip_flow_set_grid(P.h, C.byref(grid)), P.h)
+        rc.train_steps, rc.check_every, rc.plateau = plateau_steps, 50, 0.05
+        rp = abi.GraFlowResult()
+        abi.check(lib.gra_hip_flow_integrate(P.h, C.byref(rc), C.byref(g_fb), C.byref(rp)), P.h)
+        out["plateau_run"] = {
+            "max_steps": plateau_steps, "window": 50, "min_improvement": 0.05,
+            "train_steps_run": int(rp.train_steps_run), "train_s": rp.train_seconds,
+            "last_window_cv2": rp.train_cv2, "cv2": rp.cv2, "ess": rp.ess,
+            "time_to_0p1pct_s": rp.seconds_to_precision if rp.reached else None,
+            "vegas_fallback": bool(rp.fallback),
+            "time_ratio_vs_vegas": (rp.seconds_to_precision / vegas["time_to_0p1pct_s"]
+                                    if rp.reached and vegas["time_to_0p1pct_s"] else None),
+        }
     log(f"flow: {out}")
     return out
 
@@ -266,7 +283,7 @@ def rhorho_leg(args):
         G = MGraniitti(P)
         G.Integrate(vegas_cfg(n, MAXF_SAMPLE, burnin=5, seed=1818))
         fc = flow_cfg(cfg.dim, 10, 32, 256, 3, 1 << 18)
-        fl = flow_leg(P, G.grid, fc, args.flow5_steps, 1 << 20, v, 2)
+        fl = flow_leg(P, G.grid, fc, args.flow5_steps, 1 << 20, v, 2, args.flow5_steps)
     return {"preset": 3, "vegas": v, "flow": fl}
 
 
@@ -422,7 +439,7 @@ def main():
     if not args.no_flow:
         fc = flow_cfg(cfg.dim, args.flow_layers, args.flow_bins, args.flow_hidden,
                       args.flow_depth, args.flow_batch)
-        flow = flow_leg(P, G.grid, fc, args.flow_steps, N, vegas, 4)
+        flow = flow_leg(P, G.grid, fc, args.flow_steps, N, vegas, 4, args.flow_steps)
     flow5 = None
     if not args.no_flow and not args.no_flow5 and args.preset == 2:
         try:

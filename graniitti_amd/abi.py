@@ -118,8 +118,8 @@ class GraFlowStats(C.Structure):
 
 class GraFlowRunCfg(C.Structure):
     _fields_ = [
-        ("train_steps", I32), ("pad0", I32), ("n_max", U64), ("seed", U64), ("precision", D),
-        ("ess_min", D), ("vegas", GraVegasCfg),
+        ("train_steps", I32), ("check_every", I32), ("n_max", U64), ("seed", U64),
+        ("precision", D), ("ess_min", D), ("vegas", GraVegasCfg), ("plateau", D),
     ]
 
 
@@ -128,6 +128,7 @@ class GraFlowResult(C.Structure):
         ("sigma", D), ("sigma_err", D), ("cv2", D), ("ess", D), ("maxw", D),
         ("train_seconds", D), ("sample_seconds", D), ("seconds_to_precision", D),
         ("n", U64), ("fallback", I32), ("reached", I32), ("vegas", GraVegasResult),
+        ("train_steps_run", U64), ("train_cv2", D),
     ]
 
 

@@ -477,7 +477,7 @@ __global__ __launch_bounds__(256) void k_rqs_backward_t(
     const int dr = blockDim.x / ldp, dc = blockDim.x % ldp;
     for (int e2 = threadIdx.x; e2 < nrow * ldp; e2 += blockDim.x) {
       const float v = tile[rr * ldt + c];
-      gp[(size_t)row0 * ldp + e2] = v;
+      if (gp) gp[(size_t)row0 * ldp + e2] = v;
       gp16[(size_t)row0 * ldp + e2] = (__bf16)v;
       rr += dr;
       c += dc;
@@ -782,6 +782,89 @@ __global__ __launch_bounds__(256) void k_gemm_tn_part(const void* __restrict__ X
     for (int rr = 0; rr < 4; ++rr) {
       const int row = m0 + wave * 16 + 4 * (lane >> 4) + rr;
       part[((size_t)ch * kin_t + row) * N + col] = acc[t][rr];
+    }
+  }
+}
+
+// Weight-gradient partials of a BF16 product, one block per (row chunk, group of
+// 64 output columns): part[ch][m][n] = sum_{b in chunk} X'[b][m] dY[b][n] for all
+// m < 16 MT, where X' is X [rows x Kin] followed by a column of ones (row Kin of
+// the result is the bias gradient, the column sum of dY) and zeros up to 16 MT.
+// Each step of 32 rows stages X'^T [16 MT][32] and dY^T [64][32] in LDS: a thread
+// gathers 8 consecutive rows of one column with coalesced 2-byte loads (a wave
+// reads 64 consecutive columns of a row per load) and writes them as one 16-byte
+// MFMA operand. Wave w then multiplies m-tiles w, w + 4, ... against the four
+// n-tiles, whose fragments it reads once per step.
+template <int MW>
+__global__ __launch_bounds__(256) void k_wgrad(const __bf16* __restrict__ X, int ldx,
+                                               const __bf16* __restrict__ Y, int ldy, int Kin,
+                                               int N, int nrows, int chunk,
+                                               float* __restrict__ part) {
+  constexpr int kLd = 40;  // BF16 row stride of the staged tiles (32 rows + 8)
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  const int MT = Kin / 16 + 1, KT = 16 * MT;  // KT columns of X' (Kin, the ones, zeros)
+  __bf16* sx = (__bf16*)smem;                 // [KT][kLd]
+  __bf16* sy = sx + KT * kLd;                 // [64][kLd]
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int ch = blockIdx.x, n0 = blockIdx.y * 64;
+  const int b0 = ch * chunk, b1 = min(nrows, b0 + chunk);
+  f32x4 acc[MW][4];
+#pragma unroll
+  for (int i = 0; i < MW; ++i)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[i][t] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  const int kq = 8 * (lane >> 4);
+  for (int bb = b0; bb < b1; bb += 32) {
+    // X'^T: 4 KT groups of (column, 8 rows)
+    for (int q = threadIdx.x; q < 4 * KT; q += 256) {
+      const int c = q % KT, g = q / KT;
+      bf16x8 v;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int b = bb + 8 * g + j;
+        float one = (c == Kin && b < b1) ? 1.f : 0.f;
+        v[j] = (c < Kin && b < b1) ? X[(size_t)b * ldx + c] : (__bf16)one;
+      }
+      *(bf16x8*)(sx + c * kLd + 8 * g) = v;
+    }
+    {
+      const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+      bf16x8 v;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int b = bb + 8 * g + j;
+        v[j] = b < b1 ? Y[(size_t)b * ldy + n0 + c] : (__bf16)0.f;
+      }
+      *(bf16x8*)(sy + c * kLd + 8 * g) = v;
+    }
+    __syncthreads();
+    bf16x8 bv[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) bv[t] = *(const bf16x8*)(sy + (16 * t + (lane & 15)) * kLd + kq);
+#pragma unroll
+    for (int i = 0; i < MW; ++i) {
+      const int mt = wave + 4 * i;
+      if (mt < MT) {
+        const bf16x8 a = *(const bf16x8*)(sx + (16 * mt + (lane & 15)) * kLd + kq);
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+          acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv[t], acc[i][t], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < MW; ++i) {
+    const int mt = wave + 4 * i;
+    if (mt >= MT) continue;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int col = n0 + 16 * t + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = 16 * mt + 4 * (lane >> 4) + r;
+        part[((size_t)ch * KT + row) * N + col] = acc[i][t][r];
+      }
     }
   }
 }
@@ -1282,6 +1365,15 @@ static uint64_t splitmix(uint64_t& s) {
   return z ^ (z >> 31);
 }
 
+// Rows per block of k_wgrad: about 1024 blocks for n rows and N output columns,
+// between 512 and 4096 rows, a multiple of 32
+static int wgrad_chunk(int n, int N) {
+  const long c = (long)n * (N / 64) / 1024;
+  return (int)std::max(512L, std::min(4096L, (c / 32) * 32));
+}
+// m-tiles per wave of k_wgrad (0: no instance for this width)
+static int wgrad_mw(int Kin) { return Kin == 64 ? 2 : Kin == 128 ? 3 : Kin == 256 ? 5 : 0; }
+
 static int flow_alloc_batch(gra_handle* h, FlowState* F, int n) {
   if (n <= F->cap) return GRA_OK;
   const int D = F->D, L = F->L, H = F->H, dep = F->depth, ldp = F->ldp;
@@ -1306,7 +1398,13 @@ static int flow_alloc_batch(gra_handle* h, FlowState* F, int n) {
   bad |= F->wts.ensure(sizeof(double) * N) != 0;
   const size_t nch = (N + 2047) / 2048;
   const size_t big = (size_t)(std::max(H, ldp) + 64) * std::max(H, ldp);
-  bad |= F->part.ensure(sizeof(float) * std::max(nch * big, (size_t)3 * ((N + 255) / 256) * 2)) != 0;
+  // k_wgrad's partials: ceil(N / chunk) x (H + 16) x columns, for both product widths
+  size_t wg = 0;
+  for (int cols : {H, ldp})
+    wg = std::max(wg, (size_t)((N + wgrad_chunk(N, cols) - 1) / wgrad_chunk(N, cols)) *
+                          (size_t)(H + 16) * cols);
+  bad |= F->part.ensure(sizeof(float) * std::max(std::max(nch * big, wg),
+                                                 (size_t)3 * ((N + 255) / 256) * 2)) != 0;
   bad |= F->tmp.ensure(sizeof(double) * 3 * ((N + 255) / 256) + 64) != 0;
   if (bad) return fail(h, GRA_E_NOMEM, "flow batch buffers for %d points", n);
   F->cap = n;
@@ -1355,6 +1453,33 @@ static void gemm_nt(gra_handle* h, FlowState* F, const void* A, int lda, const v
 // deterministically (per-chunk partials, then a fixed-order sum)
 static void weight_grad(gra_handle* h, FlowState* F, const void* X, int ldx, const void* Y, int ldy,
                         int Kin, int N, int n, float* gW, float* gb) {
+  if (!F->mlp_fp32 && wgrad_mw(Kin) > 0) {
+    const int chunk = wgrad_chunk(n, N);
+    const int nch = (n + chunk - 1) / chunk;
+    const int KT = Kin + 16;
+    const size_t lds = sizeof(__bf16) * (size_t)(KT + 64) * 40;
+    const dim3 grid(nch, N / 64);
+    const __bf16* Xb = (const __bf16*)X;
+    const __bf16* Yb = (const __bf16*)Y;
+    float* part = F->part.as<float>();
+    switch (wgrad_mw(Kin)) {
+      case 2:
+        hipLaunchKernelGGL(k_wgrad<2>, grid, dim3(256), lds, h->stream, Xb, ldx, Yb, ldy, Kin, N, n,
+                           chunk, part);
+        break;
+      case 3:
+        hipLaunchKernelGGL(k_wgrad<3>, grid, dim3(256), lds, h->stream, Xb, ldx, Yb, ldy, Kin, N, n,
+                           chunk, part);
+        break;
+      default:
+        hipLaunchKernelGGL(k_wgrad<5>, grid, dim3(256), lds, h->stream, Xb, ldx, Yb, ldy, Kin, N, n,
+                           chunk, part);
+    }
+    const int tot = (Kin + 1) * N;
+    hipLaunchKernelGGL(k_part_reduce, dim3((tot + 31) / 32), dim3(256), 0, h->stream, part, nch,
+                       KT, N, Kin, gW, gb);
+    return;
+  }
   const int chunk = 2048;
   const int nch = (n + chunk - 1) / chunk;
   const dim3 grid(Kin / 64 + 1, N / 64, nch);
@@ -1563,7 +1688,8 @@ static int flow_backward(gra_handle* h, FlowState* F, int n, float* logq_chk) {
       const float* a_up = u_of(F, l);
       const float* a_gin = g;
       const float* a_coef = F->coef.as<float>();
-      float* a_gp = F->gprm.as<float>();
+      // the FP32 copy of the parameter gradients feeds only the FP32 MLP path
+      float* a_gp = F->mlp_fp32 ? F->gprm.as<float>() : nullptr;
       __bf16* a_gp16 = F->gprm16.as<__bf16>();
       LayerIdx idx = Ly.idx;
       int a_ldp = ldp, nn = n, DD = D, KK = F->K, RR = R;
@@ -1678,6 +1804,29 @@ static void fill_stats(gra_flow_stats* st, const double s[3], uint64_t n, double
   st->maxw = s[2];
   st->n = n;
   st->seconds = seconds;
+}
+
+// `steps` Adam steps on batches of n flow points (buffers allocated): st receives
+// the last batch's sums (sum w, sum w^2, max w) and *cv2sum the sum over the steps
+// of each batch's CV^2 of w
+static int flow_train_loop(gra_handle* h, FlowState* F, int n, int steps, double st[3],
+                           double* cv2sum) {
+  for (int it = 0; it < steps; ++it) {
+    int r = flow_sample_pass(h, F, n, F->fc.seed, (uint32_t)F->step, 0, nullptr, true);
+    if (r) return r;
+    r = flow_weights(h, F, n, st);
+    if (r) return r;
+    if (!(st[1] > 0)) return fail(h, GRA_E_NUMERIC, "flow_train: all weights are zero");
+    const double mean = st[0] / n;
+    *cv2sum += std::max(st[1] / n - mean * mean, 0.0) / (mean * mean);
+    hipLaunchKernelGGL(k_coef, dim3((n + 255) / 256), dim3(256), 0, h->stream, F->wts.as<double>(),
+                       n, 1.0 / st[1], F->coef.as<float>());
+    r = flow_backward(h, F, n, nullptr);
+    if (r) return r;
+    r = flow_adam(h, F);
+    if (r) return r;
+  }
+  return GRA_OK;
 }
 
 }  // namespace gra_hip
@@ -1855,20 +2004,9 @@ extern "C" int gra_hip_flow_train(gra_handle* h, int32_t steps, gra_flow_stats* 
   int r = flow_alloc_batch(h, F, n);
   if (r) return r;
   const auto t0 = std::chrono::steady_clock::now();
-  double st[3] = {0, 0, 0};
-  for (int it = 0; it < steps; ++it) {
-    r = flow_sample_pass(h, F, n, F->fc.seed, (uint32_t)F->step, 0, nullptr, true);
-    if (r) return r;
-    r = flow_weights(h, F, n, st);
-    if (r) return r;
-    if (!(st[1] > 0)) return fail(h, GRA_E_NUMERIC, "flow_train: all weights are zero");
-    hipLaunchKernelGGL(k_coef, dim3((n + 255) / 256), dim3(256), 0, h->stream, F->wts.as<double>(),
-                       n, 1.0 / st[1], F->coef.as<float>());
-    r = flow_backward(h, F, n, nullptr);
-    if (r) return r;
-    r = flow_adam(h, F);
-    if (r) return r;
-  }
+  double st[3] = {0, 0, 0}, cv2sum = 0.0;
+  r = flow_train_loop(h, F, n, steps, st, &cv2sum);
+  if (r) return r;
   GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
   if (last) {
     std::memset(last, 0, sizeof(*last));
@@ -1958,11 +2096,38 @@ extern "C" int gra_hip_flow_integrate(gra_handle* h, const gra_flow_run_cfg* rc,
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   };
   int r = GRA_OK;
-  if (rc->train_steps > 0) {
+  if (rc->train_steps > 0 && rc->check_every > 0 && rc->plateau > 0.0) {
+    // training in windows of check_every steps, stopped when the mean CV^2 of a
+    // window's training batches improves on the previous window's by less than
+    // the fraction rc->plateau
+    const uint64_t B64 = F->fc.batch;
+    if (B64 < 256 || B64 % 256 || B64 > (1u << 24))
+      return fail(h, GRA_E_ARG, "flow batch %llu: multiple of 256 up to 2^24",
+                  (unsigned long long)B64);
+    r = flow_alloc_batch(h, F, (int)B64);
+    if (r) return r;
+    double prev = -1.0, st[3];
+    int done_steps = 0;
+    while (done_steps < rc->train_steps) {
+      const int w = std::min(rc->check_every, rc->train_steps - done_steps);
+      double cv2sum = 0.0;
+      r = flow_train_loop(h, F, (int)B64, w, st, &cv2sum);
+      if (r) return r;
+      done_steps += w;
+      const double cur = cv2sum / w;
+      res->train_cv2 = cur;
+      if (prev > 0.0 && prev - cur < rc->plateau * prev) break;
+      prev = cur;
+    }
+    res->train_steps_run = (uint64_t)done_steps;
+  } else if (rc->train_steps > 0) {
     gra_flow_stats st;
     r = gra_hip_flow_train(h, rc->train_steps, &st);
     if (r) return r;
+    res->train_steps_run = (uint64_t)rc->train_steps;
+    res->train_cv2 = st.cv2;
   }
+  GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
   res->train_seconds = since();
   const int chunk = sample_chunk(F);
   if ((rc->n_max + chunk - 1) / chunk > 0xFFFF)

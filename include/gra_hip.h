@@ -524,13 +524,16 @@ int gra_hip_flow_sample(gra_handle* h, uint64_t n, uint64_t seed, double* w, dou
 /* The NEURO integrator run (MGraniitti::SampleNeuro integration, src/MGraniitti.cc:
  * 1527-1648) with plain VEGAS as fallback and judge. */
 typedef struct gra_flow_run_cfg {
-  int32_t train_steps; /* Adam steps before sampling (0: sample the flow as it is) */
-  int32_t pad0;
+  int32_t train_steps; /* Adam steps before sampling (0: sample the flow as it is), at most */
+  int32_t check_every; /* training window in steps for the plateau test (0: no test) */
   uint64_t n_max;      /* sampling budget in points */
   uint64_t seed;       /* sampling stream */
   double precision;    /* stop when sigma_err / sigma <= precision (0: use the budget) */
   double ess_min;      /* VEGAS takes over when the running ESS falls below this */
   gra_vegas_cfg vegas; /* the fallback VEGAS run */
+  double plateau;      /* stop training when a window's mean CV^2 of the training batches
+                          falls by less than this fraction of the previous window's (0: no
+                          test) */
 } gra_flow_run_cfg;
 
 typedef struct gra_flow_result {
@@ -543,10 +546,12 @@ typedef struct gra_flow_result {
   int32_t fallback;            /* 1: ESS < ess_min, sigma and sigma_err come from VEGAS */
   int32_t reached;             /* 1: sigma_err / sigma <= precision */
   gra_vegas_result vegas;      /* the fallback run (zero without one) */
+  uint64_t train_steps_run;    /* Adam steps taken (fewer than train_steps after a plateau) */
+  double train_cv2;            /* mean CV^2 of the last training window (or the last batch) */
 } gra_flow_result;
 
-/* Train (rc->train_steps), then sample in chunks until the precision or the
- * budget is reached; if the running effective sample size (sum w)^2 / (N sum w^2)
+/* Train (rc->train_steps, or fewer when the plateau test stops it), then sample
+ * in chunks until the precision or the budget is reached; if the running effective sample size (sum w)^2 / (N sum w^2)
  * drops below rc->ess_min, run gra_hip_vegas_run(rc->vegas) on grid (in/out) and
  * report its result with fallback = 1. */
 int gra_hip_flow_integrate(gra_handle* h, const gra_flow_run_cfg* rc, gra_grid* grid,

@@ -150,6 +150,26 @@ def test_flow_integrate_falls_back_to_vegas(trained):
     assert abs(out2.sigma - v.sigma) < 3 * comb
 
 
+def test_flow_integrate_stops_training_on_a_plateau(trained):
+    # windows of 5 steps: with a 90% improvement required per window the plateau
+    # test stops after the second window; with the test off every step is taken
+    P, G, res = trained
+    say("case: flow_integrate training with the CV^2 plateau test")
+    rc = abi.GraFlowRunCfg()
+    rc.train_steps, rc.n_max, rc.seed, rc.precision, rc.ess_min = 40, 1 << 18, 3, 0.0, 0.0
+    rc.check_every, rc.plateau = 5, 0.9
+    rc.vegas = vegas_cfg(1 << 17, 3, burnin=2, seed=55)
+    g = abi.GraGrid()
+    out = abi.GraFlowResult()
+    abi.check(P.lib.gra_hip_flow_integrate(P.h, C.byref(rc), C.byref(g), C.byref(out)), P.h)
+    say(f"plateau 0.9: {out.train_steps_run} steps, window CV^2 {out.train_cv2:.3f}")
+    assert out.train_steps_run == 10 and out.train_cv2 > 0
+    rc.plateau = 0.0
+    out2 = abi.GraFlowResult()
+    abi.check(P.lib.gra_hip_flow_integrate(P.h, C.byref(rc), C.byref(g), C.byref(out2)), P.h)
+    assert out2.train_steps_run == 40 and out2.fallback == 0
+
+
 def test_flow_on_the_screened_process_at_the_stated_size():
     # flow 6 layers, K = 16, MLP 2 x 128 (BF16 MFMA, FP32 accumulation), Adam at
     # lr 1e-3 for 500 steps on batches of 2^18 points weighted by the screened
