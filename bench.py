@@ -243,11 +243,16 @@ COPY_BYTES = 256 << 20
 
 
 def res_leg():
-    """PP[RES+CON] -> pi+ pi- (continuum + the eight ALICECUTS resonances, FP64 amplitude):
-    VEGAS integration throughput unscreened (2^20 points/iteration) and screened (2^18)."""
+    """PP[RES+CON] -> pi+ pi- (continuum + the eight ALICECUTS resonances): VEGAS
+    integration throughput unscreened (2^20 points/iteration, FP64) and screened
+    (2^18), with the loop nodes in FP64 and in the FP32-amplitude mode."""
     out = {}
-    for preset, n, warm, steps in ((6, 1 << 20, 2, 5), (7, 1 << 18, 1, 3)):
+    for name, preset, n, warm, steps, prec in (
+            ("unscreened", 6, 1 << 20, 2, 5, abi.GRA_AMP_FP64),
+            ("screened", 7, 1 << 18, 1, 3, abi.GRA_AMP_FP64),
+            ("screened_fp32_nodes", 7, 1 << 18, 1, 3, abi.GRA_AMP_FP32)):
         cfg = abi.cfg_default(preset)
+        cfg.amp_precision = prec
         with MContinuum(cfg) as P:
             grid = abi.GraGrid()
             abi.check(P.lib.gra_hip_grid_init(C.byref(grid), cfg.dim, 128))
@@ -265,10 +270,11 @@ def res_leg():
                     acc[1] += 1.0 / v
             dt = time.perf_counter() - t0
         pps = n * steps / dt
-        out["screened" if preset == 7 else "unscreened"] = {
-            "preset": preset, "points_per_step": n, "steps": steps, "points_per_s": pps,
+        out[name] = {
+            "preset": preset, "amplitude": "fp32 loop nodes" if prec == abi.GRA_AMP_FP32 else "fp64",
+            "points_per_step": n, "steps": steps, "points_per_s": pps,
             "sigma_barn": acc[0] / acc[1], "sigma_err_barn": acc[1] ** -0.5}
-        log(f"PP[RES+CON] preset {preset}: {pps:.3e} points/s, sigma {acc[0] / acc[1]:.5e} barn")
+        log(f"PP[RES+CON] {name}: {pps:.3e} points/s, sigma {acc[0] / acc[1]:.5e} barn")
     return out
 
 

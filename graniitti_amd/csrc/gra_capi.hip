@@ -276,8 +276,6 @@ int validate(const gra_cfg* c) {
     if (c->w_central > 0.0 || c->m_daughter > 0.0)
       return set_global_error(GRA_E_ARG,
                               "PP[RES(+CON)]: built for stable central particles");
-    if (c->amp_precision != GRA_AMP_FP64)
-      return set_global_error(GRA_E_ARG, "PP[RES(+CON)]: resonance amplitudes run in FP64 only");
     if (c->n_res < (c->process == GRA_PROC_RES ? 1 : 0) || c->n_res > GRA_MAX_RES)
       return set_global_error(GRA_E_ARG, "n_res %d outside %d..%d", c->n_res,
                               c->process == GRA_PROC_RES ? 1 : 0, GRA_MAX_RES);
@@ -739,7 +737,12 @@ int build_eikonal(gra_handle* h, const char* cache_dir) {
 
 }  // namespace
 
-static bool f32_kernels(const RunParams& P) { return P.amp_mode == GRA_AMP_FP32; }
+// The FP32 unit serves the FP32-amplitude mode, except for the resonance
+// processes, whose FP32 screening node (FP64 kinematics, FP32 amplitude) is built
+// in the FP64 unit beside their FP64 tree amplitude
+static bool f32_kernels(const RunParams& P) {
+  return P.amp_mode == GRA_AMP_FP32 && P.process != GRA_PROC_RES_CON && P.process != GRA_PROC_RES;
+}
 
 int ensure_point_buffers(gra_handle* h, uint64_t n) {
   const int D = h->P.dim;

@@ -249,3 +249,281 @@ __device__ __forceinline__ cxd res_sum(const RunParams& P, const Kin& K) {
   }
   return A;
 }
+
+// ---------------------------------------------------------------------------
+// The same amplitude in FP32 on FP64 kinematics (the screening node of the
+// FP32-amplitude mode): every quantity formed from the 6.5 TeV beam momenta or
+// sensitive to cancellation (m^2, s_1 = (X + p1)^2, s_2, t1, t2, the rest-frame
+// angles, the photon momentum fractions x = 1 - p_z / p_beam,z) is computed in
+// FP64 by res_kin, and the amplitude from these scalars in FP32, with the
+// transcendentals on the hardware units and FMA contraction on (the unit that
+// includes this file compiles its FP64 code without contraction).
+// ---------------------------------------------------------------------------
+struct ResKin {
+  double m2, M, s1, s2, t1, t2, thq, phq, thd, phd, thR, phR, mA, mB, x1, x2, pt1, pt2;
+};
+__device__ __forceinline__ void res_kin(const RunParams& P, const Kin& K, ResKin& k) {
+  const dv4 X = K.pf[0];
+  k.m2 = m2of(X);
+  k.M = mof(X);
+  k.s1 = m2of(v4add(X, K.pf[1]));
+  k.s2 = m2of(v4add(X, K.pf[2]));
+  k.t1 = K.t1;
+  k.t2 = K.t2;
+  const dv4 q1b = boost_rest(X, k.M, K.q1);
+  theta_phi(q1b, k.thq, k.phq);
+  theta_phi(boost_rest(X, k.M, K.dt[0]), k.thd, k.phd);
+  rho_rotation(P, K, k.M, q1b, k.thR, k.phR);
+  k.mA = mof(K.dt[0]);
+  k.mB = mof(K.dt[1]);
+  k.x1 = 1 - K.pf[1].z / P.pbeam1.z;
+  k.x2 = 1 - K.pf[2].z / P.pbeam2.z;
+  k.pt1 = ptof(K.q1);
+  k.pt2 = ptof(v4sub(P.pbeam2, K.pf[2]));
+}
+
+__device__ __forceinline__ cxf rf_add(cxf a, cxf b) { return cf(a.re + b.re, a.im + b.im); }
+__device__ __forceinline__ cxf rf_scale(cxf a, float s) { return cf(a.re * s, a.im * s); }
+__device__ __forceinline__ float rf_pow(float x, float y) {
+  return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+}
+// x^n for a small non-negative integer n
+__device__ __forceinline__ float rf_ipow(float x, int n) {
+  float r = 1.f;
+  for (int i = 0; i < n; ++i) r *= x;
+  return r;
+}
+
+__device__ __forceinline__ float wigner_d32(float c, float s, int m, int mp, int J) {
+#pragma clang fp contract(fast)
+  float acc = 0.f;
+#pragma unroll
+  for (int k = 0; k <= 2 * J; ++k) {
+    if (J - mp - k >= 0 && J + m - k >= 0 && k + mp - m >= 0) {
+      const float sg = ((mp - m + k) & 1) ? -1.f : 1.f;
+      const float v1 = sg / (float)(fact_small(J - mp - k) * fact_small(J + m - k) *
+                                    fact_small(k + mp - m) * fact_small(k));
+      acc += v1 * (rf_ipow(c, 2 * J + m - mp - 2 * k) * rf_ipow(s, mp - m + 2 * k));
+    }
+  }
+  return sqrtf((float)(fact_small(J + m) * fact_small(J - m) * fact_small(J + mp) *
+                       fact_small(J - mp))) *
+         acc;
+}
+struct RotD32 {
+  float d1[3][3], d2[5][5];
+  float phR;
+};
+__device__ __forceinline__ void rot_d32(float thR, float phR, RotD32& D) {
+#pragma clang fp contract(fast)
+  float s, c;
+  __sincosf(0.5f * thR, &s, &c);
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) D.d1[a][j] = wigner_d32(c, s, a - 1, j - 1, 1);
+#pragma unroll
+  for (int a = 0; a < 5; ++a)
+#pragma unroll
+    for (int j = 0; j < 5; ++j) D.d2[a][j] = wigner_d32(c, s, a - 2, j - 2, 2);
+  D.phR = phR;
+}
+template <int J, int NROW>
+__device__ __forceinline__ float spin_trace32(const ResDev& R, const int* lam, const float* T,
+                                              float th, float ph, const RotD32& D) {
+#pragma clang fp contract(fast)
+  constexpr int n = 2 * J + 1;
+  float s, c, sd, cd_;
+  __sincosf(0.5f * th, &s, &c);
+  __sincosf(ph - D.phR, &sd, &cd_);
+  cxf ph_[n];
+  ph_[J] = cf(1.f, 0.f);
+#pragma unroll
+  for (int k = 1; k <= J; ++k) {
+    ph_[J + k] = cfmul(ph_[J + k - 1], cf(cd_, sd));
+    ph_[J - k] = cf(ph_[J + k].re, -ph_[J + k].im);
+  }
+  float tr = 0.f;
+#pragma unroll
+  for (int r = 0; r < NROW; ++r) {
+    cxf f[n];
+#pragma unroll
+    for (int j = 0; j < n; ++j) f[j] = rf_scale(ph_[j], T[r] * wigner_d32(c, s, lam[r], j - J, J));
+    cxf g[n];
+#pragma unroll
+    for (int a = 0; a < n; ++a) {
+      cxf acc = cf(0.f, 0.f);
+#pragma unroll
+      for (int j = 0; j < n; ++j) acc = rf_add(acc, rf_scale(f[j], J == 1 ? D.d1[a][j] : D.d2[a][j]));
+      g[a] = acc;
+    }
+#pragma unroll
+    for (int a = 0; a < n; ++a)
+#pragma unroll
+      for (int b = 0; b < n; ++b) {
+        const cxf rab = cf((float)R.rho_re[a * n + b], (float)R.rho_im[a * n + b]);
+        tr += cfmul(rab, cfmul(g[a], cf(g[b].re, -g[b].im))).re;
+      }
+  }
+  return tr;
+}
+__device__ __forceinline__ cxf prod_amp32(const RunParams& P, const ResDev& R, float thq,
+                                          float phq, const RotD32& D) {
+#pragma clang fp contract(fast)
+  const cxf A0 = cf((float)R.g_re, (float)R.g_im);
+  if (!P.spin_gen || R.J == 0) return A0;
+  float tr;
+  if (R.J == 1) {
+    const int lam[3] = {1, 0, -1};
+    const float T[3] = {-0.697f, -0.169f, 0.697f};
+    tr = spin_trace32<1, 3>(R, lam, T, thq, phq, D);
+  } else {
+    const int lam[1] = {0};
+    const float T[1] = {1.f};
+    tr = spin_trace32<2, 1>(R, lam, T, thq, phq, D);
+  }
+  return rf_scale(A0, sqrtf(2.f * R.J + 1.f) * sqrtf(fmaxf(tr, 0.f)));
+}
+__device__ __forceinline__ cxf decay_amp32(const RunParams& P, const ResDev& R, float thd,
+                                           float phd, const RotD32& D) {
+#pragma clang fp contract(fast)
+  const float A0 = (float)R.gdec;
+  if (!P.spin_dec) return cf(A0, 0.f);
+  if (R.J == 0) return cf((float)R.T_re * A0, (float)R.T_im * A0);
+  const int lam[1] = {0};
+  const float T[1] = {(float)sqrt(R.T_re * R.T_re + R.T_im * R.T_im)};
+  const float tr = R.J == 1 ? spin_trace32<1, 1>(R, lam, T, thd, phd, D)
+                            : spin_trace32<2, 1>(R, lam, T, thd, phd, D);
+  return cf(A0 * (sqrtf(2.f * R.J + 1.f) * sqrtf(fmaxf(tr, 0.f))), 0.f);
+}
+__device__ __forceinline__ cxf cbw32(const ResDev& R, float m2, float mA, float mB) {
+#pragma clang fp contract(fast)
+  const float M0 = (float)R.M0, W = (float)R.W;
+  const float a = m2 - (float)R.M02;
+  float b, num = -1.f;
+  if (R.BW == GRA_BW_FIXED) {
+    b = M0 * W;
+  } else if (R.BW == GRA_BW_RUNNING) {
+    b = sqrtf(m2) * W;
+  } else if (R.BW == GRA_BW_BARRIER) {
+    const float sp = (mA + mB) * (mA + mB), sm = (mA - mB) * (mA - mB), M02 = (float)R.M02;
+    const float u = sqrtf(fmaxf((m2 - sp) * (m2 - sm), 0.f)) / (2.f * sqrtf(m2));
+    const float d = sqrtf(fmaxf((M02 - sp) * (M02 - sm), 0.f)) / (2.f * M0);
+    b = W * M02 / sqrtf(m2) * rf_ipow(u / d, 2 * R.J + 1);
+  } else {
+    b = M0 * W;
+    const float r = m2 / (float)R.M02;
+    if (R.J == 1) num = -(1.f - r);
+    if (R.J == 2) num = -(7.f / 6.f - (4.f / 3.f) * r + (2.f / 3.f) * (r * r));
+  }
+  const float iden = __builtin_amdgcn_rcpf(a * a + b * b);
+  return cf(num * a * iden, -num * b * iden);
+}
+__device__ __forceinline__ float coh_flux32(float xi, float t, float pt, float mp) {
+#pragma clang fp contract(fast)
+  const float pt2 = pt * pt, xi2 = xi * xi, mp2 = mp * mp, Q2 = fabsf(t);
+  const float tau = Q2 / (4.f * mp2);
+  const float GE = (1.f - 0.24f * tau) / (1.f + tau * (10.98f + tau * (12.82f + 21.97f * tau)));
+  const float GM = 2.792847337f * ((1.f + 0.12f * tau) /
+                                   (1.f + tau * (10.97f + tau * (18.86f + 6.55f * tau))));
+  const float PART1 = (4.f * mp2 * (GE * GE) + Q2 * (GM * GM)) / (4.f * mp2 + Q2);
+  const float DELTA = pt2 / (pt2 + xi2 * mp2);
+  const float f = (float)(0.0072973525693 / GRA_PI * 16.0 * GRA_PI * GRA_PI) *
+                  ((1.f - xi) * (DELTA * DELTA) * PART1 + (xi2 / 4.f) * DELTA * (GM * GM));
+  return f / (xi * pt2);
+}
+__device__ __forceinline__ cxf photo_prop32(const RunParams& P, float s, float t, float B0) {
+#pragma clang fp contract(fast)
+  const float alpha = (float)P.regge_a0[0] + (float)P.regge_ap[0] * t;
+  float sn, cs;
+  __sincosf((float)((-GRA_PI / 2.0) * P.regge_ap[0]) * t, &sn, &cs);
+  const cxf eta = cfmul(cf((float)P.eta0_re[0], (float)P.eta0_im[0]), cf(cs, sn));
+  const float FF = (float)P.gN_P * __expf(0.5f * B0 * t);
+  return rf_scale(eta, rf_pow(s, alpha) * FF);
+}
+__device__ __forceinline__ float s3f32(const RunParams& P, float t) {
+#pragma clang fp contract(fast)
+  return __builtin_amdgcn_rcpf((1.f - t / (float)P.fc1) * (1.f - t / (float)P.fc2));
+}
+__device__ __forceinline__ cxf regge_prop32(const RunParams& P, float s, float t) {
+#pragma clang fp contract(fast)
+  cxf M = cf(0.f, 0.f);
+  const float ls = __builtin_amdgcn_logf(s * (float)(1.0 / P.regge_s0));
+  for (int k = 0; k < 3; ++k) {
+    if (P.regge_on[k]) {
+      const float alpha = (float)P.regge_a0[k] + (float)P.regge_ap[k] * t;
+      float sn, cs;
+      __sincosf((float)((-GRA_PI / 2.0) * P.regge_ap[k]) * t, &sn, &cs);
+      const cxf eta = cfmul(cf((float)P.eta0_re[k], (float)P.eta0_im[k]), cf(cs, sn));
+      M = rf_add(M, rf_scale(eta, __builtin_amdgcn_exp2f(alpha * ls)));
+    }
+  }
+  return M;
+}
+// MRegge::ME4 in FP32 from the FP64 invariants of K
+__device__ __forceinline__ cxf regge_me4_32(const RunParams& P, const Kin& K) {
+#pragma clang fp contract(fast)
+  const float t1 = (float)K.t1, t2 = (float)K.t2;
+  const float FF = (float)P.gN_P, g = (float)P.gpp_P;
+  const float th_ = (float)(K.t_hat - P.m_prop2), tu_ = (float)(K.u_hat - P.m_prop2);
+  float fft, ffu;
+  if (P.offshellFF == 2) {
+    fft = __builtin_amdgcn_rcpf(1.f - th_ / (float)P.b_POW);
+    ffu = __builtin_amdgcn_rcpf(1.f - tu_ / (float)P.b_POW);
+  } else if (P.offshellFF == 0) {
+    fft = __expf(-(float)P.b_EXP * fabsf(th_));
+    ffu = __expf(-(float)P.b_EXP * fabsf(tu_));
+  } else {
+    const float ao = (float)P.a_OREAR, bo = (float)P.b_OREAR;
+    fft = __expf(-bo * sqrtf(fabsf(th_) + ao * ao) + ao * bo);
+    ffu = __expf(-bo * sqrtf(fabsf(tu_) + ao * ao) + ao * bo);
+  }
+  const float cA = FF * s3f32(P, t1), cB = FF * s3f32(P, t2);
+  const cxf At = rf_scale(cfmul(regge_prop32(P, (float)K.ss13, t1), regge_prop32(P, (float)K.ss24, t2)),
+                          cA * cB * (g * g) * (fft * fft) / th_);
+  const cxf Au = rf_scale(cfmul(regge_prop32(P, (float)K.ss14, t1), regge_prop32(P, (float)K.ss23, t2)),
+                          cA * cB * (g * g) * (ffu * ffu) / tu_);
+  return rf_add(At, Au);
+}
+// The resonance sum of res_sum, FP64 kinematics and FP32 amplitude
+__device__ __forceinline__ cxf res_sum32(const RunParams& P, const Kin& K) {
+#pragma clang fp contract(fast)
+  ResKin k;
+  res_kin(P, K, k);
+  const float m2 = (float)k.m2, s1 = (float)k.s1, s2 = (float)k.s2;
+  const float t1 = (float)k.t1, t2 = (float)k.t2;
+  const float FF_A = (float)P.gN_P * s3f32(P, t1), FF_B = (float)P.gN_P * s3f32(P, t2);
+  const cxf pr12 = cfmul(regge_prop32(P, s1, t1), regge_prop32(P, s2, t2));
+  const float V = rf_pow(1.f / m2, (float)P.omega);
+  RotD32 D;
+  rot_d32((float)k.thR, (float)k.phR, D);
+  const float thq = (float)k.thq, phq = (float)k.phq, thd = (float)k.thd, phd = (float)k.phd;
+  const float mA = (float)k.mA, mB = (float)k.mB;
+  const float cont = 2.f * FF_A * FF_B * V;
+  float gf1 = 0.f, gf2 = 0.f;
+  bool flux = false;
+  cxf A = cf(0.f, 0.f);
+  for (int r = 0; r < P.nres; ++r) {
+    const ResDev& R = P.res[r];
+    const cxf bw = cbw32(R, m2, mA, mB);
+    const float dm = m2 - (float)R.M02;
+    const float rff = __expf(-(dm * dm) / (float)R.gFF4);
+    const cxf spin = cfmul(prod_amp32(P, R, thq, phq, D), decay_amp32(P, R, thd, phd, D));
+    if (R.photo) {
+      if (!flux) {
+        const float x1 = (float)k.x1, x2 = (float)k.x2, mp = (float)P.tp_mp_ff;
+        gf1 = sqrtf(fmaxf(coh_flux32(x1, t1, (float)k.pt1, mp) / x1, 0.f));
+        gf2 = sqrtf(fmaxf(coh_flux32(x2, t2, (float)k.pt2, mp) / x2, 0.f));
+        flux = true;
+      }
+      const float B0 = (float)R.B0;
+      const cxf common = rf_scale(bw, rff);
+      const cxf A12 = rf_add(rf_scale(cfmul(common, photo_prop32(P, s1, t1, B0)), gf2),
+                             rf_scale(cfmul(common, photo_prop32(P, s2, t2, B0)), gf1));
+      A = rf_add(A, cfmul(A12, spin));
+    } else {
+      A = rf_add(A, rf_scale(cfmul(cfmul(pr12, bw), spin), cont * rff));
+    }
+  }
+  return A;
+}

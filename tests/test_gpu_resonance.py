@@ -42,6 +42,51 @@ def test_res_con_screened_fp64_pointwise():
     compare(Wg, Wo, fg, fo, 1e-10, "RES+CON screened fp64")
 
 
+def test_res_con_fp32_amplitude():
+    # the FP32-amplitude mode of the resonance processes: tree amplitudes in FP64
+    # (unscreened weights equal the FP64 mode's), screening-loop nodes with FP64
+    # kinematics and an FP32 amplitude, against the FP64 oracle
+    say("case: PP[RES+CON] FP32 amplitude unscreened, 5000 points, seed 44")
+    u = points(5000, 8, 44)
+    c32 = abi.cfg_default(6)
+    c32.amp_precision = abi.GRA_AMP_FP32
+    with MContinuum(c32) as P:
+        W32, _, f32 = P.eval(u)
+    with MContinuum(abi.cfg_default(6)) as P:
+        W64, _, f64 = P.eval(u)
+    assert np.array_equal(f32, f64) and np.array_equal(W32, W64)
+    # Preset 7 (continuum + eight resonances) point by point. Preset 8 (the lone
+    # f2 with |J_z| = 2, no cuts, no continuum) reaches weights 1e-150 below its
+    # largest far from the pole and at the zeros of sin^4(theta_CS) sin^4(theta_q),
+    # below FP32's range: the pointwise bound holds for weights above 1e-8 of the
+    # largest, and the summed deviation is bounded against the summed weight.
+    for preset, n, seed in ((7, 1500, 45), (8, 1500, 47)):
+        cfg = small_tables(abi.cfg_default(preset), 1000)
+        if not cfg.screening:
+            cfg.screening = 1
+        ref = small_tables(abi.cfg_default(preset), 1000)
+        ref.screening = cfg.screening
+        cfg.amp_precision = abi.GRA_AMP_FP32
+        say(f"case: preset {preset} screened, FP32 loop amplitude, {n} points, seed {seed}")
+        u = points(n, cfg.dim, seed)
+        with MContinuum(cfg) as P:
+            Wg, ag, fg = P.eval(u)
+        Wo, ao, fo = ol.OracleRun(ref).eval(u)
+        if preset == 7:
+            rel = compare(Wg, Wo, fg, fo, 1e-4, f"preset {preset} fp32 screened")
+        else:
+            assert np.array_equal(fg, fo)
+            big = Wo > 1e-8 * Wo.max()
+            rel = np.abs(Wg[big] - Wo[big]) / Wo[big]
+            dev = np.abs(Wg - Wo).sum() / Wo.sum()
+            say(f"preset 8 fp32 screened: {big.sum()} of {n} points above 1e-8 of the largest "
+                f"weight: max rel {rel.max():.2e}; sum |dW| / sum W {dev:.2e}")
+            assert rel.max() < 1e-4 and dev < 1e-5
+        say(f"preset {preset} fp32 rel err: median {np.median(rel):.2e} "
+            f"p99 {np.quantile(rel, 0.99):.2e}")
+        assert np.median(rel) < 4e-6
+
+
 def test_res_con_without_resonances_is_the_continuum():
     cfg = abi.cfg_default(6)
     cfg.n_res = 0
