@@ -255,13 +255,38 @@ __device__ __forceinline__ cxd res_sum(const RunParams& P, const Kin& K) {
 // FP32-amplitude mode): every quantity formed from the 6.5 TeV beam momenta or
 // sensitive to cancellation (m^2, s_1 = (X + p1)^2, s_2, t1, t2, the rest-frame
 // angles, the photon momentum fractions x = 1 - p_z / p_beam,z) is computed in
-// FP64 by res_kin, and the amplitude from these scalars in FP32, with the
+// FP64 by res_kin (the rest-frame vectors in FP64, their angles in FP32), and
+// the amplitude from these scalars in FP32, with the
 // transcendentals on the hardware units and FMA contraction on (the unit that
 // includes this file compiles its FP64 code without contraction).
 // ---------------------------------------------------------------------------
 struct ResKin {
-  double m2, M, s1, s2, t1, t2, thq, phq, thd, phd, thR, phR, mA, mB, x1, x2, pt1, pt2;
+  double m2, M, s1, s2, t1, t2, mA, mB, x1, x2, pt1, pt2;
+  float thq, phq, thd, phd, thR, phR;
 };
+// M4Vec::Theta / Phi of an FP64 vector in FP32 (the vectors are O(GeV) momenta
+// in the system rest frame, or unit vectors)
+__device__ __forceinline__ void theta_phi32(dv4 p, float& th, float& ph) {
+  const float x = (float)p.x, y = (float)p.y, z = (float)p.z;
+  ph = (x == 0.f && y == 0.f) ? 0.f : atan2f(y, x);
+  th = (x == 0.f && y == 0.f && z == 0.f) ? 0.f : atan2f(sqrtf(x * x + y * y), z);
+}
+// spin::GetRhoRotation with the boosts in FP64 and the angles in FP32
+__device__ __forceinline__ void rho_rotation32(const RunParams& P, const Kin& K, double M, dv4 q1b,
+                                               float& thR, float& phR) {
+  thR = 0.f;
+  phR = 0.f;
+  if (P.spin_frame == GRA_FRAME_HX) {
+    theta_phi32(K.pf[0], thR, phR);
+  } else if (P.spin_frame == GRA_FRAME_CS) {
+    const dv4 b1 = boost_rest(K.pf[0], M, P.pbeam1), b2 = boost_rest(K.pf[0], M, P.pbeam2);
+    const double n1 = sqrt(p3mod2(b1)), n2 = sqrt(p3mod2(b2));
+    theta_phi32(v4m(b1.x / n1 - b2.x / n2, b1.y / n1 - b2.y / n2, b1.z / n1 - b2.z / n2, 0.0),
+                thR, phR);
+  } else if (P.spin_frame == GRA_FRAME_GJ) {
+    theta_phi32(q1b, thR, phR);
+  }
+}
 __device__ __forceinline__ void res_kin(const RunParams& P, const Kin& K, ResKin& k) {
   const dv4 X = K.pf[0];
   k.m2 = m2of(X);
@@ -271,9 +296,9 @@ __device__ __forceinline__ void res_kin(const RunParams& P, const Kin& K, ResKin
   k.t1 = K.t1;
   k.t2 = K.t2;
   const dv4 q1b = boost_rest(X, k.M, K.q1);
-  theta_phi(q1b, k.thq, k.phq);
-  theta_phi(boost_rest(X, k.M, K.dt[0]), k.thd, k.phd);
-  rho_rotation(P, K, k.M, q1b, k.thR, k.phR);
+  theta_phi32(q1b, k.thq, k.phq);
+  theta_phi32(boost_rest(X, k.M, K.dt[0]), k.thd, k.phd);
+  rho_rotation32(P, K, k.M, q1b, k.thR, k.phR);
   k.mA = mof(K.dt[0]);
   k.mB = mof(K.dt[1]);
   k.x1 = 1 - K.pf[1].z / P.pbeam1.z;
@@ -496,8 +521,8 @@ __device__ __forceinline__ cxf res_sum32(const RunParams& P, const Kin& K) {
   const cxf pr12 = cfmul(regge_prop32(P, s1, t1), regge_prop32(P, s2, t2));
   const float V = rf_pow(1.f / m2, (float)P.omega);
   RotD32 D;
-  rot_d32((float)k.thR, (float)k.phR, D);
-  const float thq = (float)k.thq, phq = (float)k.phq, thd = (float)k.thd, phd = (float)k.phd;
+  rot_d32(k.thR, k.phR, D);
+  const float thq = k.thq, phq = k.phq, thd = k.thd, phd = k.phd;
   const float mA = (float)k.mA, mB = (float)k.mB;
   const float cont = 2.f * FF_A * FF_B * V;
   float gf1 = 0.f, gf2 = 0.f;
