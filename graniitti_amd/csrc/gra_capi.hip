@@ -691,9 +691,12 @@ int build_eikonal(gra_handle* h, const char* cache_dir) {
   std::vector<double> gx, gw;
   if (c.loop_rule == GRA_LOOP_GAUSS) gauss_legendre(NK + 1, gx, gw);
   h->nodes.resize((size_t)(NP + 1) * (NK + 1));
+  std::vector<NodeRow> rows((size_t)NP + 1);
   for (int i = 0; i <= NP; ++i) {
     const bool gauss = c.loop_rule == GRA_LOOP_GAUSS;
     const double phi = gauss ? 2.0 * GRA_PI * i / (NP + 1) : 0.0 + i * P.step_phi;
+    rows[i].c = (float)std::cos(phi);
+    rows[i].s = (float)std::sin(phi);
     for (int j = 0; j <= NK; ++j) {
       const double half = 0.5 * (c.max_loop_kt - c.min_loop_kt);
       const double kt = gauss ? c.min_loop_kt + half * (1.0 + gx[j]) : c.min_loop_kt + j * P.step_kt;
@@ -721,17 +724,17 @@ int build_eikonal(gra_handle* h, const char* cache_dir) {
     const NodeConst& nd = h->nodes[k];
     const double cre = nd.w * nd.kt * nd.aeik_re, cim = nd.w * nd.kt * nd.aeik_im;
     NodeF32& f = n32[k];
-    f.kx = nd.kxf;
-    f.ky = nd.kyf;
+    f.kt = (float)nd.kt;
     f.k2 = nd.k2f;
     f.arg = (float)(std::atan2(cim, cre) / (2.0 * GRA_PI));
     f.mod = (float)std::hypot(cre, cim);
-    f.pad[0] = f.pad[1] = f.pad[2] = 0.f;
   }
-  const size_t b0 = sizeof(NodeConst) * NN, b1 = sizeof(NodeF32) * NN;
-  if (h->d_nodes.ensure(b0 + b1)) return fail(h, GRA_E_NOMEM, "node table");
+  const size_t b0 = sizeof(NodeConst) * NN, b1 = sizeof(NodeF32) * NN,
+               b2 = sizeof(NodeRow) * rows.size();
+  if (h->d_nodes.ensure(b0 + b1 + b2)) return fail(h, GRA_E_NOMEM, "node table");
   GRA_HIP_CHECK(h, hipMemcpy(h->d_nodes.p, h->nodes.data(), b0, hipMemcpyHostToDevice));
   GRA_HIP_CHECK(h, hipMemcpy((char*)h->d_nodes.p + b0, n32.data(), b1, hipMemcpyHostToDevice));
+  GRA_HIP_CHECK(h, hipMemcpy((char*)h->d_nodes.p + b0 + b1, rows.data(), b2, hipMemcpyHostToDevice));
   return GRA_OK;
 }
 

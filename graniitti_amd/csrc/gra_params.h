@@ -87,11 +87,13 @@ struct NodeConst {
 };
 
 // Compact FP32 node of the Regge screening loop (FP32 amplitude kernels). On the
-// device the node table holds the NodeConst entries followed by one NodeF32 per
-// node in the same order.
+// device the node table holds the NodeConst entries, then one NodeF32 per node in
+// the same order, then one NodeRow per phi row (the node's direction).
 struct NodeF32 {
-  float kx, ky, k2;  // kt cos(phi_i), kt sin(phi_i), kt^2
-  float arg;         // arg(w kt A_eik) / (2 pi), in revolutions
-  float mod;         // |w kt A_eik|
-  float pad[3];
+  float kt, k2;  // kt, kt^2
+  float arg;     // arg(w kt A_eik) / (2 pi), in revolutions
+  float mod;     // |w kt A_eik|
+};
+struct NodeRow {
+  float c, s;  // cos(phi_i), sin(phi_i)
 };
