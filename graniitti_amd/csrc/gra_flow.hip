@@ -304,13 +304,16 @@ __global__ __launch_bounds__(256) void k_rqs_backward(
 // G_m = G_X (m < k), G_w (m = k), 0 (m > k) is
 //   G_X (X_k - k eps) / (1 - K eps) + G_w sw_k,
 // so no per-bin arrays are kept.
-template <int KT>
+template <int KT, bool DIRECT>
 __global__ __launch_bounds__(256) void k_rqs_backward_t(
     const float* __restrict__ prm, int ldp, const float* __restrict__ u_prev,
     const float* __restrict__ g_in, const float* __restrict__ coef, float* __restrict__ g_out,
     float* __restrict__ gp, __bf16* __restrict__ gp16, int n, int D, LayerIdx L, int Kin, int R,
     float* __restrict__ logq_chk) {
-  // KT > 0: compile-time K, logits and their exponentials held in registers
+  // KT > 0: compile-time K, logits and their exponentials held in registers.
+  // DIRECT: each thread reads its parameters from prm and writes its gradients
+  // to gp16 (and gp) itself, without the LDS tile, whose size otherwise limits
+  // the blocks per CU when the parameter rows are long
   const int K = KT > 0 ? KT : Kin;
   extern __shared__ float tile[];  // [R][ldp + 1]
   __shared__ float sLj[256];
@@ -318,7 +321,7 @@ __global__ __launch_bounds__(256) void k_rqs_backward_t(
   const int row0 = blockIdx.x * R;
   // the block's rows are contiguous in prm: walk them with an incremental (row, column)
   const int nrow = min(R, n - row0);
-  {
+  if constexpr (!DIRECT) {
     int r = threadIdx.x / ldp, c = threadIdx.x % ldp;
     const int dr = blockDim.x / ldp, dc = blockDim.x % ldp;
     for (int e = threadIdx.x; e < nrow * ldp; e += blockDim.x) {
@@ -345,7 +348,23 @@ __global__ __launch_bounds__(256) void k_rqs_backward_t(
     if (j == 0)
       for (int i = 0; i < L.dA; ++i) go[L.A[i]] = gi[L.A[i]];
     const float gld = -coef[b];
-    float* p = tile + r * ldt + j * np;
+    const float* p = DIRECT ? prm + (size_t)b * ldp + j * np : tile + r * ldt + j * np;
+    float* pw = DIRECT ? nullptr : tile + r * ldt + j * np;
+    // gradient of parameter m of this spline
+    auto put = [&](int m, float v) {
+      if constexpr (DIRECT) {
+        const size_t o = (size_t)b * ldp + j * np + m;
+        gp16[o] = (__bf16)v;
+        if (gp) gp[o] = v;
+      } else {
+        pw[m] = v;
+      }
+    };
+    if (DIRECT && j == 0)
+      for (int c = L.dB * np; c < ldp; ++c) {
+        gp16[(size_t)b * ldp + c] = (__bf16)0.f;
+        if (gp) gp[(size_t)b * ldp + c] = 0.f;
+      }
     const float x = fminf(fmaxf(up[L.B[j]], 0.f), 1.f);
     float ew[KT > 0 ? KT : 1], eh[KT > 0 ? KT : 1];
     float mw, mh, iw, ih;
@@ -464,15 +483,15 @@ __global__ __launch_bounds__(256) void k_rqs_backward_t(
       }
       const float Gwm = (m < k) ? G_X : (m == k ? G_w : 0.f);
       const float Ghm = (m < k) ? G_Y : (m == k ? G_h : 0.f);
-      p[m] = cw * swm * (Gwm - sdw);
-      p[K + m] = cw * shm * (Ghm - sdh);
+      put(m, cw * swm * (Gwm - sdw));
+      put(K + m, cw * shm * (Ghm - sdh));
     }
     for (int m = 0; m < K - 1; ++m)
-      p[2 * K + m] = (m == k - 1 ? G_dk * sgk : 0.f) + (m == k ? G_dk1 * sgk1 : 0.f);
+      put(2 * K + m, (m == k - 1 ? G_dk * sgk : 0.f) + (m == k ? G_dk1 * sgk1 : 0.f));
   }
   sLj[t] = lj;
   __syncthreads();
-  {
+  if constexpr (!DIRECT) {
     int rr = threadIdx.x / ldp, c = threadIdx.x % ldp;
     const int dr = blockDim.x / ldp, dc = blockDim.x % ldp;
     for (int e2 = threadIdx.x; e2 < nrow * ldp; e2 += blockDim.x) {
@@ -688,6 +707,122 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const void* __restrict__ Av, in
       if (col >= ep.ncols_valid) v = 0.f;
       if (ep.out16) ep.out16[(size_t)row * ep.ldc + col] = (__bf16)v;
       if (ep.out32) ep.out32[(size_t)row * ep.ldc + col] = v;
+    }
+  }
+}
+
+// BF16 C[M x N] = A[M x K] Bt[N x K]^T with every output column in one block of
+// 128 rows: each wave takes 32 rows (two m-tiles) x all 16 NT columns, so A is
+// read once, and each k-step's slice of Bt (N x 32) is staged once per block in
+// LDS (double-buffered: the next slice is loaded into registers while the
+// current one feeds the MFMAs). The epilogue of k_gemm_nt, 16 rows x 64
+// columns at a time through LDS (over the B buffers once the products are done).
+template <int NT>
+__global__ __launch_bounds__(256) void k_gemm_nt_wide(const __bf16* __restrict__ A, int lda,
+                                                      const __bf16* __restrict__ Bt, int ldb,
+                                                      int K, Epi ep) {
+  constexpr int N = 16 * NT, kLd = 40;        // B slice row stride (32 + 8) in BF16
+  constexpr int kChunks = N * 4 / 256;        // 16-byte pieces of a slice per thread
+  // two B slices, or the four waves' 16 x 68 FP32 epilogue tiles, whichever is larger
+  constexpr int kBytes = 2 * N * kLd * 2 > 4 * 16 * 68 * 4 ? 2 * N * kLd * 2 : 4 * 16 * 68 * 4;
+  __shared__ __attribute__((aligned(16))) unsigned char smem_nt[kBytes];
+  __bf16 (*sB)[N * kLd] = (__bf16 (*)[N * kLd])smem_nt;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int m0 = blockIdx.x * 128 + wave * 32;
+  const int kq = 8 * (lane >> 4);
+  f32x4 acc[2][NT];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[i][t] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  bf16x8 pre[kChunks];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int c = 0; c < kChunks; ++c) {
+      const int q = threadIdx.x + 256 * c, n = q >> 2, seg = q & 3;
+      pre[c] = *(const bf16x8*)(Bt + (size_t)n * ldb + k0 + 8 * seg);
+    }
+  };
+  auto stash = [&](int buf) {
+#pragma unroll
+    for (int c = 0; c < kChunks; ++c) {
+      const int q = threadIdx.x + 256 * c, n = q >> 2, seg = q & 3;
+      *(bf16x8*)(&sB[buf][n * kLd + 8 * seg]) = pre[c];
+    }
+  };
+  fetch(0);
+  stash(0);
+  __syncthreads();
+  int cur = 0;
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    const bool more = k0 + 32 < K;
+    if (more) fetch(k0 + 32);
+    bf16x8 a[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+      a[i] = *(const bf16x8*)(A + (size_t)(m0 + 16 * i + (lane & 15)) * lda + k0 + kq);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const bf16x8 bb = *(const bf16x8*)(&sB[cur][(16 * t + (lane & 15)) * kLd + kq]);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+        acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], bb, acc[i][t], 0, 0, 0);
+    }
+    if (more) stash(cur ^ 1);
+    __syncthreads();
+    cur ^= 1;
+  }
+  // epilogue: the wave's 16 x 64 tiles through its own slice of the B buffers
+  float* sT = (float*)smem_nt + wave * (16 * 68);
+  const int rr = lane >> 2, c0 = 16 * (lane & 3);
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const size_t row = (size_t)(m0 + 16 * i + rr);
+#pragma unroll
+    for (int ch = 0; ch < NT / 4; ++ch) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          sT[(4 * (lane >> 4) + r) * 68 + 16 * t + (lane & 15)] = acc[i][4 * ch + t][r];
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const int col0 = 64 * ch + c0;
+      float v[16];
+#pragma unroll
+      for (int c = 0; c < 16; ++c) {
+        v[c] = sT[rr * 68 + c0 + c];
+        if (ep.bias) v[c] += ep.bias[col0 + c];
+        if (ep.relu) v[c] = fmaxf(v[c], 0.f);
+      }
+      if (ep.mask) {
+        bf16x8 mk[2];
+        mk[0] = *(const bf16x8*)(ep.mask + row * ep.ldm + col0);
+        mk[1] = *(const bf16x8*)(ep.mask + row * ep.ldm + col0 + 8);
+#pragma unroll
+        for (int c = 0; c < 16; ++c)
+          if (!((float)mk[c >> 3][c & 7] > 0.f)) v[c] = 0.f;
+      }
+#pragma unroll
+      for (int c = 0; c < 16; ++c)
+        if (col0 + c >= ep.ncols_valid) v[c] = 0.f;
+      if (ep.out16) {
+        bf16x8 o[2];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) o[c >> 3][c & 7] = (__bf16)v[c];
+        *(bf16x8*)(ep.out16 + row * ep.ldc + col0) = o[0];
+        *(bf16x8*)(ep.out16 + row * ep.ldc + col0 + 8) = o[1];
+      }
+      if (ep.out32) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          *(f32x4*)(ep.out32 + row * ep.ldc + col0 + 4 * q) =
+              (f32x4){v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]};
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
   }
 }
@@ -1371,8 +1506,9 @@ static int wgrad_chunk(int n, int N) {
   const long c = (long)n * (N / 64) / 1024;
   return (int)std::max(512L, std::min(4096L, (c / 32) * 32));
 }
-// m-tiles per wave of k_wgrad (0: no instance for this width)
-static int wgrad_mw(int Kin) { return Kin == 64 ? 2 : Kin == 128 ? 3 : Kin == 256 ? 5 : 0; }
+// m-tiles per wave of k_wgrad (0: no instance for this width; at 256 the
+// 64 x 64-tile kernel with split rows is faster)
+static int wgrad_mw(int Kin) { return Kin == 64 ? 2 : Kin == 128 ? 3 : 0; }
 
 static int flow_alloc_batch(gra_handle* h, FlowState* F, int n) {
   if (n <= F->cap) return GRA_OK;
@@ -1442,6 +1578,21 @@ static int refresh_weights(gra_handle* h, FlowState* F) {
 
 static void gemm_nt(gra_handle* h, FlowState* F, const void* A, int lda, const void* Bt, int ldb,
                     int M, int N, int K, const Epi& ep) {
+  if (!F->mlp_fp32 && (ep.ldc & 7) == 0 && (ep.ldm & 7) == 0 && M % 128 == 0 && K % 32 == 0 &&
+      (N == 64 || N == 128 || N == 256)) {
+    const __bf16* Ab = (const __bf16*)A;
+    const __bf16* Bb = (const __bf16*)Bt;
+    if (N == 64)
+      hipLaunchKernelGGL(k_gemm_nt_wide<4>, dim3(M / 128), dim3(256), 0, h->stream, Ab, lda, Bb,
+                         ldb, K, ep);
+    else if (N == 128)
+      hipLaunchKernelGGL(k_gemm_nt_wide<8>, dim3(M / 128), dim3(256), 0, h->stream, Ab, lda, Bb,
+                         ldb, K, ep);
+    else
+      hipLaunchKernelGGL(k_gemm_nt_wide<16>, dim3(M / 128), dim3(256), 0, h->stream, Ab, lda, Bb,
+                         ldb, K, ep);
+    return;
+  }
   const dim3 grid(M / 64, N / 64);
   if (F->mlp_fp32)
     hipLaunchKernelGGL(k_gemm_nt<false>, grid, dim3(256), 0, h->stream, A, lda, Bt, ldb, M, N, K, ep);
@@ -1467,12 +1618,8 @@ static void weight_grad(gra_handle* h, FlowState* F, const void* X, int ldx, con
         hipLaunchKernelGGL(k_wgrad<2>, grid, dim3(256), lds, h->stream, Xb, ldx, Yb, ldy, Kin, N, n,
                            chunk, part);
         break;
-      case 3:
-        hipLaunchKernelGGL(k_wgrad<3>, grid, dim3(256), lds, h->stream, Xb, ldx, Yb, ldy, Kin, N, n,
-                           chunk, part);
-        break;
       default:
-        hipLaunchKernelGGL(k_wgrad<5>, grid, dim3(256), lds, h->stream, Xb, ldx, Yb, ldy, Kin, N, n,
+        hipLaunchKernelGGL(k_wgrad<3>, grid, dim3(256), lds, h->stream, Xb, ldx, Yb, ldy, Kin, N, n,
                            chunk, part);
     }
     const int tot = (Kin + 1) * N;
@@ -1549,6 +1696,12 @@ static int bwd_rows(const FlowState* F, int dB) {
   const int by_lds = (int)((159 * 1024) / (4 * (size_t)(F->ldp + 1)));
   return std::max(1, std::min(256 / dB, by_lds));
 }
+// The spline backward reads and writes its parameter rows directly (no LDS
+// tile) when the tile of a full block would exceed 64 KB, i.e. allow fewer than
+// three blocks per CU (long rows: many bins and transformed dimensions)
+static bool bwd_direct(const FlowState* F, int dB) {
+  return sizeof(float) * (size_t)(256 / dB) * (F->ldp + 1) > (64u << 10);
+}
 
 // Kernel instances of the fused layer by (hidden, bins) and of the spline
 // backward by bins: 8, 16 and 32 bins run the unrolled register forms, other
@@ -1566,13 +1719,17 @@ static const void* layer_inverse_kernel(int H, int K) {
   return H == 64 ? layer_inverse_kernel_h<64>(K)
                  : (H == 128 ? layer_inverse_kernel_h<128>(K) : layer_inverse_kernel_h<256>(K));
 }
-static const void* spline_backward_kernel(int K) {
+template <bool DIRECT>
+static const void* spline_backward_kernel_d(int K) {
   switch (K) {
-    case 8: return (const void*)k_rqs_backward_t<8>;
-    case 16: return (const void*)k_rqs_backward_t<16>;
-    case 32: return (const void*)k_rqs_backward_t<32>;
-    default: return (const void*)k_rqs_backward_t<0>;
+    case 8: return (const void*)k_rqs_backward_t<8, DIRECT>;
+    case 16: return (const void*)k_rqs_backward_t<16, DIRECT>;
+    case 32: return (const void*)k_rqs_backward_t<32, DIRECT>;
+    default: return (const void*)k_rqs_backward_t<0, DIRECT>;
   }
+}
+static const void* spline_backward_kernel(int K, bool direct) {
+  return direct ? spline_backward_kernel_d<true>(K) : spline_backward_kernel_d<false>(K);
 }
 
 // Bin count that selects the kernel instance: 0 (the runtime-K loops) when the
@@ -1682,8 +1839,9 @@ static int flow_backward(gra_handle* h, FlowState* F, int n, float* logq_chk) {
     const FlowLayerHost& Ly = F->layers[l];
     // g = dL/du_{l+1}; spline backward gives dL/dprm and dL/du_l (B dims; A passes through)
     {
-      const int R = bwd_rows(F, Ly.idx.dB);
-      const size_t lds = sizeof(float) * (size_t)R * (ldp + 1);
+      const bool direct = bwd_direct(F, Ly.idx.dB);
+      const int R = direct ? 256 / Ly.idx.dB : bwd_rows(F, Ly.idx.dB);
+      const size_t lds = direct ? 0 : sizeof(float) * (size_t)R * (ldp + 1);
       const float* a_prm = prm_of(F, l);
       const float* a_up = u_of(F, l);
       const float* a_gin = g;
@@ -1697,8 +1855,8 @@ static int flow_backward(gra_handle* h, FlowState* F, int n, float* logq_chk) {
                       (void*)&a_coef, (void*)&g2,   (void*)&a_gp,   (void*)&a_gp16,
                       (void*)&nn,     (void*)&DD,   (void*)&idx,    (void*)&KK,
                       (void*)&RR,     (void*)&logq_chk};
-      GRA_HIP_CHECK(h, hipLaunchKernel(spline_backward_kernel(spline_k(F)), dim3((n + R - 1) / R),
-                                       dim3(256), args, lds, h->stream));
+      GRA_HIP_CHECK(h, hipLaunchKernel(spline_backward_kernel(spline_k(F), direct),
+                                       dim3((n + R - 1) / R), dim3(256), args, lds, h->stream));
     }
     // MLP backward from the output layer down
     const float* dy32 = F->gprm.as<float>();
@@ -1959,7 +2117,7 @@ extern "C" int gra_hip_flow_init(gra_handle* h, const gra_flow_cfg* fc_in) {
     size_t bl = 0;
     for (int l = 0; l < F->L; ++l)
       bl = std::max(bl, sizeof(float) * (size_t)bwd_rows(F, F->layers[l].idx.dB) * (F->ldp + 1));
-    if (hipFuncSetAttribute(spline_backward_kernel(spline_k(F)),
+    if (hipFuncSetAttribute(spline_backward_kernel(spline_k(F), false),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bl) != hipSuccess) {
       flow_release(F);
       return fail(h, GRA_E_HIP, "flow: cannot reserve %zu B of LDS for the spline backward", bl);
