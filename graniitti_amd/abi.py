@@ -106,6 +106,7 @@ class GraFlowCfg(C.Structure):
         ("dim", I32), ("layers", I32), ("bins", I32), ("hidden", I32), ("depth", I32),
         ("steps", I32), ("batch", U64), ("lr", D), ("beta1", D), ("beta2", D), ("eps", D),
         ("seed", U64), ("ess_fallback", D), ("spline_loops", I32), ("out_groups", I32),
+        ("wgrad_split", I32), ("pad1", I32),
     ]
 
 

@@ -1004,6 +1004,171 @@ __global__ __launch_bounds__(256) void k_wgrad(const __bf16* __restrict__ X, int
   }
 }
 
+// Weight-gradient partials of a BF16 product in output tiles of TM x TN, one block
+// per (row chunk, tile): part[ch][m][n] = sum_{b in chunk} X'[b][m] dY[b][n].
+// Plain form: X' = X [rows x Kin]; the blocks of m-tile 0 also multiply their dY
+// operands by an all-ones operand, which gives row m = Kin, the bias gradient
+// sum_b dY[b][n]. Gather form (the input layer, TM = 16): X'[b][m] = u[b][A[m]]
+// for m < dA, 1 for m = dA (the bias row), 0 above.
+// Each step of 64 rows stages the X' slab [64 x TM] and the dY slab [64 x TN] in
+// LDS column-major (a column's 64 rows contiguous), so that every MFMA operand
+// is one 16-byte LDS read: a thread loads an 8 x 8 block as 8 row pieces of 16
+// bytes, transposes it in registers and stores 8 column pieces; the next step's
+// blocks are loaded while the current step multiplies. Wave (wm, wn) computes
+// MI x NI MFMA tiles of 16 x 16. The hardware hands consecutive block ids to the
+// 8 XCDs in turn; the blocks are renumbered so that the tiles of one row chunk
+// run together on one XCD and share its L2.
+template <int MI, int NI, int WM, int WN, bool GATHER>
+__global__ __launch_bounds__(256, 2) void k_wgrad_tile(
+    const __bf16* __restrict__ X, int ldx, const float* __restrict__ ug, int D, LayerIdx L,
+    const __bf16* __restrict__ Y, int ldy, int Kin, int N, int nrows, int chunk, int tiles_m,
+    int ntile, int nblk, int kin_t, float* __restrict__ part) {
+  constexpr int TM = 16 * MI * WM, TN = 16 * NI * WN, kLd = 72;  // kLd: 64 rows + 8
+  constexpr int NB = NI / WM;  // n-tiles of the bias row per wave
+  static_assert(WM * WN == 4 && TN <= 256 && (GATHER ? TM == 16 : TM + TN <= 256 && NB * WM == NI),
+                "tile shape");
+  __shared__ __attribute__((aligned(16))) __bf16 sx[TM * kLd];
+  __shared__ __attribute__((aligned(16))) __bf16 sy[TN * kLd];
+  // the grid is a multiple of 8 blocks; block b runs on XCD b % 8
+  const int per = (int)(gridDim.x / 8);
+  const int lb = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if (lb >= nblk) return;  // block-uniform
+  const int tile = lb % ntile, ch = lb / ntile;
+  const int m0 = (tile % tiles_m) * TM, n0 = (tile / tiles_m) * TN;
+  const int b0 = ch * chunk, b1 = min(nrows, b0 + chunk);
+  // the wave index in a scalar register: wave-uniform branches, no per-lane selects
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  const int wm = wave % WM, wn = wave / WM;
+  // staging roles. Plain form: 8 x 8 block q of the X slab (q < TM) or of the dY
+  // slab. Gather form: every thread gathers 4 values of the X' slab (row q & 63,
+  // columns 4 (q >> 6) ..), and threads q < TN load block q of the dY slab.
+  const int q = threadIdx.x;
+  const bool stx = !GATHER && q < TM;
+  const bool sty = GATHER ? q < TN : !stx && q < TM + TN;
+  const int qq = (GATHER || stx) ? q : q - TM;
+  const int rb = qq & 7, cb = qq >> 3;  // rows 8 rb .. 8 rb + 7, columns 8 cb .. 8 cb + 7
+  bf16x8 v[8];
+  float gx[4];
+  auto load = [&](int bb) {
+    if constexpr (GATHER) {
+      const int b = bb + (q & 63);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int m = 4 * (q >> 6) + e;
+        float x = 0.f;
+        if (b < b1) x = m < L.dA ? ug[(size_t)b * D + L.A[m]] : (m == L.dA ? 1.f : 0.f);
+        gx[e] = x;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int b = bb + 8 * rb + j;
+      bf16x8 z;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) z[e] = (__bf16)0.f;
+      if (stx) {
+        const int c = m0 + 8 * cb;
+        if (b < b1 && c < Kin) z = *(const bf16x8*)(X + (size_t)b * ldx + c);
+      } else if (sty) {
+        const int c = n0 + 8 * cb;
+        if (b < b1 && c < N) z = *(const bf16x8*)(Y + (size_t)b * ldy + c);
+      }
+      v[j] = z;
+    }
+  };
+  auto store = [&]() {
+    if constexpr (GATHER) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) sx[(4 * (q >> 6) + e) * kLd + (q & 63)] = (__bf16)gx[e];
+    }
+    if (!stx && !sty) return;
+    __bf16* s = stx ? sx : sy;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      bf16x8 o;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) o[j] = v[j][e];
+      *(bf16x8*)(s + (8 * cb + e) * kLd + 8 * rb) = o;
+    }
+  };
+  f32x4 acc[MI][NI];
+#pragma unroll
+  for (int i = 0; i < MI; ++i)
+#pragma unroll
+    for (int t = 0; t < NI; ++t) acc[i][t] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  f32x4 accb[NB > 0 ? NB : 1];
+#pragma unroll
+  for (int u = 0; u < (NB > 0 ? NB : 1); ++u) accb[u] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  bf16x8 ones;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) ones[e] = (__bf16)1.f;
+  const int mrows = GATHER ? L.dA + 1 : Kin;
+  const bool active = m0 + 16 * MI * wm < mrows && n0 + 16 * NI * wn < N;
+  const bool bias = !GATHER && m0 == 0;
+  const int kq = 8 * (lane >> 4), r16 = lane & 15;
+  load(b0);
+  for (int bb = b0; bb < b1; bb += 64) {
+    store();
+    __syncthreads();
+    if (bb + 64 < b1) load(bb + 64);
+    if (active) {
+#pragma unroll
+      for (int ks = 0; ks < 64; ks += 32) {
+        bf16x8 a[MI], bv[NI];
+#pragma unroll
+        for (int i = 0; i < MI; ++i)
+          a[i] = *(const bf16x8*)(sx + (16 * (MI * wm + i) + r16) * kLd + ks + kq);
+#pragma unroll
+        for (int t = 0; t < NI; ++t)
+          bv[t] = *(const bf16x8*)(sy + (16 * (NI * wn + t) + r16) * kLd + ks + kq);
+#pragma unroll
+        for (int i = 0; i < MI; ++i)
+#pragma unroll
+          for (int t = 0; t < NI; ++t)
+            acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], bv[t], acc[i][t], 0, 0, 0);
+        if constexpr (!GATHER) {
+          // n-tiles NB wm .. NB wm + NB - 1 of the wave's dY operands (constant
+          // register indices in each branch)
+          if (bias) {
+#pragma unroll
+            for (int w = 0; w < WM; ++w)
+              if (wm == w) {
+#pragma unroll
+                for (int u = 0; u < NB; ++u)
+                  accb[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ones, bv[NB * w + u], accb[u],
+                                                                    0, 0, 0);
+              }
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (!active) return;
+#pragma unroll
+  for (int i = 0; i < MI; ++i)
+#pragma unroll
+    for (int t = 0; t < NI; ++t) {
+      const int col = n0 + 16 * (NI * wn + t) + r16;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + 16 * (MI * wm + i) + 4 * (lane >> 4) + r;
+        if (row < mrows) part[((size_t)ch * kin_t + row) * N + col] = acc[i][t][r];
+      }
+    }
+  if constexpr (!GATHER) {
+    // row 0 of the all-ones product holds the column sums
+    if (bias && (lane >> 4) == 0) {
+#pragma unroll
+      for (int u = 0; u < NB; ++u) {
+        const int col = n0 + 16 * (NI * wn + NB * wm + u) + r16;
+        part[((size_t)ch * kin_t + Kin) * N + col] = accb[u][0];
+      }
+    }
+  }
+}
+
 // Adam (Kingma & Ba 2015) on the FP32 master parameters; refreshes the bf16 copies
 __global__ void k_adam(float* __restrict__ th, const float* __restrict__ g, float* __restrict__ m,
                        float* __restrict__ v, size_t n, float lr, float b1, float b2, float eps,
@@ -1506,9 +1671,23 @@ static int wgrad_chunk(int n, int N) {
   const long c = (long)n * (N / 64) / 1024;
   return (int)std::max(512L, std::min(4096L, (c / 32) * 32));
 }
-// m-tiles per wave of k_wgrad (0: no instance for this width; at 256 the
-// 64 x 64-tile kernel with split rows is faster)
+// m-tiles per wave of k_wgrad (0: no instance for this width)
 static int wgrad_mw(int Kin) { return Kin == 64 ? 2 : Kin == 128 ? 3 : 0; }
+// Rows per block of k_wgrad_tile: about 1024 blocks over ntile output tiles, a
+// multiple of 64 between 512 and 8192
+static int wgt_chunk(int n, int ntile) {
+  const long c = (long)n * ntile / 1024;
+  return (int)std::max(512L, std::min(8192L, (c / 64) * 64));
+}
+// Output tiles of k_wgrad_tile: 128 x 128 (plain), 16 x 128 (gather)
+static int wgt_tiles(int Kin, int N, bool gather) {
+  return (gather ? 1 : (Kin + 127) / 128) * ((N + 127) / 128);
+}
+// Floats of k_wgrad_tile's partials for n rows
+static size_t wgt_part_floats(int n, int Kin, int N, bool gather) {
+  const int chunk = wgt_chunk(n, wgt_tiles(Kin, N, gather));
+  return (size_t)((n + chunk - 1) / chunk) * (gather ? 16 : Kin + 16) * N;
+}
 
 static int flow_alloc_batch(gra_handle* h, FlowState* F, int n) {
   if (n <= F->cap) return GRA_OK;
@@ -1539,6 +1718,12 @@ static int flow_alloc_batch(gra_handle* h, FlowState* F, int n) {
   for (int cols : {H, ldp})
     wg = std::max(wg, (size_t)((N + wgrad_chunk(N, cols) - 1) / wgrad_chunk(N, cols)) *
                           (size_t)(H + 16) * cols);
+  // k_wgrad_tile's partials: hidden and output products, and the input layer
+  if (H % 128 == 0) {
+    wg = std::max(wg, wgt_part_floats(n, H, H, false));
+    wg = std::max(wg, wgt_part_floats(n, H, ldp, false));
+  }
+  wg = std::max(wg, wgt_part_floats(n, 16, H, true));
   bad |= F->part.ensure(sizeof(float) * std::max(std::max(nch * big, wg),
                                                  (size_t)3 * ((N + 255) / 256) * 2)) != 0;
   bad |= F->tmp.ensure(sizeof(double) * 3 * ((N + 255) / 256) + 64) != 0;
@@ -1602,10 +1787,45 @@ static void gemm_nt(gra_handle* h, FlowState* F, const void* A, int lda, const v
 
 // dW[Kin x N] = X^T dY and the bias gradient db[N] = sum_b dY[b], accumulated
 // deterministically (per-chunk partials, then a fixed-order sum)
+// k_wgrad_tile over n rows, then the fixed-order sum of its partials into gW (kw
+// rows) and gb; gather: the input layer (X' from u through L)
+static void wgrad_tile(gra_handle* h, FlowState* F, const __bf16* X, int ldx, const float* ug,
+                       const LayerIdx& L, const __bf16* Y, int ldy, int Kin, int N, int n,
+                       bool gather, float* gW, float* gb) {
+  const int tiles_m = gather ? 1 : (Kin + 127) / 128;
+  const int ntile = wgt_tiles(Kin, N, gather);
+  // chunks sized for the allocated batch, so the partials fit for every n <= cap
+  const int chunk = wgt_chunk(F->cap, ntile);
+  const int nch = (n + chunk - 1) / chunk;
+  const int nblk = nch * ntile;
+  const int kin_t = gather ? 16 : Kin + 16;
+  const unsigned grid = (unsigned)((nblk + 7) / 8 * 8);
+  float* part = F->part.as<float>();
+  if (gather)
+    hipLaunchKernelGGL((k_wgrad_tile<1, 2, 1, 4, true>), dim3(grid), dim3(256), 0, h->stream,
+                       nullptr, 0, ug, F->D, L, Y, ldy, Kin, N, n, chunk, tiles_m, ntile, nblk,
+                       kin_t, part);
+  else
+    hipLaunchKernelGGL((k_wgrad_tile<4, 4, 2, 2, false>), dim3(grid), dim3(256), 0, h->stream, X,
+                       ldx, nullptr, 0, L, Y, ldy, Kin, N, n, chunk, tiles_m, ntile, nblk, kin_t,
+                       part);
+  const int kw = gather ? L.dA : Kin;
+  const int tot = (kw + 1) * N;
+  hipLaunchKernelGGL(k_part_reduce, dim3((tot + 31) / 32), dim3(256), 0, h->stream, part, nch,
+                     kin_t, N, kw, gW, gb);
+}
+
 static void weight_grad(gra_handle* h, FlowState* F, const void* X, int ldx, const void* Y, int ldy,
                         int Kin, int N, int n, float* gW, float* gb) {
+  if (!F->mlp_fp32 && !F->fc.wgrad_split && Kin % 128 == 0) {
+    LayerIdx none;
+    std::memset(&none, 0, sizeof(none));
+    wgrad_tile(h, F, (const __bf16*)X, ldx, nullptr, none, (const __bf16*)Y, ldy, Kin, N, n, false,
+               gW, gb);
+    return;
+  }
   if (!F->mlp_fp32 && wgrad_mw(Kin) > 0) {
-    const int chunk = wgrad_chunk(n, N);
+    const int chunk = wgrad_chunk(F->cap, N);  // sized for the allocated batch
     const int nch = (n + chunk - 1) / chunk;
     const int KT = Kin + 16;
     const size_t lds = sizeof(__bf16) * (size_t)(KT + 64) * 40;
@@ -1646,6 +1866,10 @@ static void weight_grad(gra_handle* h, FlowState* F, const void* X, int ldx, con
 // Input-layer weight and bias gradients: dW1[i][j] = sum_b u[b][A_i] dh[b][j], db1
 static void input_weight_grad(gra_handle* h, FlowState* F, const float* u, const LayerIdx& idx,
                               const void* Y, int ldy, int N, int n, float* gW1, float* gb1) {
+  if (!F->mlp_fp32 && !F->fc.wgrad_split && N % 128 == 0) {
+    wgrad_tile(h, F, nullptr, 0, u, idx, (const __bf16*)Y, ldy, 16, N, n, true, gW1, gb1);
+    return;
+  }
   const int chunk = 2048;
   const int nch = (n + chunk - 1) / chunk;
   const dim3 grid(1, N / 64, nch);

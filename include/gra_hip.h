@@ -271,6 +271,10 @@ typedef struct gra_flow_cfg {
                            (0: unrolled kernels for K = 8, 16, 32; used to check them) */
   int32_t out_groups;   /* transformed dimensions per output group of the fused sampling layer
                            (0: the largest group that fits two blocks per CU in LDS) */
+  int32_t wgrad_split;  /* nonzero: BF16 weight gradients in split 64 x 64 tiles for every width
+                           (0: 128 x 128 tiles for widths that are multiples of 128, and for
+                           the input layer; used to check them) */
+  int32_t pad1;
 } gra_flow_cfg;
 
 typedef struct gra_flow_stats {

@@ -130,6 +130,49 @@ def test_flow_forward_matches_reference(proc, fp32, tol, lqtol, D, L, K, H, dept
     assert np.all((u >= 0) & (u < 1))
 
 
+@pytest.mark.parametrize("D,L,K,H,depth,scale,n", [
+    (8, 6, 16, 128, 2, 0.05, 8192),
+    (14, 4, 32, 256, 3, 0.01, 5056),
+])
+def test_flow_weight_gradient_tiles(proc, D, L, K, H, depth, scale, n):
+    # BF16 weight gradients of the 128 x 128-tile kernel (hidden and output
+    # products, bias rows from an all-ones operand) and of its 16 x 128 input-layer
+    # form, against the split 64 x 64-tile kernel on the same BF16 operands: only
+    # the FP32 summation order differs. n spans several row chunks of 512.
+    say(f"case: BF16 weight-gradient tiles vs split tiles, D={D} L={L} K={K} H={H} n={n}")
+    rng = np.random.default_rng(5)
+    u = rng.random((n, D)).astype(np.float32)
+    c = rng.random(n).astype(np.float32)
+    gs = []
+    P = MContinuum(rho_cfg()) if D == 14 else proc
+    for split in (0, 1):
+        fc = flow_cfg(D, L, K, H, depth, False)
+        fc.wgrad_split = split
+        th = init_random(P, fc, scale, 21)
+        g, _ = grad(P, u, c, len(th))
+        gs.append(g.astype(np.float64))
+    if D == 14:
+        P.close()
+    rel = np.linalg.norm(gs[0] - gs[1]) / np.linalg.norm(gs[1])
+    dmax = np.abs(gs[0] - gs[1]).max() / np.abs(gs[1]).max()
+    say(f"tiles vs split: relative L2 {rel:.2e}, max |dg| / max |g| {dmax:.2e}")
+    assert rel < 2e-5 and dmax < 2e-5
+    if H == 128:
+        # the whole BF16 gradient against float64 autograd: six layers of BF16
+        # operands and BF16 backward products (both kernels: cosine 0.9849)
+        fc.wgrad_split = 0
+        init_random(proc, fc, scale, 21)
+        g, _ = grad(proc, u[:2048], c[:2048], len(th))
+        for rounded in (False, True):
+            gr, _ = FlowRef(th, D, L, K, H, depth, bf16=rounded).grad(
+                u[:2048].astype(np.float64), c[:2048].astype(np.float64))
+            cos = float(g @ gr / (np.linalg.norm(g) * np.linalg.norm(gr)))
+            say(f"tile-kernel gradient, 2048 rows, vs float64 (operands rounded to bf16: "
+                f"{rounded}): cosine {cos:.5f}")
+            if not rounded:
+                assert cos > 0.98
+
+
 @pytest.mark.parametrize("K", [8, 10, 16])
 def test_flow_gradient_matches_reference(proc, K):
     D, L, H, depth = 8, 4, 64, 2
