@@ -37,6 +37,11 @@ constexpr float kEpsW = 1e-3f;  // minimum bin width / height
 constexpr float kEpsD = 1e-3f;  // minimum knot derivative
 constexpr int kMaxK = 32;       // spline bins
 constexpr int kMaxDB = 8;       // transformed dimensions per layer
+// Parameter columns per transformed dimension: the 3K - 1 spline parameters
+// (K width logits, K height logits, K - 1 interior derivative logits) padded to
+// a multiple of 8, so that every dimension's block starts 32-byte aligned in
+// FP32 and 16-byte aligned in BF16. The padding columns are frozen at zero.
+__host__ __device__ constexpr int prm_stride(int K) { return (3 * K - 1 + 7) / 8 * 8; }
 
 __device__ __forceinline__ float softplusf(float x) {
   return x > 20.f ? x : log1pf(__expf(x));
@@ -153,9 +158,9 @@ __global__ __launch_bounds__(256) void k_rqs_inverse(const float* __restrict__ p
   float* uo = u_out + (size_t)b * D;
   if (m < L.dA) uo[L.A[m]] = uin[L.A[m]];
   float acc = 0.f;
-  const int np = 3 * K - 1;
+  const int nps = prm_stride(K);
   for (int j = 0; j < L.dB; ++j) {
-    const float* p = prm + (size_t)b * ldp + j * np;
+    const float* p = prm + (size_t)b * ldp + j * nps;
     BinLane q;
     grp_knots<G>(p, K, m, q);
     const float zv = fminf(fmaxf(uin[L.B[j]], 0.f), 1.f);
@@ -191,9 +196,9 @@ __global__ __launch_bounds__(256) void k_rqs_forward(const float* __restrict__ p
   const float* uin = u_in + (size_t)b * D;
   float* uo = u_out + (size_t)b * D;
   if (m < L.dA) uo[L.A[m]] = uin[L.A[m]];
-  const int np = 3 * K - 1;
+  const int nps = prm_stride(K);
   for (int j = 0; j < L.dB; ++j) {
-    const float* p = prm + (size_t)b * ldp + j * np;
+    const float* p = prm + (size_t)b * ldp + j * nps;
     BinLane q;
     grp_knots<G>(p, K, m, q);
     const float x = fminf(fmaxf(uin[L.B[j]], 0.f), 1.f);
@@ -207,100 +212,14 @@ __global__ __launch_bounds__(256) void k_rqs_forward(const float* __restrict__ p
   }
 }
 
-// Training direction, backward of layer l: from g = dL/du_l and the loss
-// coefficient c_b (dL/dlog q = -c_b) compute dL/d(raw spline params) (fp32 and
-// bf16 copies) and dL/du_{l-1} for the transformed dimensions; the conditioning
-// dimensions pass g through (their MLP-input gradient is added later).
-template <int G>
-__global__ __launch_bounds__(256) void k_rqs_backward(
-    const float* __restrict__ prm, int ldp, const float* __restrict__ u_prev,
-    const float* __restrict__ g_in, const float* __restrict__ coef, float* __restrict__ g_out,
-    float* __restrict__ gp, __bf16* __restrict__ gp16, int n, int D, LayerIdx L, int K,
-    float* __restrict__ logq_chk) {
-  const int m = threadIdx.x & (G - 1);
-  const int b = (int)(((size_t)blockIdx.x * blockDim.x + threadIdx.x) / G);
-  if (b >= n) return;
-  const float* up = u_prev + (size_t)b * D;
-  const float* gi = g_in + (size_t)b * D;
-  float* go = g_out + (size_t)b * D;
-  if (m < L.dA) go[L.A[m]] = gi[L.A[m]];
-  const float gld = -coef[b];
-  const int np = 3 * K - 1;
-  const float cw = 1.f - K * kEpsW;
-  float ljsum = 0.f;
-  for (int j = 0; j < L.dB; ++j) {
-    const float* p = prm + (size_t)b * ldp + j * np;
-    float* g = gp + (size_t)b * ldp + j * np;
-    __bf16* g16 = gp16 + (size_t)b * ldp + j * np;
-    BinLane q;
-    grp_knots<G>(p, K, m, q);
-    const float x = fminf(fmaxf(up[L.B[j]], 0.f), 1.f);
-    const int k = max(grp_count<G>(m < K && q.X <= x) - 1, 0);
-    Bin e;
-    grp_bin<G>(p, K, q, k, e);
-    const float xi = fminf(fmaxf((x - e.X) / e.w, 0.f), 1.f);
-    const float t = xi * (1.f - xi), om = 1.f - xi;
-    const float Dl = e.dk1 + e.dk - 2.f * e.s;
-    const float Dn = e.s + Dl * t;
-    const float N = e.h * (e.s * xi * xi + e.dk * t);
-    const float Pn = e.dk1 * xi * xi + 2.f * e.s * t + e.dk * om * om;
-    ljsum += 2.f * __logf(e.s) + __logf(Pn) - 2.f * __logf(Dn);
-    const float gu = gi[L.B[j]];
-    // partials of N, Dn, Pn with respect to (xi, s, h, dk, dk1)
-    const float N_xi = e.h * (2.f * e.s * xi + e.dk * (1.f - 2.f * xi));
-    const float N_s = e.h * xi * xi, N_h = e.s * xi * xi + e.dk * t, N_dk = e.h * t;
-    const float D_xi = Dl * (1.f - 2.f * xi), D_s = 1.f - 2.f * t, D_dk = t, D_dk1 = t;
-    const float P_xi = 2.f * e.dk1 * xi + 2.f * e.s * (1.f - 2.f * xi) - 2.f * e.dk * om;
-    const float P_s = 2.f * t, P_dk = om * om, P_dk1 = xi * xi;
-    const float iD = 1.f / Dn, iD2 = iD * iD, iP = 1.f / Pn;
-    // G_q = gu dz/dq + gld dlogJ/dq
-    const float G_xi = gu * (N_xi * Dn - N * D_xi) * iD2 + gld * (P_xi * iP - 2.f * D_xi * iD);
-    const float G_s = gu * (N_s * Dn - N * D_s) * iD2 + gld * (2.f / e.s + P_s * iP - 2.f * D_s * iD);
-    const float G_h0 = gu * N_h * iD;
-    const float G_dk = gu * (N_dk * Dn - N * D_dk) * iD2 + gld * (P_dk * iP - 2.f * D_dk * iD);
-    const float G_dk1 = gu * (-N * D_dk1) * iD2 + gld * (P_dk1 * iP - 2.f * D_dk1 * iD);
-    const float G_Y = gu;
-    // chain through xi = (x - X) / w and s = h / w
-    const float iw = 1.f / e.w;
-    const float G_X = -G_xi * iw;
-    const float G_w = -G_xi * xi * iw - G_s * e.s * iw;
-    const float G_h = G_h0 + G_s * iw;
-    if (m == 0) go[L.B[j]] = G_xi * iw;
-    // widths: w_m = eps + cw softmax(a)_m, X_k = sum_{m<k} w_m (heights alike)
-    const bool act = m < K;
-    const float Gwm = (m == k ? G_w : 0.f) + (m < k ? G_X : 0.f);
-    const float Ghm = (m == k ? G_h : 0.f) + (m < k ? G_Y : 0.f);
-    const float sdw = grp_sum<G>(act ? q.sw * Gwm : 0.f);
-    const float sdh = grp_sum<G>(act ? q.sh * Ghm : 0.f);
-    if (act) {
-      const float ga = cw * q.sw * (Gwm - sdw);
-      const float gb = cw * q.sh * (Ghm - sdh);
-      g[m] = ga;
-      g[K + m] = gb;
-      g16[m] = (__bf16)ga;
-      g16[K + m] = (__bf16)gb;
-    }
-    // interior derivatives d_{m+1} = eps + softplus(c_m), m = 0..K-2
-    if (m < K - 1) {
-      const float sg = sigmoidf(p[2 * K + m]);
-      const float gc = (m == k - 1 ? G_dk * sg : 0.f) + (m == k ? G_dk1 * sg : 0.f);
-      g[2 * K + m] = gc;
-      g16[2 * K + m] = (__bf16)gc;
-    }
-  }
-  // zero the padding columns of the parameter gradient
-  for (int c = L.dB * np + m; c < ldp; c += G) {
-    gp[(size_t)b * ldp + c] = 0.f;
-    gp16[(size_t)b * ldp + c] = (__bf16)0.f;
-  }
-  if (logq_chk && m == 0) logq_chk[b] += ljsum;
-}
-
 // Training direction, backward of layer l with one thread per (point,
-// transformed dimension): the block's spline-parameter rows are staged in LDS
-// (coalesced), each thread overwrites its own 3K - 1 parameters with their
-// gradients, and the rows are written back as FP32 and BF16 (coalesced). The
-// softmax backward needs sum_m sw_m G_m, which for the width (height) gradient
+// transformed dimension). Tile form: the block's spline-parameter rows are
+// staged in LDS (coalesced), each thread overwrites its own 3K - 1 parameters
+// with their gradients, and the rows are written back as FP32 and BF16
+// (coalesced). Direct form (compile-time K): each thread reads its parameter
+// block (32-byte aligned) with 16-byte loads and writes its gradients with
+// 16-byte stores of 8 BF16 (and 4 FP32) values, padding included. The softmax
+// backward needs sum_m sw_m G_m, which for the width (height) gradient
 // G_m = G_X (m < k), G_w (m = k), 0 (m > k) is
 //   G_X (X_k - k eps) / (1 - K eps) + G_w sw_k,
 // so no per-bin arrays are kept.
@@ -314,6 +233,7 @@ __global__ __launch_bounds__(256) void k_rqs_backward_t(
   // DIRECT: each thread reads its parameters from prm and writes its gradients
   // to gp16 (and gp) itself, without the LDS tile, whose size otherwise limits
   // the blocks per CU when the parameter rows are long
+  static_assert(KT > 0 || !DIRECT, "the direct form needs a compile-time K");
   const int K = KT > 0 ? KT : Kin;
   extern __shared__ float tile[];  // [R][ldp + 1]
   __shared__ float sLj[256];
@@ -338,7 +258,7 @@ __global__ __launch_bounds__(256) void k_rqs_backward_t(
   const int t = threadIdx.x;
   const int r = t / L.dB, j = t % L.dB;
   const int b = row0 + r;
-  const int np = 3 * K - 1;
+  const int nps = prm_stride(K);
   const float cw = 1.f - K * kEpsW;
   float lj = 0.f;
   if (r < R && b < n) {
@@ -348,33 +268,43 @@ __global__ __launch_bounds__(256) void k_rqs_backward_t(
     if (j == 0)
       for (int i = 0; i < L.dA; ++i) go[L.A[i]] = gi[L.A[i]];
     const float gld = -coef[b];
-    const float* p = DIRECT ? prm + (size_t)b * ldp + j * np : tile + r * ldt + j * np;
-    float* pw = DIRECT ? nullptr : tile + r * ldt + j * np;
-    // gradient of parameter m of this spline
-    auto put = [&](int m, float v) {
-      if constexpr (DIRECT) {
-        const size_t o = (size_t)b * ldp + j * np + m;
-        gp16[o] = (__bf16)v;
-        if (gp) gp[o] = v;
-      } else {
-        pw[m] = v;
+    const size_t o = (size_t)b * ldp + j * nps;  // this spline's parameter block
+    const float* p = DIRECT ? prm + o : tile + r * ldt + j * nps;
+    float* pw = DIRECT ? nullptr : tile + r * ldt + j * nps;
+    if constexpr (DIRECT) {
+      // zero the row's padding columns past the last dimension's block
+      if (j == 0) {
+        bf16x8 z16;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) z16[e] = (__bf16)0.f;
+        for (int c = L.dB * nps; c < ldp; c += 8) {
+          *(bf16x8*)(gp16 + (size_t)b * ldp + c) = z16;
+          if (gp) {
+            *(float4*)(gp + (size_t)b * ldp + c) = make_float4(0.f, 0.f, 0.f, 0.f);
+            *(float4*)(gp + (size_t)b * ldp + c + 4) = make_float4(0.f, 0.f, 0.f, 0.f);
+          }
+        }
       }
-    };
-    if (DIRECT && j == 0)
-      for (int c = L.dB * np; c < ldp; ++c) {
-        gp16[(size_t)b * ldp + c] = (__bf16)0.f;
-        if (gp) gp[(size_t)b * ldp + c] = 0.f;
-      }
+    }
     const float x = fminf(fmaxf(up[L.B[j]], 0.f), 1.f);
     float ew[KT > 0 ? KT : 1], eh[KT > 0 ? KT : 1];
     float mw, mh, iw, ih;
     float X = 0.f, Y = 0.f, swk = 0.f, shk = 0.f;
     int k = 0;
     if constexpr (KT > 0) {
+      if constexpr (DIRECT) {
 #pragma unroll
-      for (int m = 0; m < KT; ++m) {
-        ew[m] = p[m];
-        eh[m] = p[KT + m];
+        for (int m = 0; m < KT; m += 4) {
+          const float4 a = *(const float4*)(p + m), c = *(const float4*)(p + KT + m);
+          ew[m] = a.x, ew[m + 1] = a.y, ew[m + 2] = a.z, ew[m + 3] = a.w;
+          eh[m] = c.x, eh[m + 1] = c.y, eh[m + 2] = c.z, eh[m + 3] = c.w;
+        }
+      } else {
+#pragma unroll
+        for (int m = 0; m < KT; ++m) {
+          ew[m] = p[m];
+          eh[m] = p[KT + m];
+        }
       }
       mw = ew[0];
       mh = eh[0];
@@ -471,23 +401,67 @@ __global__ __launch_bounds__(256) void k_rqs_backward_t(
     const float icw = 1.f / cw;
     const float sdw = G_X * (X - k * kEpsW) * icw + G_w * swk;
     const float sdh = G_Y * (Y - k * kEpsW) * icw + G_h * shk;
+    // gradient of width / height logit m and of interior derivative logit m
+    auto gwid = [&](int m, float swm) {
+      return cw * swm * (((m < k) ? G_X : (m == k ? G_w : 0.f)) - sdw);
+    };
+    auto ghei = [&](int m, float shm) {
+      return cw * shm * (((m < k) ? G_Y : (m == k ? G_h : 0.f)) - sdh);
+    };
+    auto gder = [&](int m) {
+      return (m == k - 1 ? G_dk * sgk : 0.f) + (m == k ? G_dk1 * sgk1 : 0.f);
+    };
+    if constexpr (DIRECT) {
+      // 8 gradients per 16-byte BF16 store: widths, heights, then the K - 1
+      // derivative logits and the block's padding (K % 8 == 0: the block is 3K)
 #pragma unroll
-    for (int m = 0; m < K; ++m) {
-      float swm, shm;
-      if constexpr (KT > 0) {
-        swm = ew[m] * iw;
-        shm = eh[m] * ih;
-      } else {
-        swm = __expf(p[m] - mw) * iw;
-        shm = __expf(p[K + m] - mh) * ih;
+      for (int m0 = 0; m0 < KT; m0 += 8) {
+        float vw[8], vh[8], vd[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int m = m0 + e;
+          vw[e] = gwid(m, ew[m] * iw);
+          vh[e] = ghei(m, eh[m] * ih);
+          vd[e] = m < KT - 1 ? gder(m) : 0.f;
+        }
+        bf16x8 a16, b16, c16;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          a16[e] = (__bf16)vw[e];
+          b16[e] = (__bf16)vh[e];
+          c16[e] = (__bf16)vd[e];
+        }
+        *(bf16x8*)(gp16 + o + m0) = a16;
+        *(bf16x8*)(gp16 + o + KT + m0) = b16;
+        *(bf16x8*)(gp16 + o + 2 * KT + m0) = c16;
+        if (gp) {
+#pragma unroll
+          for (int e = 0; e < 8; e += 4) {
+            *(float4*)(gp + o + m0 + e) = make_float4(vw[e], vw[e + 1], vw[e + 2], vw[e + 3]);
+            *(float4*)(gp + o + KT + m0 + e) =
+                make_float4(vh[e], vh[e + 1], vh[e + 2], vh[e + 3]);
+            *(float4*)(gp + o + 2 * KT + m0 + e) =
+                make_float4(vd[e], vd[e + 1], vd[e + 2], vd[e + 3]);
+          }
+        }
       }
-      const float Gwm = (m < k) ? G_X : (m == k ? G_w : 0.f);
-      const float Ghm = (m < k) ? G_Y : (m == k ? G_h : 0.f);
-      put(m, cw * swm * (Gwm - sdw));
-      put(K + m, cw * shm * (Ghm - sdh));
+    } else {
+#pragma unroll
+      for (int m = 0; m < K; ++m) {
+        float swm, shm;
+        if constexpr (KT > 0) {
+          swm = ew[m] * iw;
+          shm = eh[m] * ih;
+        } else {
+          swm = __expf(p[m] - mw) * iw;
+          shm = __expf(p[K + m] - mh) * ih;
+        }
+        pw[m] = gwid(m, swm);
+        pw[K + m] = ghei(m, shm);
+      }
+      for (int m = 0; m < K - 1; ++m) pw[2 * K + m] = gder(m);
+      // the block's padding columns keep the parameters' zeros
     }
-    for (int m = 0; m < K - 1; ++m)
-      put(2 * K + m, (m == k - 1 ? G_dk * sgk : 0.f) + (m == k ? G_dk1 * sgk1 : 0.f));
   }
   sLj[t] = lj;
   __syncthreads();
@@ -1530,7 +1504,7 @@ __global__ __launch_bounds__(fused_threads(H), fused_waves(H)) void k_flow_layer
     }
   }
   __syncthreads();
-  const int np = 3 * K - 1;
+  const int nps = prm_stride(K);
   for (int i = 1; i < depth; ++i) {
     fused_product<H, R, NT / 64>(cur, lda, W.wt[i - 1], W.bias[i - 1], 0, H, H, false, nxt, lda,
                                  sP, ldpp, W.act_out[i], nullptr, row0, n, ldp);
@@ -1540,14 +1514,14 @@ __global__ __launch_bounds__(fused_threads(H), fused_waves(H)) void k_flow_layer
     nxt = tmp;
   }
   // output product and spline inverses by groups of gd transformed dimensions:
-  // a group's columns [j0 np, j1 np) span the n-tiles [tb, te); thread -> (row,
+  // a group's columns [j0 nps, j1 nps) span the n-tiles [tb, te); thread -> (row,
   // dimension of the group)
   const int t = threadIdx.x;
   for (int j0 = 0; j0 < L.dB; j0 += gd) {
     const int j1 = min(L.dB, j0 + gd), g = j1 - j0;
-    const int tb = (j0 * np) / 16, te = (j1 * np + 15) / 16;
+    const int tb = (j0 * nps) / 16, te = (j1 * nps + 15) / 16;
     fused_product<H, R, NT / 64>(cur, lda, W.wt[depth - 1], W.bias[depth - 1], tb, 16 * (te - tb),
-                                 L.dB * np, true, nxt, lda, sP, ldpp, nullptr, W.prm_out, row0, n,
+                                 L.dB * nps, true, nxt, lda, sP, ldpp, nullptr, W.prm_out, row0, n,
                                  ldp);
     __syncthreads();
     const int r = t / g, j = j0 + t % g;
@@ -1555,7 +1529,7 @@ __global__ __launch_bounds__(fused_threads(H), fused_waves(H)) void k_flow_layer
     if (r < R && row < n) {
       if (j == 0)
         for (int i = 0; i < L.dA; ++i) u_out[(size_t)row * D + L.A[i]] = sU[r * D + L.A[i]];
-      const float* p = sP + r * ldpp + j * np - 16 * tb;
+      const float* p = sP + r * ldpp + j * nps - 16 * tb;
       const float zv = fminf(fmaxf(sU[r * D + L.B[j]], 0.f), 1.f);
       Bin e;
       inverse_bin<KT>(p, K, zv, e);
@@ -1614,7 +1588,7 @@ struct FlowLayerHost {
 
 struct FlowState {
   gra_flow_cfg fc;
-  int D = 0, L = 0, K = 0, H = 0, depth = 0, np = 0, ldp = 0;
+  int D = 0, L = 0, K = 0, H = 0, depth = 0, np = 0, nps = 0, ldp = 0;
   int mlp_fp32 = 0;
   std::vector<FlowLayerHost> layers;
   size_t nparam = 0;
@@ -1900,7 +1874,7 @@ static void mlp_forward(gra_handle* h, FlowState* F, int l, const float* u, int 
     ep.bias = th + (out ? Ly.obo : Ly.obh[i]);
     ep.relu = out ? 0 : 1;
     ep.ldc = out ? F->ldp : H;
-    ep.ncols_valid = out ? Ly.idx.dB * F->np : H;
+    ep.ncols_valid = out ? Ly.idx.dB * F->nps : H;
     if (out) ep.out32 = prm_of(F, l);
     else {
       ep.out16 = act16_of(F, l, i);
@@ -1920,11 +1894,12 @@ static int bwd_rows(const FlowState* F, int dB) {
   const int by_lds = (int)((159 * 1024) / (4 * (size_t)(F->ldp + 1)));
   return std::max(1, std::min(256 / dB, by_lds));
 }
-// The spline backward reads and writes its parameter rows directly (no LDS
-// tile) when the tile of a full block would exceed 64 KB, i.e. allow fewer than
-// three blocks per CU (long rows: many bins and transformed dimensions)
+// The spline backward reads and writes each parameter block directly with
+// 16-byte accesses (no LDS tile) for the unrolled bin counts; the runtime-K
+// loops use the tile
 static bool bwd_direct(const FlowState* F, int dB) {
-  return sizeof(float) * (size_t)(256 / dB) * (F->ldp + 1) > (64u << 10);
+  (void)dB;
+  return F->fc.spline_loops == 0 && (F->K == 8 || F->K == 16 || F->K == 32);
 }
 
 // Kernel instances of the fused layer by (hidden, bins) and of the spline
@@ -1943,17 +1918,13 @@ static const void* layer_inverse_kernel(int H, int K) {
   return H == 64 ? layer_inverse_kernel_h<64>(K)
                  : (H == 128 ? layer_inverse_kernel_h<128>(K) : layer_inverse_kernel_h<256>(K));
 }
-template <bool DIRECT>
-static const void* spline_backward_kernel_d(int K) {
-  switch (K) {
-    case 8: return (const void*)k_rqs_backward_t<8, DIRECT>;
-    case 16: return (const void*)k_rqs_backward_t<16, DIRECT>;
-    case 32: return (const void*)k_rqs_backward_t<32, DIRECT>;
-    default: return (const void*)k_rqs_backward_t<0, DIRECT>;
-  }
-}
 static const void* spline_backward_kernel(int K, bool direct) {
-  return direct ? spline_backward_kernel_d<true>(K) : spline_backward_kernel_d<false>(K);
+  switch (K) {
+    case 8: return direct ? (const void*)k_rqs_backward_t<8, true> : (const void*)k_rqs_backward_t<8, false>;
+    case 16: return direct ? (const void*)k_rqs_backward_t<16, true> : (const void*)k_rqs_backward_t<16, false>;
+    case 32: return direct ? (const void*)k_rqs_backward_t<32, true> : (const void*)k_rqs_backward_t<32, false>;
+    default: return (const void*)k_rqs_backward_t<0, false>;
+  }
 }
 
 // Bin count that selects the kernel instance: 0 (the runtime-K loops) when the
@@ -1969,19 +1940,19 @@ static size_t fused_lds_bytes(const FlowState* F, int ldg) {
 
 // Output groups of the fused layer for a layer with dB transformed dimensions:
 // the group of gd dimensions [j0, j1) computes the n-tiles covering columns
-// [j0 np, j1 np); the tile stride is that of the widest group. Without a
+// [j0 nps, j1 nps); the tile stride is that of the widest group. Without a
 // requested size the largest group that keeps fused_blocks(H) blocks per CU is taken.
 // LDS per block that lets fused_blocks(H) blocks share a CU's 160 KB
 static size_t fused_lds_target(int H) {
   return (size_t)(160 * 1024 / fused_blocks(H)) - 512;
 }
 static void fused_groups(const FlowState* F, int dB, int want, int& gd, int& ldg, size_t& lds) {
-  const int np = 3 * F->K - 1;
+  const int nps = prm_stride(F->K);
   auto stride = [&](int g) {
     int w = 0;
     for (int j0 = 0; j0 < dB; j0 += g) {
       const int j1 = std::min(dB, j0 + g);
-      w = std::max(w, (j1 * np + 15) / 16 - (j0 * np) / 16);
+      w = std::max(w, (j1 * nps + 15) / 16 - (j0 * nps) / 16);
     }
     return 16 * w + 1;
   };
@@ -2244,6 +2215,7 @@ extern "C" int gra_hip_flow_init(gra_handle* h, const gra_flow_cfg* fc_in) {
   F->H = fc.hidden;
   F->depth = fc.depth;
   F->np = 3 * fc.bins - 1;
+  F->nps = prm_stride(fc.bins);
   int maxdB = 0;
   for (int l = 0; l < F->L; ++l) {
     FlowLayerHost Ly;
@@ -2252,7 +2224,7 @@ extern "C" int gra_hip_flow_init(gra_handle* h, const gra_flow_cfg* fc_in) {
     maxdB = std::max(maxdB, Ly.idx.dB);
     F->layers.push_back(Ly);
   }
-  F->ldp = ((maxdB * F->np + 63) / 64) * 64;
+  F->ldp = ((maxdB * F->nps + 63) / 64) * 64;
   // parameter layout
   size_t off = 0;
   for (int l = 0; l < F->L; ++l) {
@@ -2301,11 +2273,15 @@ extern "C" int gra_hip_flow_init(gra_handle* h, const gra_flow_cfg* fc_in) {
     for (int i = 1; i < F->depth; ++i)
       for (int k = 0; k < F->H * F->H; ++k) th[Ly.oWh[i] + k] = (float)((2 * unif() - 1) * ah);
     for (int j = 0; j < Ly.idx.dB; ++j)
-      for (int m = 0; m < F->K - 1; ++m) th[Ly.obo + j * F->np + 2 * F->K + m] = dinv;
-    // padding columns of the output layer stay zero and frozen
-    for (int hrow = 0; hrow < F->H; ++hrow)
-      for (int c = Ly.idx.dB * F->np; c < F->ldp; ++c) fr[Ly.oWo + (size_t)hrow * F->ldp + c] = 1;
-    for (int c = Ly.idx.dB * F->np; c < F->ldp; ++c) fr[Ly.obo + c] = 1;
+      for (int m = 0; m < F->K - 1; ++m) th[Ly.obo + j * F->nps + 2 * F->K + m] = dinv;
+    // padding columns of the output layer (each dimension's block past its 3K - 1
+    // parameters, and the row past the last block) stay zero and frozen
+    for (int c = 0; c < F->ldp; ++c) {
+      const int j = c / F->nps;
+      if (j < Ly.idx.dB && c - j * F->nps < F->np) continue;
+      for (int hrow = 0; hrow < F->H; ++hrow) fr[Ly.oWo + (size_t)hrow * F->ldp + c] = 1;
+      fr[Ly.obo + c] = 1;
+    }
   }
   hipError_t e1 = hipMemcpy(F->theta.p, th.data(), 4 * off, hipMemcpyHostToDevice);
   hipError_t e2 = hipMemcpy(F->frozen.p, fr.data(), off, hipMemcpyHostToDevice);

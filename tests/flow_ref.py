@@ -23,12 +23,17 @@ def masks(L, D):
     return out
 
 
+def prm_stride(K):
+    """Output columns per transformed dimension: the 3K - 1 spline parameters
+    padded to a multiple of 8 (the padding columns are zero)."""
+    return (3 * K - 1 + 7) // 8 * 8
+
+
 def layout(D, L, K, H, depth):
     """Offsets of every parameter block in the flat vector, as the library lays them out."""
-    np_ = 3 * K - 1
     ms = masks(L, D)
     maxdB = max(len(B) for _, B in ms)
-    ldp = ((maxdB * np_ + 63) // 64) * 64
+    ldp = ((maxdB * prm_stride(K) + 63) // 64) * 64
     off = 0
     lay = []
     for A, B in ms:
@@ -115,7 +120,8 @@ class FlowRef:
             prm = self.mlp(l, cur[:, A])
             nxt = cur.clone()
             for j, bi in enumerate(B):
-                p = prm[:, j * (3 * self.K - 1):(j + 1) * (3 * self.K - 1)]
+                c0 = j * prm_stride(self.K)
+                p = prm[:, c0:c0 + 3 * self.K - 1]
                 z, lj = self.spline_fwd(p, cur[:, bi])
                 nxt[:, bi] = z
                 tot = tot + lj

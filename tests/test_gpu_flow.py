@@ -8,7 +8,7 @@ import pytest
 
 from graniitti_amd import abi
 from graniitti_amd.process import MContinuum, MGraniitti, vegas_cfg
-from flow_ref import FlowRef, layout
+from flow_ref import FlowRef, layout, prm_stride
 
 pytestmark = pytest.mark.gpu
 
@@ -45,11 +45,13 @@ def init_random(P, fc, scale, seed):
         for (Wo, Ws), (bo, bs) in d["Wh"]:
             th[Wo:Wo + Ws[0] * Ws[1]] += rng.normal(0, scale, Ws[0] * Ws[1])
         off, (H, ld) = d["Wo"]
-        valid = len(d["B"]) * (3 * fc.bins - 1)
+        # the 3K - 1 parameter columns of each transformed dimension (padded blocks)
+        cols = np.concatenate([j * prm_stride(fc.bins) + np.arange(3 * fc.bins - 1)
+                               for j in range(len(d["B"]))])
         W = th[off:off + H * ld].reshape(H, ld)
-        W[:, :valid] += rng.normal(0, scale, (H, valid))
+        W[:, cols] += rng.normal(0, scale, (H, len(cols)))
         off, _ = d["bo"]
-        th[off:off + valid] += rng.normal(0, scale, valid)
+        th[off + cols] += rng.normal(0, scale, len(cols))
     abi.check(lib.gra_hip_flow_set_params(P.h, abi.fptr(th)), P.h)
     return th
 
