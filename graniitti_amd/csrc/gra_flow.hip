@@ -504,11 +504,13 @@ __global__ void k_mlp_in(const float* __restrict__ u, int D, LayerIdx L, const f
   if (h32) h32[(size_t)b * H + j] = acc;
 }
 
-// Input layer backward, dx part: du[b][A_i] += sum_j dh[b][j] W1[i][j]. A
-// block takes 256 points; 32-column slices of dh are staged in LDS with
-// coalesced loads (rows padded to 33 words), then each thread accumulates the
-// dot products of its own point.
-__global__ __launch_bounds__(256) void k_mlp_in_dx(const float* __restrict__ dh,
+// Input layer backward, dx part: du[b][A_i] += sum_j dh[b][j] W1[i][j], with dh
+// in FP32 (FP32 MLP path) or BF16 (the BF16 path's masked dX product). A block
+// takes 256 points; 32-column slices of dh are staged in LDS with coalesced
+// loads (rows padded to 33 words; BF16 rows as 16-byte pieces of 8 columns),
+// then each thread accumulates the dot products of its own point.
+template <class T>
+__global__ __launch_bounds__(256) void k_mlp_in_dx(const T* __restrict__ dh,
                                                    const float* __restrict__ W1, int H, int n, int D,
                                                    LayerIdx L, float* __restrict__ g) {
   __shared__ float tile[256][33];
@@ -519,9 +521,21 @@ __global__ __launch_bounds__(256) void k_mlp_in_dx(const float* __restrict__ dh,
 #pragma unroll
   for (int i = 0; i < kMaxDB; ++i) acc[i] = 0.f;
   for (int c0 = 0; c0 < H; c0 += 32) {
-    for (int e = t; e < 256 * 32; e += 256) {
-      const int r = e >> 5, c = e & 31;
-      tile[r][c] = (b0 + r < n) ? dh[(size_t)(b0 + r) * H + c0 + c] : 0.f;
+    if constexpr (sizeof(T) == 2) {
+      for (int e = t; e < 256 * 4; e += 256) {
+        const int r = e >> 2, c = (e & 3) * 8;
+        bf16x8 v;
+        if (b0 + r < n) v = *(const bf16x8*)(dh + (size_t)(b0 + r) * H + c0 + c);
+        else
+          for (int q = 0; q < 8; ++q) v[q] = (__bf16)0.f;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) tile[r][c + q] = (float)v[q];
+      }
+    } else {
+      for (int e = t; e < 256 * 32; e += 256) {
+        const int r = e >> 5, c = e & 31;
+        tile[r][c] = (b0 + r < n) ? (float)dh[(size_t)(b0 + r) * H + c0 + c] : 0.f;
+      }
     }
     if (t < kMaxDB * 32) {
       const int i = t >> 5, c = t & 31;
@@ -1241,12 +1255,13 @@ __global__ void k_coef(const double* __restrict__ w, int n, double inv_s2, float
 // backward pass, the activations and spline parameters are written to HBM.
 // LDS holds two activation tiles; the FP32 parameter tile of the output
 // product overlays the one that product does not read. R = 64 rows per block
-// below H = 256 and 32 at H = 256 (GRA_FUSED_ROWS_256 = 16 halves the tile there
-// for two blocks per CU, which measured slower: the B fragments are then read
-// twice as often).
+// (the output product's B fragments, read through L2, each feed R / 16 MFMA row
+// tiles; at H = 256 it is two thirds of the layer's time). At H = 256 one
+// block of 512 threads per CU: 216 VGPRs without spills; 32 rows at two blocks
+// per CU (128 VGPRs, 108-byte spill) sampled 10% slower, 128 rows spill 448 bytes.
 // ---------------------------------------------------------------------------
 #ifndef GRA_FUSED_ROWS_256
-#define GRA_FUSED_ROWS_256 32
+#define GRA_FUSED_ROWS_256 64
 #endif
 #ifndef GRA_FUSED_THREADS_256
 #define GRA_FUSED_THREADS_256 512
@@ -1269,14 +1284,18 @@ __host__ __device__ constexpr int fused_rows(int H) {
 __host__ __device__ constexpr int fused_threads(int H) {
   return H == 256 ? GRA_FUSED_THREADS_256 : GRA_FUSED_THREADS;
 }
-// Minimum waves per SIMD the register allocation must allow: two blocks per CU
-// Blocks per CU the register allocation must allow: two at H = 256; three
+// Blocks per CU the register allocation must allow: one at H = 256; three
 // below (168 VGPRs, a 40-byte spill outside the products), with the LDS tile
 // kept to a third of the CU (fused_lds_target)
 #ifndef GRA_FUSED_BLOCKS
 #define GRA_FUSED_BLOCKS 3
 #endif
-__host__ __device__ constexpr int fused_blocks(int H) { return H == 256 ? 2 : GRA_FUSED_BLOCKS; }
+#ifndef GRA_FUSED_BLOCKS_256
+#define GRA_FUSED_BLOCKS_256 1
+#endif
+__host__ __device__ constexpr int fused_blocks(int H) {
+  return H == 256 ? GRA_FUSED_BLOCKS_256 : GRA_FUSED_BLOCKS;
+}
 __host__ __device__ constexpr int fused_waves(int H) {
   return fused_blocks(H) * fused_threads(H) / 256;
 }
@@ -1677,7 +1696,7 @@ static int flow_alloc_batch(gra_handle* h, FlowState* F, int n) {
   bad |= F->gprm.ensure(sizeof(float) * N * ldp) != 0;
   bad |= F->gprm16.ensure(sizeof(__bf16) * N * ldp) != 0;
   bad |= F->dact16.ensure(sizeof(__bf16) * N * H * 2) != 0;
-  bad |= F->dact32.ensure(sizeof(float) * N * H * 2) != 0;
+  bad |= F->dact32.ensure(F->mlp_fp32 ? sizeof(float) * N * H * 2 : 16) != 0;
   bad |= F->gu0.ensure(sizeof(float) * N * D) != 0;
   bad |= F->gu1.ensure(sizeof(float) * N * D) != 0;
   bad |= F->coef.ensure(sizeof(float) * N) != 0;
@@ -2074,9 +2093,9 @@ static int flow_backward(gra_handle* h, FlowState* F, int n, float* logq_chk) {
       ep.mask = act16_of(F, l, i - 1);
       ep.ldm = H;
       ep.out16 = F->dact16.as<__bf16>() + (size_t)cur * F->cap * H;
-      // the FP32 copy is read only by the input-layer dx kernel (BF16 path)
-      ep.out32 = (F->mlp_fp32 || i == 1) ? F->dact32.as<float>() + (size_t)cur * F->cap * H
-                                         : nullptr;
+      // the FP32 copy feeds the FP32 path only (the BF16 path's input-layer dx
+      // kernel reads the BF16 product)
+      ep.out32 = F->mlp_fp32 ? F->dact32.as<float>() + (size_t)cur * F->cap * H : nullptr;
       ep.ldc = H;
       ep.ncols_valid = H;
       if (F->mlp_fp32)
@@ -2095,8 +2114,12 @@ static int flow_backward(gra_handle* h, FlowState* F, int n, float* logq_chk) {
       input_weight_grad(h, F, u_of(F, l), Ly.idx, dy32, H, H, n, grad + Ly.oW1, grad + Ly.ob1);
     else
       input_weight_grad(h, F, u_of(F, l), Ly.idx, dy16, H, H, n, grad + Ly.oW1, grad + Ly.ob1);
-    hipLaunchKernelGGL(k_mlp_in_dx, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                       h->stream, dy32, th + Ly.oW1, H, n, D, Ly.idx, g2);
+    if (F->mlp_fp32)
+      hipLaunchKernelGGL(k_mlp_in_dx<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                         h->stream, dy32, th + Ly.oW1, H, n, D, Ly.idx, g2);
+    else
+      hipLaunchKernelGGL(k_mlp_in_dx<__bf16>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                         h->stream, dy16, th + Ly.oW1, H, n, D, Ly.idx, g2);
     std::swap(g, g2);
   }
   GRA_HIP_CHECK(h, hipGetLastError());
