@@ -574,8 +574,11 @@ __global__ __launch_bounds__(256) void k_part_reduce(const float* __restrict__ p
   const bool on = idx < (kw + 1) * N;
   const int row = on ? idx / N : 0, col = on ? idx % N : 0;
   float acc = 0.f;
-  if (on)
+  // unrolled so that eight loads are in flight before their (in-order) adds
+  if (on) {
+#pragma unroll 8
     for (int k = g; k < nchunk; k += kReduceGroups) acc += part[((size_t)k * kin_t + row) * N + col];
+  }
   s[g][o] = acc;
   __syncthreads();
   if (g != 0 || !on) return;
@@ -1543,9 +1546,11 @@ __global__ __launch_bounds__(fused_threads(H), fused_waves(H)) void k_flow_layer
                                  L.dB * nps, true, nxt, lda, sP, ldpp, nullptr, W.prm_out, row0, n,
                                  ldp);
     __syncthreads();
-    const int r = t / g, j = j0 + t % g;
-    const int row = row0 + r;
-    if (r < R && row < n) {
+    // (row, dimension) pairs of the group; R g may exceed the block's threads
+    for (int q = t; q < R * g; q += NT) {
+      const int r = q / g, j = j0 + q % g;
+      const int row = row0 + r;
+      if (row >= n) continue;
       if (j == 0)
         for (int i = 0; i < L.dA; ++i) u_out[(size_t)row * D + L.A[i]] = sU[r * D + L.A[i]];
       const float* p = sP + r * ldpp + j * nps - 16 * tb;
@@ -1571,10 +1576,11 @@ __global__ __launch_bounds__(fused_threads(H), fused_waves(H)) void k_flow_layer
     }
     __syncthreads();
   }
-  if (t < R && row0 + t < n) {
+  for (int r = t; r < R; r += NT) {
+    if (row0 + r >= n) break;
     float acc = 0.f;
-    for (int jj = 0; jj < L.dB; ++jj) acc += sLj[t * 8 + jj];
-    ld[row0 + t] += acc;
+    for (int jj = 0; jj < L.dB; ++jj) acc += sLj[r * 8 + jj];
+    ld[row0 + r] += acc;
   }
 }
 

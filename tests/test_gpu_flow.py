@@ -269,6 +269,25 @@ def test_fused_layer_output_groups_agree():
     say("output groups 1, 2, 3, auto: identical u and log q")
 
 
+def test_fused_layer_groups_wider_than_the_block():
+    """At H = 128 a block has 256 threads for 64 rows: a group of all 7
+    transformed dimensions (D = 14, out_groups = 7) has 448 (row, dimension)
+    splines, which the block's threads take in turns. Same u and log q as one
+    dimension per group."""
+    D, L, K, H, depth = 14, 4, 16, 128, 2
+    say("case: fused layer groups 1 and 7 at D=14 L=4 K=16 H=128 depth=2")
+    z = np.random.default_rng(9).random((1024, D)).astype(np.float32)
+    out = {}
+    with MContinuum(rho_cfg()) as P:
+        for g in (1, 7):
+            fc = flow_cfg(D, L, K, H, depth, False)
+            fc.out_groups = g
+            init_random(P, fc, 0.05, 33)
+            out[g] = forward(P, z)
+    assert np.array_equal(out[7][0], out[1][0]) and np.array_equal(out[7][1], out[1][1])
+    say("output groups 1 and 7 at H=128: identical u and log q")
+
+
 def rho_cfg():
     cfg = abi.cfg_default(1)
     cfg.dim = 14
