@@ -787,14 +787,17 @@ int events_to_host(gra_handle* h, gra_event* dst, const void* d_src, uint64_t n)
     return GRA_OK;
   }
   if (bytes > h->h_ev_stage_bytes) {
+    // grown by half again as much, so that passes whose event counts drift
+    // upwards do not reallocate the pinned buffer each time
     if (h->h_ev_stage) (void)hipHostFree(h->h_ev_stage);
     h->h_ev_stage = nullptr;
     h->h_ev_stage_bytes = 0;
-    if (hipHostMalloc(&h->h_ev_stage, bytes, hipHostMallocDefault) != hipSuccess) {
+    const size_t want = bytes + bytes / 2;
+    if (hipHostMalloc(&h->h_ev_stage, want, hipHostMallocDefault) != hipSuccess) {
       h->h_ev_stage = nullptr;
-      return fail(h, GRA_E_NOMEM, "pinned event staging (%zu bytes)", bytes);
+      return fail(h, GRA_E_NOMEM, "pinned event staging (%zu bytes)", want);
     }
-    h->h_ev_stage_bytes = bytes;
+    h->h_ev_stage_bytes = want;
   }
   GRA_HIP_CHECK(h, hipMemcpyAsync(h->h_ev_stage, d_src, bytes, hipMemcpyDeviceToHost, h->stream));
   GRA_HIP_CHECK(h, hipStreamSynchronize(h->stream));
