@@ -6,10 +6,12 @@ single-channel eikonal screening (16 phi x 31 kt loop nodes, MaxLoopKT 3 GeV,
 row sums of the loop nodes added in FP64, 2^20 points per pass.
 
 The run follows the reference's SampleVegas (src/MGraniitti.cc:1074-1135):
-1. integration: VEGAS with W burn-in iterations (grid adaptation) and 4
+1. integration: VEGAS with 5 burn-in iterations (grid adaptation) and 4
    iterations of 2^20 points that fix max f, the largest f = W vw seen while
    integrating (the maxf that SaveEvent's hit-or-miss uses);
-2. generation, the timed steps: K passes of 2^20 points through the adapted grid,
+2. W untimed generation passes (other iterations' points), which also bring the
+   GPU to its steady clock: the first few tens of milliseconds of work run slower;
+3. generation, the timed steps: K passes of 2^20 points through the adapted grid,
    each drawing the points (Philox), weighting them (tree + screening kernels),
    accepting U < f / max f (src/MGraniitti.cc:1651-1663) and writing the accepted
    events' four-momenta to host memory.
@@ -53,6 +55,7 @@ VALU_PEAK_TFLOPS = 157.3
 # the run): profiles/k_screen_hbm_traffic_screened_pipi.md
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_screen_hbm_traffic_screened_pipi.md")
 MAXF_SAMPLE = 4  # integration iterations (of the pass size) that fix max f / max w
+BURNIN = 5  # VEGAS burn-in iterations of every integration
 EVENT_CAP = 1 << 17
 
 
@@ -214,10 +217,10 @@ def flow_leg(P, grid, fc, steps, n_pass, vegas, gen_passes, plateau_steps=0):
     return out
 
 
-def vegas_precision_run(P, n, warmup, seed):
+def vegas_precision_run(P, n, burnin, seed):
     """Plain VEGAS to 0.1% relative sigma error, grid adaptation included"""
     G = MGraniitti(P)
-    vc = vegas_cfg(n, 1, burnin=warmup, seed=seed, precision=1e-3, max_iter=60)
+    vc = vegas_cfg(n, 1, burnin=burnin, seed=seed, precision=1e-3, max_iter=60)
     res = G.Integrate(vc)
     t = res.seconds_to_precision if res.seconds_to_precision > 0 else None
     out = {"sigma": res.sigma, "sigma_err": res.sigma_err, "iterations": res.iterations,
@@ -293,12 +296,12 @@ def rhorho_leg(args):
     return {"preset": 3, "vegas": v, "flow": fl}
 
 
-def config_mode_leg(N, warmup, base):
+def config_mode_leg(N, burnin, base):
     """The workload in its stated loop discretisation (preset 11: 32 Gauss-Legendre
     kt nodes x 16 azimuths, A_eik on 64 log-spaced kt nodes) against the reference's
     (Simpson 3/8 on 16 x 31 nodes, 10001-row A_eik table): VEGAS to 0.1%."""
     with MContinuum(abi.cfg_default(11)) as P:
-        v = vegas_precision_run(P, N, warmup, 424242)
+        v = vegas_precision_run(P, N, burnin, 424242)
     comb = np.hypot(v["sigma_err"], base["sigma_err"])
     return {"preset": 11, "sigma_barn": v["sigma"], "sigma_err_barn": v["sigma_err"],
             "time_to_0p1pct_s": v["time_to_0p1pct_s"],
@@ -336,7 +339,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=30,
+                    help="untimed generation passes before the timed ones")
     ap.add_argument("--points", type=int, default=1 << 20)
     ap.add_argument("--preset", type=int, default=2)
     ap.add_argument("--no-res", action="store_true",
@@ -369,11 +373,14 @@ def main():
     P = MContinuum(cfg)
     G = MGraniitti(P)
     seed = 12345678
-    log(f"workload preset {args.preset}: {N} points/pass, burn-in {args.warmup}, "
-        f"generation passes {args.steps}")
+    log(f"workload preset {args.preset}: {N} points/pass, burn-in {BURNIN}, "
+        f"generation passes {args.warmup} untimed + {args.steps} timed")
     # integration: burn-in adapts the grid, MAXF_SAMPLE iterations fix max f
-    integ = G.Integrate(vegas_cfg(N, MAXF_SAMPLE, burnin=args.warmup, seed=seed))
+    integ = G.Integrate(vegas_cfg(N, MAXF_SAMPLE, burnin=BURNIN, seed=seed))
     maxf = integ.maxf
+    # untimed warm-up passes (iterations 900.. ; the timed passes draw 1000..)
+    if args.warmup > 0:
+        vegas_generate(P, G, N, seed, 900, args.warmup, maxf)
     for s in range(4):
         P.kernel_time(s, reset=True)
     # timed steps: generation passes
@@ -429,11 +436,11 @@ def main():
     vegas = {"sigma": integ.sigma, "sigma_err": integ.sigma_err, "time_to_0p1pct_s": None}
     prec_res = None
     if not args.no_precision_run:
-        prec_res = vegas_precision_run(P, N, args.warmup, 424242)
+        prec_res = vegas_precision_run(P, N, BURNIN, 424242)
         vegas = prec_res
     cmode = None
     if prec_res is not None and args.preset == 2:
-        cmode = config_mode_leg(N, args.warmup, prec_res)
+        cmode = config_mode_leg(N, BURNIN, prec_res)
         log(f"config mode: {cmode}")
     res = None
     if not args.no_res and args.preset == 2:
@@ -468,7 +475,7 @@ def main():
         "points_per_s": pts_per_s, "unweighting_efficiency": eff,
         "generation": {**tot, "maxf": maxf,
                        "maxf_sample": f"{MAXF_SAMPLE} x {N} integration points after "
-                                      f"{args.warmup} burn-in iterations"},
+                                      f"{BURNIN} burn-in iterations"},
         "time_to_0p1pct_s": vegas.get("time_to_0p1pct_s"), "precision_run": prec_res,
         "sigma_barn": integ.sigma, "sigma_err_barn": integ.sigma_err,
         "kernel_ms": {"tree": ms_tree, "screen": ms_scr},
