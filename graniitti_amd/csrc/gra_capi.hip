@@ -1128,6 +1128,23 @@ static int check_grid(gra_handle* h, const gra_grid* g) {
   return GRA_OK;
 }
 
+// Copies the grid's edges (dim x GRA_VEGAS_BINS) to h->d_grid, unless the buffer
+// already holds exactly these values from an earlier call.
+static int upload_grid(gra_handle* h, const gra_grid* grid) {
+  if (h->d_grid.ensure(sizeof(double) * GRA_MAX_DIM * GRA_VEGAS_BINS))
+    return fail(h, GRA_E_NOMEM, "grid buffer");
+  const size_t n = (size_t)grid->dim * GRA_VEGAS_BINS;
+  if (h->grid_dev_at == h->d_grid.p && h->grid_host.size() == n &&
+      std::memcmp(h->grid_host.data(), grid->xmat, n * sizeof(double)) == 0)
+    return GRA_OK;
+  h->grid_dev_at = nullptr;
+  GRA_HIP_CHECK(h, hipMemcpyAsync(h->d_grid.p, grid->xmat, n * sizeof(double),
+                                  hipMemcpyHostToDevice, h->stream));
+  h->grid_host.assign(grid->xmat, grid->xmat + n);
+  h->grid_dev_at = h->d_grid.p;
+  return GRA_OK;
+}
+
 extern "C" int gra_hip_vegas_iterate(gra_handle* h, const gra_grid* grid, uint64_t n_calls,
                                      uint64_t seed, uint32_t iter, gra_iter_stats* out) {
   if (!h) return set_global_error(GRA_E_ARG, "handle is NULL");
@@ -1143,13 +1160,12 @@ extern "C" int gra_hip_vegas_iterate(gra_handle* h, const gra_grid* grid, uint64
   // one wave per 512 points, at most 8192 blocks; +32 rows of stage-1 reduction space
   const int nblk = (int)std::min<uint64_t>(8192, std::max<uint64_t>(1, (n_calls + 511) / 512));
   const size_t H = 2 * (size_t)D * GRA_VEGAS_BINS;
-  if (h->d_grid.ensure(sizeof(double) * GRA_MAX_DIM * GRA_VEGAS_BINS) ||
-      h->d_part_hist.ensure(sizeof(double) * (H * (nblk + 32) + 32 * kIterScalars)) ||
+  if (h->d_part_hist.ensure(sizeof(double) * (H * (nblk + 32) + 32 * kIterScalars)) ||
       h->d_part_scal.ensure(sizeof(double) * kIterScalars * nblk) ||
       h->d_stats.ensure(sizeof(double) * (kIterScalars + H)))
     return fail(h, GRA_E_NOMEM, "VEGAS buffers");
-  GRA_HIP_CHECK(h, hipMemcpyAsync(h->d_grid.p, grid->xmat, sizeof(double) * D * GRA_VEGAS_BINS,
-                                  hipMemcpyHostToDevice, h->stream));
+  r = upload_grid(h, grid);
+  if (r) return r;
   r = run_integrand(h, h->d_grid.as<double>(), (unsigned int)grid->bins, seed, iter, 0, nullptr,
                     n_calls, h->d_W.as<double>(), nullptr, h->d_flags.as<uint8_t>(),
                     h->d_vw.as<double>(), h->d_bins.as<uint8_t>());
@@ -1217,11 +1233,10 @@ extern "C" int gra_hip_vegas_map(gra_handle* h, const gra_grid* grid, const doub
   const int D = grid->dim;
   DevBuf du, dx, dv, db;
   if (du.ensure(sizeof(double) * D * n) || dx.ensure(sizeof(double) * D * n) ||
-      dv.ensure(sizeof(double) * n) || db.ensure(sizeof(int32_t) * D * n) ||
-      h->d_grid.ensure(sizeof(double) * GRA_MAX_DIM * GRA_VEGAS_BINS))
+      dv.ensure(sizeof(double) * n) || db.ensure(sizeof(int32_t) * D * n))
     return fail(h, GRA_E_NOMEM, "vegas_map buffers");
-  GRA_HIP_CHECK(h, hipMemcpyAsync(h->d_grid.p, grid->xmat, sizeof(double) * D * GRA_VEGAS_BINS,
-                                  hipMemcpyHostToDevice, h->stream));
+  r = upload_grid(h, grid);
+  if (r) return r;
   GRA_HIP_CHECK(h, hipMemcpyAsync(du.p, u, sizeof(double) * D * n, hipMemcpyHostToDevice, h->stream));
   GRA_HIP_CHECK(h, launch_map(h->d_grid.as<double>(), D, (unsigned int)grid->bins, du.as<double>(),
                               n, dx.as<double>(), dv.as<double>(), db.as<int32_t>(), h->stream));
@@ -1249,7 +1264,6 @@ extern "C" int gra_hip_generate(gra_handle* h, const gra_grid* grid, uint64_t n_
   if (n_calls == 0 || n_calls > 0xFFFFFFFFull)
     return fail(h, GRA_E_ARG, "generate: n_calls %llu outside [1, 2^32)",
                 (unsigned long long)n_calls);
-  const int D = h->P.dim;
   r = ensure_point_buffers(h, n_calls);
   if (r) return r;
   // the accepted indices come out in ascending order (ordered compaction), so
@@ -1257,10 +1271,8 @@ extern "C" int gra_hip_generate(gra_handle* h, const gra_grid* grid, uint64_t n_
   const uint64_t cap = std::min<uint64_t>(max_events, n_calls);
   r = ensure_gen_buffers(h, n_calls, cap);
   if (r) return r;
-  if (h->d_grid.ensure(sizeof(double) * GRA_MAX_DIM * GRA_VEGAS_BINS))
-    return fail(h, GRA_E_NOMEM, "generation buffers");
-  GRA_HIP_CHECK(h, hipMemcpyAsync(h->d_grid.p, grid->xmat, sizeof(double) * D * GRA_VEGAS_BINS,
-                                  hipMemcpyHostToDevice, h->stream));
+  r = upload_grid(h, grid);
+  if (r) return r;
   r = run_integrand(h, h->d_grid.as<double>(), (unsigned int)grid->bins, seed, iter, 0, nullptr,
                     n_calls, h->d_W.as<double>(), nullptr, h->d_flags.as<uint8_t>(),
                     h->d_vw.as<double>(), h->d_bins.as<uint8_t>());
@@ -1270,6 +1282,23 @@ extern "C" int gra_hip_generate(gra_handle* h, const gra_grid* grid, uint64_t n_
                                  h->d_flags.as<uint8_t>(), n_calls, maxf, seed, iter, 0,
                                  h->d_gen_mask.as<uint64_t>(), h->d_gen_tile.as<uint32_t>(),
                                  h->d_gen_idx.as<uint64_t>(), cap, d_cnt, h->stream));
+  // The event records follow on the same stream: the kernel takes the accepted
+  // count (d_cnt[2]) from device memory and writes the first min(count, cap)
+  // records. A pinned events buffer that the device sees at the same address is
+  // written directly, so the pass ends with one synchronisation; any other
+  // buffer is filled from a device copy afterwards.
+  bool direct = false;
+  if (cap > 0) {
+    hipPointerAttribute_t a;
+    direct = hipPointerGetAttributes(&a, events) == hipSuccess && a.type == hipMemoryTypeHost &&
+             a.devicePointer == (void*)events;
+    (void)hipGetLastError();  // an unregistered pointer leaves an error behind
+    GRA_HIP_CHECK(h, k64::launch_event_record(h->P, h->d_grid.as<double>(),
+                                              (unsigned int)grid->bins, seed, iter, 0, nullptr,
+                                              h->d_gen_idx.as<uint64_t>(), h->d_W.as<double>(),
+                                              h->d_vw.as<double>(), cap, d_cnt + 2,
+                                              direct ? (void*)events : h->d_gen_ev.p, h->stream));
+  }
   unsigned long long* c = h->h_gen_cnt;
   GRA_HIP_CHECK(h, hipMemcpyAsync(c, d_cnt, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                                   h->stream));
@@ -1284,12 +1313,7 @@ extern "C" int gra_hip_generate(gra_handle* h, const gra_grid* grid, uint64_t n_
   st->accepted = c[2];
   st->overflow = c[3];
   st->stored = stored;
-  if (stored == 0) return GRA_OK;
-  GRA_HIP_CHECK(h, k64::launch_event_record(h->P, h->d_grid.as<double>(),
-                                            (unsigned int)grid->bins, seed, iter, 0, nullptr,
-                                            h->d_gen_idx.as<uint64_t>(), h->d_W.as<double>(),
-                                            h->d_vw.as<double>(), stored, h->d_gen_ev.p,
-                                            h->stream));
+  if (stored == 0 || direct) return GRA_OK;
   return events_to_host(h, events, h->d_gen_ev.p, stored);
 }
 

@@ -2623,7 +2623,8 @@ extern "C" int gra_hip_flow_generate(gra_handle* h, uint64_t n_calls, uint64_t s
     if (take > 0) {
       GRA_HIP_CHECK(h, k64::launch_event_record(h->P, nullptr, 0, 0, 0, 0, F->xx.as<double>(),
                                                 h->d_gen_idx.as<uint64_t>(), F->wts.as<double>(),
-                                                nullptr, take, h->d_gen_ev.p, h->stream));
+                                                nullptr, take, nullptr, h->d_gen_ev.p,
+                                                h->stream));
       r = events_to_host(h, events + st->stored, h->d_gen_ev.p, take);
       if (r) return r;
       for (uint64_t e = 0; e < take; ++e) events[st->stored + e].index += done;

@@ -87,6 +87,10 @@ struct gra_handle {
   void* h_ev_stage = nullptr;
   size_t h_ev_stage_bytes = 0;
   std::vector<double> h_stats;
+  // the grid edges last copied to d_grid: consecutive generation passes draw from
+  // one grid, which is then not copied again
+  std::vector<double> grid_host;
+  const void* grid_dev_at = nullptr;
 
   // timing
   hipEvent_t ev[2 * gra_hip::TS_N] = {};

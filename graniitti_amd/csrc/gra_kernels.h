@@ -22,7 +22,8 @@ namespace gra_hip {
   hipError_t launch_event_record(const RunParams& P, const double* grid, unsigned int bins,         \
                                  uint64_t seed, uint32_t iter, uint64_t first, const double* xin,   \
                                  const uint64_t* idx, const double* W, const double* vw,            \
-                                 uint64_t n_ev, void* out, hipStream_t st);                         \
+                                 uint64_t n_ev, const unsigned long long* n_dev, void* out,         \
+                                 hipStream_t st);                                                   \
   }
 
 GRA_DECLARE_INTEGRAND_LAUNCHERS(k64)

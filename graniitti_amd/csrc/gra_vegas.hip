@@ -296,11 +296,10 @@ __global__ void k_accept(const double* __restrict__ W, const double* __restrict_
   }
 }
 
-// Ordered compaction of the hit masks: a tile is kAccTileWords groups of 64
-// points (one block of 256 threads, kAccWordsPerThread consecutive groups per
-// thread)
-constexpr int kAccWordsPerThread = 16;
-constexpr int kAccTileWords = 256 * kAccWordsPerThread;
+// Ordered compaction of the hit masks: a tile is 256 wpt groups of 64 points
+// (one block of 256 threads, wpt consecutive groups per thread). wpt is 1 up to
+// 2^28 points, so that a pass of 2^20 points spreads over 64 blocks, and 16
+// above (accept_wpt)
 constexpr int kAccMaxTiles = 1024 * 16;  // k_tile_scan: 1024 threads x 16 tiles
 
 // Exclusive prefix sum of v over the 256 threads of a block; total in *tot.
@@ -326,11 +325,12 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
 
 // Per tile: the five mask counts (tile_cnt[5 t + m])
 __global__ __launch_bounds__(256) void k_tile_count(const uint64_t* __restrict__ masks,
-                                                    uint64_t ngroups, uint32_t* __restrict__ tile_cnt) {
+                                                    uint64_t ngroups, int wpt,
+                                                    uint32_t* __restrict__ tile_cnt) {
   __shared__ uint32_t sh[kAccMasks][4];
-  const uint64_t g0 = (uint64_t)blockIdx.x * kAccTileWords + threadIdx.x * kAccWordsPerThread;
+  const uint64_t g0 = ((uint64_t)blockIdx.x * 256 + threadIdx.x) * wpt;
   uint32_t c[kAccMasks] = {};
-  for (int j = 0; j < kAccWordsPerThread; ++j) {
+  for (int j = 0; j < wpt; ++j) {
     if (g0 + j >= ngroups) break;
 #pragma unroll
     for (int k = 0; k < kAccMasks; ++k) c[k] += (uint32_t)__popcll(masks[kAccMasks * (g0 + j) + k]);
@@ -402,19 +402,19 @@ __global__ __launch_bounds__(1024) void k_tile_scan(const uint32_t* __restrict__
 }
 
 __global__ __launch_bounds__(256) void k_tile_write(const uint64_t* __restrict__ masks,
-                                                    uint64_t ngroups,
+                                                    uint64_t ngroups, int wpt,
                                                     const uint32_t* __restrict__ tile_off,
                                                     uint64_t* __restrict__ idx, uint64_t max_out) {
   __shared__ uint32_t sh[4];
-  const uint64_t g0 = (uint64_t)blockIdx.x * kAccTileWords + threadIdx.x * kAccWordsPerThread;
+  const uint64_t g0 = ((uint64_t)blockIdx.x * 256 + threadIdx.x) * wpt;
   uint32_t c = 0;
-  for (int j = 0; j < kAccWordsPerThread; ++j) {
+  for (int j = 0; j < wpt; ++j) {
     if (g0 + j >= ngroups) break;
     c += (uint32_t)__popcll(masks[kAccMasks * (g0 + j)]);
   }
   uint32_t tot;
   uint64_t pos = (uint64_t)tile_off[blockIdx.x] + block_exclusive_scan(c, sh, &tot);
-  for (int j = 0; j < kAccWordsPerThread; ++j) {
+  for (int j = 0; j < wpt; ++j) {
     if (g0 + j >= ngroups || pos >= max_out) break;
     uint64_t m = masks[kAccMasks * (g0 + j)];
     while (m) {
@@ -443,8 +443,10 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st) {
 }
 
 size_t accept_mask_words(uint64_t n) { return kAccMasks * (size_t)((n + 63) / 64); }
+static int accept_wpt(uint64_t n) { return n <= (1ull << 28) ? 1 : 16; }
 static size_t accept_tiles(uint64_t n) {
-  return (size_t)(((n + 63) / 64 + kAccTileWords - 1) / kAccTileWords);
+  const uint64_t tile_words = 256 * (uint64_t)accept_wpt(n);
+  return (size_t)(((n + 63) / 64 + tile_words - 1) / tile_words);
 }
 size_t accept_tile_words(uint64_t n) { return (kAccMasks + 1) * accept_tiles(n); }
 
@@ -459,11 +461,12 @@ hipError_t launch_accept(const double* W, const double* vw, const uint8_t* flags
   const unsigned int nb = (unsigned int)((n + 255) / 256);
   hipLaunchKernelGGL(k_accept, dim3(nb), dim3(256), 0, st, W, vw, flags, n, maxf, seed, iter, first,
                      masks);
-  hipLaunchKernelGGL(k_tile_count, dim3((unsigned int)nt), dim3(256), 0, st, masks, ng, tiles);
+  const int wpt = accept_wpt(n);
+  hipLaunchKernelGGL(k_tile_count, dim3((unsigned int)nt), dim3(256), 0, st, masks, ng, wpt, tiles);
   uint32_t* off = tiles + kAccMasks * nt;
   hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, st, tiles, (int)nt, n, off, counts);
-  hipLaunchKernelGGL(k_tile_write, dim3((unsigned int)nt), dim3(256), 0, st, masks, ng, off, idx,
-                     max_out);
+  hipLaunchKernelGGL(k_tile_write, dim3((unsigned int)nt), dim3(256), 0, st, masks, ng, wpt, off,
+                     idx, max_out);
   return hipGetLastError();
 }
 

@@ -456,14 +456,17 @@ int gra_hip_vegas_map(gra_handle* h, const gra_grid* grid, const double* u, size
  * point is accepted when U < f / maxf (U from Philox, its own counter stream)
  * and never when f > maxf (counted as overflow). The first max_events accepted
  * points, in increasing point index, are written to events (host memory; pinned
- * memory from gra_hip_host_alloc is written directly, pageable memory through a
- * pinned staging buffer of the handle). */
+ * memory from gra_hip_host_alloc is written by the event-record kernel itself,
+ * pageable memory through a device buffer and a pinned staging buffer of the
+ * handle). Entries past st->stored are left untouched; after an error return the
+ * first max_events entries are unspecified. */
 int gra_hip_generate(gra_handle* h, const gra_grid* grid, uint64_t n_calls, uint64_t seed,
                      uint32_t iter, double maxf, gra_event* events, uint64_t max_events,
                      gra_gen_stats* st);
 
-/* Pinned (page-locked) host memory for event buffers, so that the event copies
- * of gra_hip_generate and gra_hip_flow_generate run at DMA rate. No counterpart
+/* Pinned (page-locked) host memory for event buffers: gra_hip_generate's kernel
+ * writes its events there directly, and gra_hip_flow_generate's event copies run
+ * at DMA rate. No counterpart
  * in the reference (its events never leave host memory). gra_hip_host_free
  * releases a buffer of gra_hip_host_alloc (NULL is ignored). */
 int gra_hip_host_alloc(size_t bytes, void** p);

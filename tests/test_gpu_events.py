@@ -56,7 +56,7 @@ def test_hit_or_miss_matches_host_restatement():
 
 def test_generation_into_pinned_and_pageable_buffers_agree():
     # the same pass into pageable memory (through the handle's pinned staging)
-    # and into pinned memory from gra_hip_host_alloc (direct copy), with a
+    # and into pinned memory from gra_hip_host_alloc (written by the kernel), with a
     # max_events cut that keeps the accepted points of smallest index; a pass
     # of 2^20 + 37 points covers a partial last group of 64 and several tiles
     cfg = abi.cfg_default(1)
@@ -71,11 +71,13 @@ def test_generation_into_pinned_and_pageable_buffers_agree():
             page = (abi.GraEvent * cap)()
             with abi.PinnedEvents(cap) as pin:
                 for buf in (page, pin.buf):
+                    arr = np.ctypeslib.as_array(buf)
+                    arr["index"] = np.uint64(0xDEADBEEF)  # entries past stored stay as they were
                     st = abi.GraGenStats()
                     P._chk(P.lib.gra_hip_generate(P.h, C.byref(G.grid), n, seed, it, maxf, buf,
                                                   cap, C.byref(st)))
-                    out.append((cap, st.trials, st.accepted, st.stored,
-                                np.ctypeslib.as_array(buf)[: st.stored].copy()))
+                    assert np.all(arr["index"][st.stored:] == 0xDEADBEEF)
+                    out.append((cap, st.trials, st.accepted, st.stored, arr[: st.stored].copy()))
     say("accepted " + ", ".join(f"{a} (stored {s}, cap {c})" for c, _, a, s, _ in out))
     for cap, trials, acc, stored, ev in out:
         assert trials == n and stored == min(acc, cap) and acc > 500

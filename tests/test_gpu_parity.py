@@ -117,8 +117,13 @@ def test_config2_fp32_screened_error():
 
 
 def test_vegas_map_matches_oracle():
+    # the handle keeps the grid it last copied to the device and skips the copy
+    # when the next call's grid is the same: the uniform grid, the adapted one and
+    # the uniform one again on one handle must each map with their own edges
     cfg = abi.cfg_default(1)
     say("case: vegas map on an adapted grid")
+    g0 = abi.GraGrid()
+    abi.check(abi.load().gra_hip_grid_init(C.byref(g0), 8, 128))
     g = abi.GraGrid()
     abi.check(abi.load().gra_hip_grid_init(C.byref(g), 8, 128))
     rng = np.random.default_rng(5)
@@ -128,7 +133,12 @@ def test_vegas_map_matches_oracle():
     abi.check(abi.load().gra_hip_grid_optimize(C.byref(g), C.byref(st), 1.5))
     u = rng.random((5000, 8))
     with MContinuum(cfg) as P:
+        x0, _, _ = P.vegas_map(g0, u)
         x, vw, b = P.vegas_map(g, u)
+        x0b, _, _ = P.vegas_map(g0, u)
+    assert np.abs(x0 - u).max() < 1e-15  # the uniform grid maps u to itself
+    np.testing.assert_array_equal(x0b, x0)
+    assert np.abs(x - u).max() > 0.01
     L = ol.oracle()
     for i in range(0, 5000, 7):
         xo = np.zeros(8)
