@@ -21,6 +21,7 @@ GRA_PROC_TENSOR_CON = 12
 GRA_PROC_RES_CON = 17
 GRA_PROC_RES = 9
 GRA_MAX_RES = 8
+GRA_MAX_VETO = 8
 GRA_BW_FIXED, GRA_BW_RUNNING, GRA_BW_BARRIER, GRA_BW_SPIN = 1, 2, 3, 4
 GRA_FRAME_CM, GRA_FRAME_HX, GRA_FRAME_CS, GRA_FRAME_GJ = 0, 1, 2, 3
 GRA_AMP_FP64 = 0
@@ -70,6 +71,9 @@ class GraCfg(C.Structure):
         ("phase_space", I32), ("pad_ps", I32), ("gen_M_min", D), ("gen_M_max", D),
         ("n_res", I32), ("spin_gen", I32), ("spin_dec", I32), ("spin_frame", I32),
         ("regge_omega", D), ("res", GraRes * GRA_MAX_RES),
+        ("veto_active", I32), ("n_veto", I32),
+        ("veto_eta_min", D * GRA_MAX_VETO), ("veto_eta_max", D * GRA_MAX_VETO),
+        ("veto_pt_min", D * GRA_MAX_VETO), ("veto_pt_max", D * GRA_MAX_VETO),
     ]
 
 

@@ -323,6 +323,17 @@ int validate(const gra_cfg* c) {
     return set_global_error(GRA_E_ARG, "asymmetric beams are not supported (E1 != E2)");
   if (!(c->fwd_pt_max > c->fwd_pt_min) || !(c->kt_max > c->kt_min) || !(c->rap_max > c->rap_min))
     return set_global_error(GRA_E_ARG, "empty generation-cut interval");
+  if (c->veto_active) {
+    if (c->n_veto < 0 || c->n_veto > GRA_MAX_VETO)
+      return set_global_error(GRA_E_ARG, "n_veto %d outside 0..%d", c->n_veto, GRA_MAX_VETO);
+    // aux::AssertCut(.., dothrow = true) on each domain's Eta and Pt, src/MGraniitti.cc:950-956
+    for (int d = 0; d < c->n_veto; ++d)
+      if (!(c->veto_eta_max[d] > c->veto_eta_min[d]) || !(c->veto_pt_max[d] > c->veto_pt_min[d]))
+        return set_global_error(GRA_E_ARG,
+                                "VETOCUTS::%d: Eta [%g, %g], Pt [%g, %g] (maximum value not above "
+                                "minimum value)", d, c->veto_eta_min[d], c->veto_eta_max[d],
+                                c->veto_pt_min[d], c->veto_pt_max[d]);
+  }
   if (c->S_factor <= 0.0) return set_global_error(GRA_E_ARG, "S_factor must be > 0");
   if (c->screening) {
     if (c->loop_rule == GRA_LOOP_SIMPSON38) {
@@ -441,6 +452,14 @@ void fill_params(const gra_cfg& c, RunParams& P) {
     P.bw_dmin = dm;
   }
   P.fid_active = c.fid_active;
+  P.veto_active = c.veto_active && c.n_veto > 0;
+  P.n_veto = c.veto_active ? c.n_veto : 0;
+  for (int d = 0; d < GRA_MAX_VETO; ++d) {
+    P.veto_eta_min[d] = c.veto_eta_min[d];
+    P.veto_eta_max[d] = c.veto_eta_max[d];
+    P.veto_pt_min[d] = c.veto_pt_min[d];
+    P.veto_pt_max[d] = c.veto_pt_max[d];
+  }
   P.fid_eta_min = c.fid_eta_min;
   P.fid_eta_max = c.fid_eta_max;
   P.fid_rap_min = c.fid_rap_min;

@@ -37,6 +37,17 @@ void set_common(gra_cfg* c) {
   c->kt_min = 0.0;
   c->kt_max = 10.0 / 2.0;
 
+  // VETOCUT: off; unused domains hold VETODOMAIN's defaults
+  // (include/Graniitti/MKinematics.h:1613-1626)
+  c->veto_active = 0;
+  c->n_veto = 0;
+  for (int d = 0; d < GRA_MAX_VETO; ++d) {
+    c->veto_eta_min[d] = -30.0;
+    c->veto_eta_max[d] = 30.0;
+    c->veto_pt_min[d] = 0.0;
+    c->veto_pt_max[d] = 1000000.0;
+  }
+
   // FIDCUT (ALICECUTS.json values where the presets state no other)
   c->fid_active = 1;
   c->fid_rap_min = -10.0;

@@ -406,6 +406,33 @@ __device__ __forceinline__ bool leg_cuts(const RunParams& P, dv4 p) {
          rap <= P.fid_rap_max;
 }
 
+// MProcess::VetoCuts with MProcess::FindVetoCuts, src/MProcess.cc:1069-1082,
+// 1157-1175: false when a final-state central particle lies in a veto domain
+// (pt and eta both inside). The forward protons are not tested: the reference
+// tests its forward decay branches, which are empty for elastic protons.
+__device__ __forceinline__ bool veto_hit(const RunParams& P, dv4 p) {
+  const double pt = ptof(p), eta = etaof(p);
+  bool hit = false;
+  for (int d = 0; d < P.n_veto; ++d)
+    if (pt >= P.veto_pt_min[d] && pt <= P.veto_pt_max[d] && eta >= P.veto_eta_min[d] &&
+        eta <= P.veto_eta_max[d])
+      hit = true;
+  return hit;
+}
+__device__ __forceinline__ bool veto_cuts(const RunParams& P, const Kin& K) {
+  if (!P.veto_active) return true;
+  bool ok = true;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    if (P.m_daughter > 0.0) {
+      if (veto_hit(P, K.legs[i][0]) || veto_hit(P, K.legs[i][1])) ok = false;
+    } else {
+      if (veto_hit(P, K.dt[i])) ok = false;
+    }
+  }
+  return ok;
+}
+
 // MProcess::CommonCuts, src/MProcess.cc:1085-1135
 __device__ __forceinline__ bool fiducial_cuts(const RunParams& P, const Kin& K) {
   if (!P.fid_active) return true;

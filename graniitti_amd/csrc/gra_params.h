@@ -45,6 +45,10 @@ struct RunParams {
   double fid_pt_min, fid_pt_max, fid_et_min, fid_et_max;
   double sys_M_min, sys_M_max, sys_Y_min, sys_Y_max, sys_Pt_min, sys_Pt_max;
   double fwd_t_min, fwd_t_max;
+  // veto domains (elastic forward protons are never tested)
+  int32_t veto_active, n_veto;
+  double veto_eta_min[GRA_MAX_VETO], veto_eta_max[GRA_MAX_VETO];
+  double veto_pt_min[GRA_MAX_VETO], veto_pt_max[GRA_MAX_VETO];
   // weight constants
   double psw_c1, psw_c2, lspan2, KT_vol, PHI_vol, invS, GeV2barn, flux;
   // <F> phase space (MFactorized B51): M_MIN, M_MAX of the system (B51RandomKin,

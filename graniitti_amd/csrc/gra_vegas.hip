@@ -76,7 +76,8 @@ __global__ __launch_bounds__(kAccWave* kAccWaves) void k_accumulate(
         const uint8_t fl = flags[i];
         c_kin += (fl & GRA_FLAG_KIN_OK) ? 1 : 0;
         c_fid += (fl & GRA_FLAG_FID_OK) ? 1 : 0;
-        c_val += ((fl & GRA_FLAG_KIN_OK) && (fl & GRA_FLAG_FID_OK)) ? 1 : 0;
+        c_val += ((fl & GRA_FLAG_KIN_OK) && (fl & GRA_FLAG_FID_OK) && (fl & GRA_FLAG_VETO_OK)) ? 1
+                                                                                           : 0;
         const bool bad = !(fl & GRA_FLAG_AMP_OK);
         c_nan += (bad && !(fl & GRA_FLAG_INF)) ? 1 : 0;
         c_inf += (bad && (fl & GRA_FLAG_INF)) ? 1 : 0;

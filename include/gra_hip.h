@@ -63,6 +63,7 @@ typedef struct gra_handle gra_handle; /* opaque: device state of one process */
 
 /* Resonances of PP[RES+CON] */
 #define GRA_MAX_RES 8
+#define GRA_MAX_VETO 8 /* veto domains (the reference's card reader takes up to 100) */
 /* Breit-Wigner forms, PARAM_RES "BW" (form::CBW, src/MForm.cc:743-828) */
 #define GRA_BW_FIXED 1   /* fixed-width relativistic */
 #define GRA_BW_RUNNING 2 /* running-width relativistic */
@@ -199,6 +200,20 @@ typedef struct gra_cfg {
   int32_t spin_frame; /* GRA_FRAME_* */
   double regge_omega;
   gra_res res[GRA_MAX_RES];
+  /* Veto domains (VETOCUT, include/Graniitti/MKinematics.h:1613-1626; read from
+   * the card's "VETOCUTS" block, src/MGraniitti.cc:922-962). With veto_active, a
+   * point whose final-state central particles include one with pt in
+   * [veto_pt_min, veto_pt_max] and eta in [veto_eta_min, veto_eta_max] of any of
+   * the n_veto domains is vetoed: W = 0 and GRA_FLAG_VETO_OK clear
+   * (MProcess::VetoCuts / FindVetoCuts, src/MProcess.cc:1069-1082, 1157-1175).
+   * Every domain needs max > min in both ranges (aux::AssertCut). The elastic
+   * forward protons are never vetoed: the reference tests its forward decay
+   * branches, which stay empty (a zero four-vector, whose pseudorapidity is NaN)
+   * unless the proton is excited. */
+  int32_t veto_active;
+  int32_t n_veto;
+  double veto_eta_min[GRA_MAX_VETO], veto_eta_max[GRA_MAX_VETO];
+  double veto_pt_min[GRA_MAX_VETO], veto_pt_max[GRA_MAX_VETO];
 } gra_cfg;
 
 /* ------------------------------------------------------------------------- */

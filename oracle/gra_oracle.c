@@ -735,6 +735,29 @@ static int leg_cuts(const gra_cfg* c, v4 p) {
           rap <= c->fid_rap_max);
 }
 
+/* MProcess::VetoCuts and FindVetoCuts, src/MProcess.cc:1069-1082, 1157-1175: the
+ * recursion reaches the final states of the central decay tree (the forward
+ * decay branches of elastic protons are empty: a zero four-vector, whose Eta()
+ * is NaN, triggers no domain). 1 = no final state in any domain. */
+static int veto_cuts(const ora_run* R, const kin_t* K) {
+  const gra_cfg* c = &R->c;
+  if (!c->veto_active) return 1;
+  int ok = 1;
+  for (int i = 0; i < 2; ++i) {
+    const int nl = c->m_daughter > 0.0 ? 2 : 1;
+    for (int l = 0; l < nl; ++l) {
+      const v4 p = c->m_daughter > 0.0 ? K->legs[i][l] : K->dt[i];
+      for (int d = 0; d < c->n_veto; ++d) {
+        if (v4_pt(p) >= c->veto_pt_min[d] && v4_pt(p) <= c->veto_pt_max[d] &&
+            v4_eta(p) >= c->veto_eta_min[d] && v4_eta(p) <= c->veto_eta_max[d]) {
+          ok = 0; /* VETO, does not pass */
+        }
+      }
+    }
+  }
+  return ok;
+}
+
 /* MProcess::CommonCuts, src/MProcess.cc:1085-1135 (USERCUTS = 0, elastic legs) */
 static int fiducial_cuts(const ora_run* R, const kin_t* K) {
   const gra_cfg* c = &R->c;
@@ -1380,8 +1403,10 @@ static double event_weight(ora_run* R, const double* x, double* amp2_out, uint8_
   if (kin_ok) fl |= GRA_FLAG_KIN_OK;
   const int fid_ok = kin_ok ? fiducial_cuts(R, &K) : 0;
   if (fid_ok) fl |= GRA_FLAG_FID_OK;
-  fl |= GRA_FLAG_VETO_OK;
-  if (kin_ok && fid_ok) {
+  /* tested on valid kinematics only, like the fiducial flag */
+  const int veto_ok = kin_ok ? veto_cuts(R, &K) : 1;
+  if (veto_ok) fl |= GRA_FLAG_VETO_OK;
+  if (kin_ok && fid_ok && veto_ok) {
     amp2 = screened_amp2(R, &K);
     if (c->phase_space == GRA_PS_F) {
       /* MFactorized::EventWeight, src/MFactorized.cc:142-182: C_space = DW / (2 pi)
@@ -1632,7 +1657,7 @@ static void accumulate_point(gra_iter_stats* st, const gra_grid* g, double W, do
   st->n_calls += 1;
   if (fl & GRA_FLAG_KIN_OK) st->n_kin_ok += 1;
   if (fl & GRA_FLAG_FID_OK) st->n_fid_ok += 1;
-  if ((fl & GRA_FLAG_KIN_OK) && (fl & GRA_FLAG_FID_OK)) st->n_valid += 1;
+  if ((fl & GRA_FLAG_KIN_OK) && (fl & GRA_FLAG_FID_OK) && (fl & GRA_FLAG_VETO_OK)) st->n_valid += 1;
   if (!(fl & GRA_FLAG_AMP_OK)) {
     if (fl & GRA_FLAG_INF)
       st->n_inf += 1;
