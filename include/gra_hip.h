@@ -201,7 +201,7 @@ typedef struct gra_cfg {
   double regge_omega;
   gra_res res[GRA_MAX_RES];
   /* Veto domains (VETOCUT, include/Graniitti/MKinematics.h:1613-1626; read from
-   * the card's "VETOCUTS" block, src/MGraniitti.cc:922-962). With veto_active, a
+   * the card's "VETOCUTS" block, src/MGraniitti.cc:910-962). With veto_active, a
    * point whose final-state central particles include one with pt in
    * [veto_pt_min, veto_pt_max] and eta in [veto_eta_min, veto_eta_max] of any of
    * the n_veto domains is vetoed: W = 0 and GRA_FLAG_VETO_OK clear
