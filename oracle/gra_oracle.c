@@ -39,7 +39,18 @@
  *     Collins-Soper frame  f2 and rho spin factors at generated events equal the
  *                          closed forms in the CS angle computed from the 4-vectors
  *     decay coupling       g^2 p* / (8 pi M^2) = BR Gamma
- *     CBW, resonance form factor, PhotoProp, coherent photon flux: parity unpinned
+ *   form factors (tests/test_oracle_pins.py)
+ *     Kelly F1             F1(0) = 1; within 40% of the dipole Dirac form factor
+ *                          (the reference's testbench3 check)
+ *     CBW                  all forms equal i / (M0 Gamma) at the pole; Lorentzian
+ *                          fixed form; running/barrier widths; J = 1, 2 numerators
+ *     resonance FF         Gaussian in s_hat of width S0^2
+ *     CohFlux              its Q^2 -> 0 limits for pt >> xi m_p and pt << xi m_p
+ *                          against CODATA alpha and mu_p
+ *     PhotoProp            phase of the signature factor; slope B0 + 2 alpha' log s
+ *                          with the reference's stated B0 (11 rho, 4.8 J/psi)
+ *   CohFlux away from Q^2 = 0 (its Q^2 dependence through Kelly's G_E, G_M):
+ *   parity unpinned beyond the pinned F1
  */
 #include <math.h>
 #include <pthread.h>
@@ -1046,6 +1057,10 @@ static cx o_photo_prop(const gra_cfg* c, double s, double t, double m2) {
   const double FF = c->gN_P * exp(B0 * t / 2.0);
   return cx_scale(cx_scale(eta, pow(s / 1.0, alpha)), FF);
 }
+/* PARAM_REGGE::ResonanceFormFactor, src/MRegge.cc:1233-1235 */
+static double o_res_ff(double s_hat, double M2, double S0) {
+  return exp(-((s_hat - M2) * (s_hat - M2)) / pow(S0, 4));
+}
 /* sum over resonances of MRegge::ME3 (src/MRegge.cc:677-706) and, for 1-,
  * MRegge::PhotoME3 (src/MRegge.cc:781-843) */
 static cx res_sum(const ora_run* R, const kin_t* K) {
@@ -1058,7 +1073,7 @@ static cx res_sum(const ora_run* R, const kin_t* K) {
     const gra_res* q = &c->res[r];
     const cx bw = o_cbw(q, K->m2, v4_m(K->dt[0]), v4_m(K->dt[1]));
     const double M2 = q->mass * q->mass;
-    const double rff = exp(-((K->m2 - M2) * (K->m2 - M2)) / pow(q->g_FF, 4));
+    const double rff = o_res_ff(K->m2, M2, q->g_FF);
     const cx spin = cx_mul(o_prod_amp(R, q, K, thR, phR), o_decay_amp(R, q, K, thR, phR));
     if (q->J == 1 && q->P == -1) {
       const cx common = cx_scale(bw, rff);
@@ -2182,6 +2197,18 @@ ORA_EXPORT int ora_tensor_amplitudes(ora_run* R, const double* p, double* hamp_r
 
 /* form::F1 with the Kelly fits at Q2 (test access) */
 ORA_EXPORT double ora_kelly_f1(double Q2, double mp) { return (double)kelly_f1(Q2, mp); }
+
+/* PARAM_REGGE::ResonanceFormFactor, form::CohFlux and the elastic leg of
+ * MRegge::PhotoProp (out = (re, im)) for tests */
+ORA_EXPORT double ora_res_ff(double s_hat, double M2, double S0) { return o_res_ff(s_hat, M2, S0); }
+ORA_EXPORT double ora_coh_flux(double xi, double t, double pt, double mp) {
+  return o_coh_flux(xi, t, pt, mp);
+}
+ORA_EXPORT void ora_photo_prop(const gra_cfg* c, double s, double t, double m2, double* out) {
+  const cx z = o_photo_prop(c, s, t, m2);
+  out[0] = z.re;
+  out[1] = z.im;
+}
 
 /* form::CBW of resonance q at m2 with decay masses mA, mB: out = (re, im) */
 ORA_EXPORT void ora_cbw(const gra_res* q, double m2, double mA, double mB, double* out) {

@@ -53,6 +53,9 @@ def _load_oracle():
         "ora_gauss_legendre": (None, [C.c_int, P(D), P(D)]),
         "ora_log_node_amp": (None, [VP, D, P(D)]),
         "ora_cbw": (None, [P(abi.GraRes), D, D, D, P(D)]),
+        "ora_res_ff": (D, [D, D, D]),
+        "ora_coh_flux": (D, [D, D, D, D]),
+        "ora_photo_prop": (None, [P(abi.GraCfg), D, D, D, P(D)]),
         "ora_vegas_iterate": (None, [VP, P(abi.GraGrid), U64, U64, U64, C.c_uint32, C.c_int,
                                      C.c_int, P(abi.GraIterStats)]),
         "ora_vegas_combine": (None, [D, D, D, C.c_int, P(D), P(D)]),

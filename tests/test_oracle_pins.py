@@ -10,8 +10,13 @@ own tests and the mathematics fix:
   Lorentzian 1 / ((m^2 - M0^2)^2 + M0^2 Gamma^2) in |CBW|^2 with phase
   arg(-1 / (m^2 - M0^2 + i M0 Gamma)); the barrier form's width scales as
   (q / q0)^(2J+1) M0 / m; the spin-dependent numerators are -(1 - m^2/M0^2) for
-  J = 1 (a zero at the pole) and -(7/6 - 4/3 m^2/M0^2 + 2/3 m^4/M0^4) for J = 2.
-The EPA photon flux normalisation and PhotoProp's slope stay unpinned."""
+  J = 1 (a zero at the pole) and -(7/6 - 4/3 m^2/M0^2 + 2/3 m^4/M0^4) for J = 2;
+- PARAM_REGGE::ResonanceFormFactor: a Gaussian in s_hat of width S0^2;
+- form::CohFlux: its limits at Q^2 -> 0 for pt >> xi m_p and pt << xi m_p,
+  against CODATA alpha and mu_p;
+- MRegge::PhotoProp: the signature factor's phase, and the slope
+  B = B0 + 2 alpha' log s of |A|^2 with the reference's stated B0 ~ 11 (rho)
+  and 4.8 (J/psi), frozen below m = 0.7 GeV."""
 import ctypes as C
 
 import numpy as np
@@ -114,3 +119,97 @@ def test_spin_dependent_numerators():
         assert abs(z2 - n2 / den) <= 1e-12 * abs(z2)
         z0 = cbw(_res(M, W, 0, abi.GRA_BW_SPIN), m2)
         assert abs(z0 - (-1.0) / den) <= 1e-12 * abs(z0)
+
+
+# ---------------------------------------------------------------------------
+# PARAM_REGGE::ResonanceFormFactor (src/MRegge.cc:1233-1235): a Gaussian in
+# s_hat around M^2 with width S0^2
+# ---------------------------------------------------------------------------
+def test_resonance_form_factor_is_a_gaussian_in_s_hat():
+    f = ol.oracle().ora_res_ff
+    M2, S0 = 1.2755 ** 2, 0.7
+    assert f(M2, M2, S0) == 1.0
+    assert f(M2 + S0 ** 2, M2, S0) == pytest.approx(np.exp(-1.0), rel=1e-14)
+    assert f(M2 - 2 * S0 ** 2, M2, S0) == pytest.approx(np.exp(-4.0), rel=1e-14)
+    for d in (0.05, 0.3, 0.9):
+        assert f(M2 + d, M2, S0) == pytest.approx(f(M2 - d, M2, S0), rel=1e-14)
+
+
+# ---------------------------------------------------------------------------
+# form::CohFlux (src/MForm.cc:655-690), the coherent photon flux of the proton
+# with the reference's 2 -> N sampling factors (1/xi, 1/pt^2, 16 pi^2). Its two
+# limits at Q^2 -> 0, where Kelly's G_E = 1 and G_M = mu_p, fix the
+# normalisation with CODATA constants that the oracle does not share:
+#   pt >> xi m_p: xi pt^2 f / (16 pi^2) -> (alpha/pi) [(1 - xi) + xi^2 mu_p^2 / 4]
+#   pt << xi m_p: the electric part vanishes as pt^2, the magnetic part leaves
+#                 f -> 4 pi alpha mu_p^2 / (m_p^2 xi)
+# ---------------------------------------------------------------------------
+ALPHA = 1.0 / 137.035999084
+
+
+@pytest.mark.parametrize("xi", [1e-3, 0.05, 0.3])
+def test_photon_flux_at_large_transverse_momentum(xi):
+    pt = 1e4  # (xi m_p / pt)^2 below 1e-9
+    f = ol.oracle().ora_coh_flux(xi, -1e-14, pt, MP)
+    want = ALPHA / np.pi * ((1 - xi) + xi * xi * MU_P ** 2 / 4)
+    assert xi * pt * pt * f / (16 * np.pi ** 2) == pytest.approx(want, rel=1e-8)
+
+
+@pytest.mark.parametrize("xi", [0.01, 0.2])
+def test_photon_flux_at_small_transverse_momentum(xi):
+    f = ol.oracle().ora_coh_flux
+    want = 4 * np.pi * ALPHA * MU_P ** 2 / (MP * MP * xi)
+    assert f(xi, -1e-14, 1e-7 * xi, MP) == pytest.approx(want, rel=1e-8)
+    # the approach to the limit is quadratic in pt (the electric part and the
+    # expansion of the magnetic part)
+    d1 = f(xi, -1e-14, 1e-3 * xi, MP) - want
+    d2 = f(xi, -1e-14, 2e-3 * xi, MP) - want
+    assert d2 / d1 == pytest.approx(4.0, rel=1e-3)
+
+
+# ---------------------------------------------------------------------------
+# MRegge::PhotoProp, elastic leg (src/MRegge.cc:849-880): the pomeron exchange
+# of vector-meson photoproduction. Its phase is the signature factor's, and
+# |A|^2 falls as exp(B t) with B = B0(m^2) + 2 alpha' log s (the reference's
+# comment: B = B0 + 2 alpha' log(W^2 / W0^2), W0 = 1 GeV), B0 ~ 11 GeV^-2 for the
+# rho and 4.8 for the J/psi, frozen below m = 0.7 GeV.
+# ---------------------------------------------------------------------------
+def photo_prop(cfg, s, t, m2):
+    o = np.zeros(2)
+    ol.oracle().ora_photo_prop(C.byref(cfg), s, t, m2, abi.dptr(o))
+    return complex(o[0], o[1])
+
+
+def eta(cfg, t):
+    o = np.zeros(2)
+    ol.oracle().ora_regge_eta_linear(t, cfg.regge_a0[0], cfg.regge_ap[0], cfg.regge_sgn[0],
+                                     abi.dptr(o))
+    return complex(o[0], o[1])
+
+
+def slope_b(cfg, s, m2, t=-0.2, h=1e-4):
+    def lg(tt):
+        return np.log(abs(photo_prop(cfg, s, tt, m2)) ** 2 / abs(eta(cfg, tt)) ** 2)
+    return (lg(t + h) - lg(t - h)) / (2 * h)
+
+
+def test_photo_prop_phase_is_the_signature_factor():
+    cfg = abi.cfg_default(6)
+    for s, t in [(100.0, -0.1), (5e3, -0.5), (1e5, -1.2)]:
+        a, e = photo_prop(cfg, s, t, 0.775 ** 2), eta(cfg, t)
+        assert np.angle(a) == pytest.approx(np.angle(e), abs=1e-12)
+
+
+def test_photo_prop_slope():
+    cfg = abi.cfg_default(6)
+    ap = cfg.regge_ap[0]
+    s1, s2 = 1e2, 1e4
+    b_rho = slope_b(cfg, s1, 0.775 ** 2) - 2 * ap * np.log(s1)
+    b_jpsi = slope_b(cfg, s1, 3.0969 ** 2) - 2 * ap * np.log(s1)
+    assert 10.5 < b_rho < 12.0
+    assert 4.6 < b_jpsi < 5.0
+    # the energy dependence: B grows by 2 alpha' log(s2 / s1)
+    assert slope_b(cfg, s2, 0.775 ** 2) - slope_b(cfg, s1, 0.775 ** 2) == pytest.approx(
+        2 * ap * np.log(s2 / s1), rel=1e-5)
+    # frozen below m = 0.7 GeV
+    assert slope_b(cfg, s1, 0.1) == pytest.approx(slope_b(cfg, s1, 0.49), rel=1e-12)
