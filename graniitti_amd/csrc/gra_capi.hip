@@ -319,8 +319,15 @@ int validate(const gra_cfg* c) {
   }
   if (c->amp_precision != GRA_AMP_FP64 && c->amp_precision != GRA_AMP_FP32)
     return set_global_error(GRA_E_ARG, "amp_precision %d unknown", c->amp_precision);
-  if (std::fabs(c->beam_E1 - c->beam_E2) > 0.0)
-    return set_global_error(GRA_E_ARG, "asymmetric beams are not supported (E1 != E2)");
+  // the boost to the lab applies when |p_z(beam 1) + p_z(beam 2)| > 1e-6 GeV
+  // (src/MContinuum.cc:386-387)
+  const double pz_sum = msqrt(c->beam_E1 * c->beam_E1 - c->m_beam * c->m_beam) -
+                        msqrt(c->beam_E2 * c->beam_E2 - c->m_beam * c->m_beam);
+  if (std::fabs(pz_sum) > 1e-6 && (c->screening || c->phase_space != GRA_PS_C))
+    return set_global_error(GRA_E_ARG,
+                            "asymmetric beams (E1 != E2): built for the unscreened <C> class (the "
+                            "screening nodes and the <F> kinematics assume the centre-of-mass "
+                            "frame)");
   if (!(c->fwd_pt_max > c->fwd_pt_min) || !(c->kt_max > c->kt_min) || !(c->rap_max > c->rap_min))
     return set_global_error(GRA_E_ARG, "empty generation-cut interval");
   if (c->veto_active) {
@@ -426,6 +433,7 @@ void fill_params(const gra_cfg& c, RunParams& P) {
         (P.beamsum.x * P.beamsum.x + P.beamsum.y * P.beamsum.y + P.beamsum.z * P.beamsum.z);
   P.sqrt_s = msqrt(P.s);
   P.rs = std::sqrt(P.s);
+  P.asym = std::fabs(P.beamsum.z) > 1e-6 ? 1 : 0;
   P.s32 = std::pow(P.s, 3.0 / 2.0);
   P.c_beam1 = (m * m) / (P.pbeam1.e + P.pbeam1.z);
   P.c_beam2 = (m * m) / (P.pbeam2.e - P.pbeam2.z);

@@ -364,6 +364,15 @@ __device__ __forceinline__ bool build_kin(const RunParams& P, const double* x, K
   p1.e = msq(m1 * m1 + pt1 * pt1 + p1z * p1z);
   p2.z = p2z;
   p2.e = msq(m2 * m2 + pt2 * pt2 + p2z * p2z);
+  if (P.asym) {
+    // asymmetric beams: the centre-of-mass momenta boosted to the lab
+    // (src/MContinuum.cc:386-395, kinematics::LorentzBoost with sign +1)
+    p1 = lorentz_boost(P.beamsum, P.sqrt_s, p1);
+    p2 = lorentz_boost(P.beamsum, P.sqrt_s, p2);
+    sumP = lorentz_boost(P.beamsum, P.sqrt_s, sumP);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) p[i] = lorentz_boost(P.beamsum, P.sqrt_s, p[i]);
+  }
   K.pf[1] = p1;
   K.pf[2] = p2;
   K.pf[0] = sumP;

@@ -55,6 +55,9 @@ struct RunParams {
   // src/MFactorized.cc:256-285), B51IntegralVolume's (M_MAX^2 - M_MIN^2)(Y_max - Y_min)
   // and B51PhaseSpaceWeight's (1/2)(2 pi)^-5
   int32_t phase_space, pad_ps;
+  // unequal beam energies: the <C> kinematics are boosted from the centre-of-mass
+  // frame to the lab (|beamsum.z| > 1e-6 GeV, as the reference tests)
+  int32_t asym, pad_asym;
   double f_Mmin, f_Mmax, f_vol, f_psw;
   // Regge amplitude
   int32_t regge_on[3];

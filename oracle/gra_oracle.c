@@ -604,6 +604,14 @@ static int build_kin_c(const ora_run* R, const double* x, kin_t* K) {
   p1.e = msqrt(m1 * m1 + pt1 * pt1 + p1z * p1z);
   p2.z = p2z;
   p2.e = msqrt(m2 * m2 + pt2 * pt2 + p2z * p2z);
+  /* asymmetric beams: the centre-of-mass momenta boosted to the lab
+   * (src/MContinuum.cc:386-395, kinematics::LorentzBoost with sign +1) */
+  if (fabs(R->beamsum.z) > 1e-6) {
+    p1 = lorentz_boost(R->beamsum, R->sqrt_s, p1);
+    p2 = lorentz_boost(R->beamsum, R->sqrt_s, p2);
+    sumP = lorentz_boost(R->beamsum, R->sqrt_s, sumP);
+    for (int i = 0; i < 2; ++i) p[i] = lorentz_boost(R->beamsum, R->sqrt_s, p[i]);
+  }
   K->pf[1] = p1;
   K->pf[2] = p2;
   K->pf[0] = sumP;
