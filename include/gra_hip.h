@@ -119,7 +119,9 @@ typedef struct gra_cfg {
   int32_t amp_precision; /* GRA_AMP_* */
   int32_t dim;      /* hypercube dimension (output of gra_hip_cfg_default) */
 
-  /* Beams (MProcess::SetBeamEnergies, src/MProcess.cc:313-334) */
+  /* Beams (MProcess::SetBeamEnergies, src/MProcess.cc:313-334). Unequal energies
+   * boost the <C> kinematics to the lab (src/MContinuum.cc:386-395); they are
+   * accepted for the unscreened <C> class only. */
   double beam_E1, beam_E2; /* GeV */
   double m_beam;           /* proton mass, modeldata/mass_width_2018.mcd:169 */
 
