@@ -546,7 +546,7 @@ int gra_hip_flow_sample(gra_handle* h, uint64_t n, uint64_t seed, double* w, dou
                         gra_flow_stats* st);
 
 /* The NEURO integrator run (MGraniitti::SampleNeuro integration, src/MGraniitti.cc:
- * 1527-1648) with plain VEGAS as fallback and judge. */
+ * 1527-1648) with plain VEGAS as the fallback and as the cross-check of sigma. */
 typedef struct gra_flow_run_cfg {
   int32_t train_steps; /* Adam steps before sampling (0: sample the flow as it is), at most */
   int32_t check_every; /* training window in steps for the plateau test (0: no test) */
