@@ -322,10 +322,13 @@ def test_flow_backward_finite_differences_large():
     # biases the difference, and FP32 rounding of log q bounds the step from
     # below. A near-identity point keeps log q small (so its rounding is small in
     # absolute terms), and directions along the gradient maximise the signal.
-    # Each layer's output block (whose gradient runs through the spline backward
-    # and every later layer's MLP and input derivative) is held to 1e-3; the
-    # directions that move hidden weights, whose steps cross ReLU kinks, to 1e-2.
-    # The exact comparison at this size is against float64 autograd above.
+    # The full gradient direction and each layer's output block (whose gradient
+    # runs through the spline backward and every later layer's MLP and input
+    # derivative) are held to 1e-3; the directions that move hidden weights,
+    # whose steps cross ReLU kinks, to 1e-2 (measured 1.5e-3 to 3.4e-3). A hidden
+    # direction restricted to units clear of their kink needs a step so small
+    # that the FP32 rounding of log q dominates the difference (4e-3). The exact
+    # comparison at this size is against float64 autograd above (1.2e-4).
     say("case: flow FD check D=14 L=10 K=32 H=256 depth=3")
     with MContinuum(rho_cfg()) as P:
         fc = flow_cfg(14, 10, 32, 256, 3, True)
@@ -372,7 +375,7 @@ def test_flow_backward_finite_differences_large():
     say(f"flow FD check: full gradient {rel_full:.2e}; output blocks "
         f"{' '.join('%.1e' % r for r in rel_out)}; hidden blocks "
         f"{' '.join('%.1e' % r for r in rel_hid)}")
-    assert rel_full < 1e-2
+    assert rel_full < 1e-3
     assert max(rel_out) < 1e-3
     assert max(rel_hid) < 1e-2
 
